@@ -1,0 +1,280 @@
+"""PM force-chain benchmark on one MI355X: KDK steps/s of the full particle-mesh step.
+
+Workloads (seeded synthetic inputs, built on the host, resident in HBM before timing):
+  collision10m  two-galaxy collision, 10M particles, 256^3 grid, box 40, dt 0.01,
+                mixed-precision multigrid to an estimated phi error of 1e-6 (default)
+  collision2m   the same recipe with 2 x 1,048,576 particles on a 128^3 grid, box 40
+  plummer       131,072-particle Plummer sphere, 64^3 grid over a box of 20a, dt 0.01
+
+A step is one fused KDK step (kick+drift, deposit, Poisson solve, -grad, gather+kick).
+Pass 1 times --steps steps with no instrumentation (the reported value). Pass 2 repeats
+them with per-kernel HIP events on the launch stream, giving per-kernel times and the
+roofline figures of the dominant kernel. The CPU baseline runs the reference algorithm
+(OpenMP C port: atomic CIC scatter, plain float32 multigrid tol 1e-3) on the same input.
+
+Prints one JSON line. ``--gpus`` above 1 is refused: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "galaxy-collision-with-cuda-fft_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+REFERENCE_STEPS_PER_S = 24.8  # reference docs/performance.md:17 (RTX 3070, 256^3, 10M)
+
+WORKLOADS = {
+    "collision10m": dict(kind="two_galaxy", n_per=5_000_000, grid=256, box=40.0, dt=0.01,
+                         solver="multigrid_mp"),
+    "collision2m": dict(kind="two_galaxy", n_per=1_048_576, grid=128, box=40.0, dt=0.01,
+                        solver="multigrid"),
+    "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
+}
+
+
+def build_ic(w: dict, seed: int):
+    from galaxy_collision_amd import ic
+
+    n, box = w["grid"], w["box"]
+    dx = box / n
+    if w["kind"] == "plummer":
+        c = (n // 2) * dx
+        icr = ic.plummer_ic(w["n"], total_mass=1.0, scale_a=1.0, centre=(c, c, c), seed=seed)
+    else:
+        c = (n - 1) * dx / 2.0
+        icr = ic.two_galaxy_ic(w["n_per"], box_centre=(c, c, c), separation=20.0, seed=seed)
+    return ic.sort_by_cell(icr, n, dx), dx
+
+
+def _dst_exact(f, boundary, dx):
+    """Exact float64 solution of the 7-point Dirichlet problem (scipy DST-I)."""
+    import numpy as np
+    from scipy.fft import dstn, idstn
+
+    n = f.shape[0]
+    b = f[1:-1, 1:-1, 1:-1].astype(np.float64) * dx * dx
+    bd = boundary.astype(np.float64).copy()
+    bd[1:-1, 1:-1, 1:-1] = 0.0
+    b -= (bd[2:, 1:-1, 1:-1] + bd[:-2, 1:-1, 1:-1] + bd[1:-1, 2:, 1:-1] + bd[1:-1, :-2, 1:-1]
+          + bd[1:-1, 1:-1, 2:] + bd[1:-1, 1:-1, :-2])
+    m = n - 2
+    lam1 = 2.0 * (np.cos(np.pi * np.arange(1, m + 1) / (m + 1)) - 1.0)
+    lam = lam1[:, None, None] + lam1[None, :, None] + lam1[None, None, :]
+    u = idstn(dstn(b, type=1, workers=-1) / lam, type=1, workers=-1)
+    out = boundary.astype(np.float64).copy()
+    out[1:-1, 1:-1, 1:-1] = u
+    return out
+
+
+def phi_accuracy(chain, grav: float) -> dict:
+    """phi of the last step against (a) the exact solution of the same discrete problem and
+    (b) the zero-padded continuum-kernel FFT solve (float64, on the device)."""
+    import numpy as np
+    import torch
+
+    from galaxy_collision_amd.solver.fft import FFTPoissonSolver
+
+    rho = chain.grid.rho.to_numpy()
+    phi = chain.grid.phi.to_numpy().astype(np.float64)
+    f = (np.float32(4.0 * math.pi * grav) * rho).astype(np.float32)
+    exact = _dst_exact(f, phi, chain.dx)
+    alg = float(np.linalg.norm(phi - exact) / np.linalg.norm(exact))
+    fft = FFTPoissonSolver(chain.n, dx=chain.dx, grav_constant=grav)
+    out = torch.empty_like(chain.grid.phi.tensor)
+    fft.solve(chain.grid.rho, out)
+    ref = out.cpu().numpy().astype(np.float64)
+    del fft, out
+    torch.cuda.empty_cache()
+    tot = float(np.linalg.norm(phi - ref) / np.linalg.norm(ref))
+    return {"phi_rel_l2_vs_exact_discrete": alg, "phi_rel_l2_vs_fft_continuum": tot}
+
+
+def cpu_baseline(icr, w: dict, dx: float, budget_s: float, threads: int | None) -> dict:
+    """The reference algorithm on host cores: prime once, then time whole steps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pm_oracle
+
+    chain = pm_oracle.CpuChain(icr.pos, icr.vel, icr.mass, w["grid"], dx, 1.0, threads=threads)
+    t0 = time.perf_counter()
+    chain.accel()
+    prime_s = time.perf_counter() - t0
+    steps, elapsed = 0, 0.0
+    while steps < 1 or (elapsed < budget_s and steps < 10):
+        t = time.perf_counter()
+        chain.step(w["dt"])
+        elapsed += time.perf_counter() - t
+        steps += 1
+    chain.close()
+    return {"value": steps / elapsed, "unit": "steps/s", "cores": chain.threads, "kind": "port",
+            "sample": (f"{icr.n:,} particles on {w['grid']}^3, reference algorithm (OpenMP C "
+                       f"port, atomic CIC, plain float32 multigrid tol 1e-3): {steps} KDK steps "
+                       f"timed after a cold prime of {prime_s:.1f} s")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="collision10m", choices=sorted(WORKLOADS))
+    ap.add_argument("--solver", default=None, help="override the workload's solver")
+    ap.add_argument("--tol", type=float, default=None, help="solver tolerance override")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of timed CPU steps")
+    ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the phi accuracy check")
+    ap.add_argument("--no-kernel-pass", action="store_true", help="skip the per-kernel pass")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported in this "
+              "build (the multi-GPU slab decomposition is a design note, not code).",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from galaxy_collision_amd import _native as nat
+    from galaxy_collision_amd.chain import PMChain
+    from galaxy_collision_amd.data import ParticleState, estimate_memory
+    from galaxy_collision_amd.ic import box_fit_fraction
+    from galaxy_collision_amd.solver import make_solver
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+    w = dict(WORKLOADS[args.workload])
+    solver_name = args.solver or w["solver"]
+    grav = 1.0
+    t_ic = time.perf_counter()
+    icr, dx = build_ic(w, args.seed)
+    ic_s = time.perf_counter() - t_ic
+    outside = box_fit_fraction(icr, w["grid"], dx)
+    print(f"[bench] workload={args.workload} N={icr.n:,} grid={w['grid']}^3 dx={dx:.5f} "
+          f"solver={solver_name} ic={ic_s:.1f}s outside={outside:.2%}", file=sys.stderr, flush=True)
+
+    parts = ParticleState(icr.n)
+    parts.load(icr.pos, icr.vel, icr.mass, icr.gid)
+    kw = {"dx": dx, "grav_constant": grav}
+    if args.tol is not None:
+        kw["tol"] = args.tol
+    solver = make_solver(solver_name, w["grid"], **kw)
+    chain = PMChain(parts, w["grid"], dx, solver)
+    chain.prime()
+    for _ in range(args.warmup):
+        chain.step(w["dt"])
+    torch.cuda.synchronize()
+
+    # pass 1: throughput
+    chain.last_cycles.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        chain.step(w["dt"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cycles = list(chain.last_cycles)
+    ms_per_step = wall / args.steps * 1e3
+    value = args.steps / wall
+    print(f"[bench] pass 1: {ms_per_step:.3f} ms/step, V-cycles/step {np.mean(cycles):.2f}",
+          file=sys.stderr, flush=True)
+
+    # pass 2: per-kernel HIP events (same steps, instrumented)
+    kernels, roofline, pass2_ms = {}, None, None
+    if not args.no_kernel_pass:
+        nat.timing_reset()
+        nat.timing_enable(True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            chain.step(w["dt"])
+        torch.cuda.synchronize()
+        pass2_ms = (time.perf_counter() - t1) / args.steps * 1e3
+        nat.timing_enable(False)
+        rep = nat.timing_report()
+        for name, r in rep.items():
+            avg_ms = r["total_ms"] / r["launches"]
+            kernels[name] = {
+                "ms_per_step": r["total_ms"] / args.steps,
+                "launches_per_step": r["launches"] / args.steps,
+                "avg_launch_us": avg_ms * 1e3,
+                "GBps": (r["bytes"] / r["launches"]) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
+            }
+        dom = max(rep.items(), key=lambda kv: kv[1]["total_ms"])
+        name, r = dom
+        avg_s = r["total_ms"] / r["launches"] * 1e-3
+        per_launch = r["bytes"] / r["launches"]
+        achieved = per_launch / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": avg_s * 1e6}
+        print(f"[bench] pass 2: {pass2_ms:.3f} ms/step; dominant {name} {achieved:.0f} GB/s",
+              file=sys.stderr, flush=True)
+
+    acc = {}
+    if not args.no_accuracy:
+        acc = phi_accuracy(chain, grav)
+        print(f"[bench] accuracy {acc}", file=sys.stderr, flush=True)
+    stats = getattr(solver, "last_stats", None) or {}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(icr, w, dx, args.cpu_budget, args.cpu_threads)
+        print(f"[bench] CPU baseline {cpu['value']:.4f} steps/s on {cpu['cores']} threads",
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "PM steps/s @256^3,10M particles" if args.workload == "collision10m"
+        else f"PM steps/s ({args.workload})",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REFERENCE_STEPS_PER_S if args.workload == "collision10m" else None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "particles": icr.n,
+            "grid": w["grid"],
+            "box": w["box"],
+            "dt": w["dt"],
+            "solver": solver_name,
+            "solver_tol": getattr(solver, "tol", None),
+            "precision": "f32 particles/V-cycles; f64 residual, coarsest solve, reductions"
+            if solver_name == "multigrid_mp" else "f32",
+            "parallelism": "single GPU",
+            "particle_order": "sorted by initial cell",
+        },
+        "particle_steps_per_s": value * icr.n,
+        "mean_vcycles_per_step": float(np.mean(cycles)) if cycles else None,
+        "solver_last_stats": stats,
+        "ms_per_step_instrumented": pass2_ms,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": dict(sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"])[:16]),
+        "mem_estimate_gb": estimate_memory(icr.n, w["grid"]).total_gb,
+        **acc,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,86 @@
+// Shared helpers for the pmchain HIP sources: error state, launch checks, grid math.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "pmchain.h"
+
+namespace pm {
+
+// Thread-local last-error message (read back through pm_last_error()).
+void set_error(const std::string& msg);
+void clear_error();
+
+inline pm_status invalid(const std::string& msg) {
+  set_error(msg);
+  return PM_ERR_INVALID;
+}
+
+inline pm_status hip_fail(const char* what, hipError_t e) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? PM_ERR_NOMEM : PM_ERR_HIP;
+}
+
+constexpr int kBlock = 256;  // threads per workgroup for the streaming kernels
+
+inline unsigned blocks_for(int64_t work, int per_block = kBlock) {
+  return static_cast<unsigned>((work + per_block - 1) / per_block);
+}
+
+// Grid-stride launch size for streaming kernels: enough workgroups to fill 256 CUs
+// several times over without one workgroup per element for very large arrays.
+inline unsigned stream_blocks(int64_t work, int per_block = kBlock) {
+  int64_t b = (work + per_block - 1) / per_block;
+  const int64_t cap = 256 * 64;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return static_cast<unsigned>(b);
+}
+
+// Correctly rounded float32 square root. The hardware estimate (v_sqrt_f32) is within one
+// ulp; each candidate is then checked against the exact midpoints to its neighbours: a
+// midpoint of two floats has at most 25 significant bits and its square at most 50, so both
+// are exact in float64 and the comparison with x decides the rounding exactly (no ties are
+// possible). Two correction rounds cover an estimate up to two ulps off.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  float s = __builtin_sqrtf(x);
+  if (!(x > 0.0f) || !(x < __builtin_huge_valf())) return s;
+  const double xd = static_cast<double>(x);
+#pragma unroll
+  for (int round = 0; round < 2; ++round) {
+    const float lo = __int_as_float(__float_as_int(s) - 1);
+    const double mlo = (static_cast<double>(lo) + static_cast<double>(s)) * 0.5;
+    if (xd < mlo * mlo) {
+      s = lo;
+      continue;
+    }
+    const float hi = __int_as_float(__float_as_int(s) + 1);
+    const double mhi = (static_cast<double>(s) + static_cast<double>(hi)) * 0.5;
+    if (xd > mhi * mhi) s = hi;
+  }
+  return s;
+}
+
+}  // namespace pm
+
+#define PM_TRY_HIP(expr)                                   \
+  do {                                                     \
+    hipError_t pm_e_ = (expr);                             \
+    if (pm_e_ != hipSuccess) return pm::hip_fail(#expr, pm_e_); \
+  } while (0)
+
+// Check the launch that was just enqueued.
+#define PM_CHECK_LAUNCH(name)                              \
+  do {                                                     \
+    hipError_t pm_e_ = hipGetLastError();                  \
+    if (pm_e_ != hipSuccess) return pm::hip_fail(name, pm_e_); \
+  } while (0)
+
+#define PM_REQUIRE(cond, msg)                              \
+  do {                                                     \
+    if (!(cond)) return pm::invalid(msg);                  \
+  } while (0)

@@ -1,0 +1,634 @@
+// Deterministic CIC deposit: integer cell keys -> stable LSD radix sort -> segmented
+// reduction of per-cell corner sums -> node assembly. Replaces the float-atomic scatter of
+// reference deposit.py:47-82 `_deposit_cic`.
+//
+// Summation order (the contract the CPU oracle restates):
+//   1. A particle's contribution to corner c of its cell is ((m*wx)*wy)*wz with
+//      m = mass * (1/((dx*dx)*dx)) (the reference's `m * wx * wy * wz`).
+//   2. Particles are ordered by a stable sort on the extended cell key.
+//   3. The sorted array is cut into aligned chunks of kChunk positions. A cell's particles
+//      that fall into one chunk form a "piece", summed sequentially in sorted order.
+//   4. A cell's pieces (in order) are summed by the aligned pairwise tree: at level s,
+//      element l (a multiple of 2s) absorbs element l+s when it exists; odd leftovers
+//      pass up unchanged.
+//   5. Node value = 0 + sum over corners c = 0..7 (bits cx,cy,cz) of the corner-c sum of
+//      cell (i-cx, j-cy, k-cz), skipping absent cells, in that order.
+#include <algorithm>
+
+#include "pm_cic.h"
+#include "pm_common.h"
+
+namespace {
+
+constexpr int kChunk = 16;           // positions per piece chunk (one thread each)
+constexpr int kSortWaves = 4;        // waves per radix workgroup
+constexpr int kSortItems = 16;       // keys per lane
+constexpr int kTile = kSortWaves * 64 * kSortItems;  // 4096 keys per radix tile
+constexpr int kMaxDigitBits = 8;
+constexpr int kSmallPieces = 8;      // open cells with <= this many pieces finish in one thread
+constexpr int kBigBatch = 256;       // pieces per LDS batch for large cells
+
+struct BigCell {
+  int64_t slot;   // sorted position of the cell's first particle
+  int64_t chunk;  // chunk holding that first particle
+  int64_t pieces; // number of pieces
+};
+
+// ---------------------------------------------------------------------------------------
+// 1. keys and per-particle records
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) key_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float* __restrict__ mass, int64_t np, int n, float dx, float inv_vol,
+    uint32_t sentinel, uint32_t* __restrict__ keys, int32_t* __restrict__ vals,
+    float4* __restrict__ rec, uint32_t* __restrict__ keys_out) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  int ix = 0, iy = 0, iz = 0;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  const bool in = pm::cic_axis(px[p], dx, n, ix, fx) && pm::cic_axis(py[p], dx, n, iy, fy) &&
+                  pm::cic_axis(pz[p], dx, n, iz, fz);
+  const uint32_t key = in ? pm::cell_key(ix, iy, iz, n) : sentinel;
+  keys[p] = key;
+  vals[p] = static_cast<int32_t>(p);
+  rec[p] = make_float4(fx, fy, fz, mass[p] * inv_vol);
+  if (keys_out) keys_out[p] = key;
+}
+
+// ---------------------------------------------------------------------------------------
+// 2. stable LSD radix sort (reduce-then-scan; no inter-workgroup waiting)
+// ---------------------------------------------------------------------------------------
+// Per-tile digit histogram, stored digit-major: hist[d * tiles + tile].
+__global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restrict__ keys,
+                                                         int64_t np, int tiles, int shift,
+                                                         int bits, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[1 << kMaxDigitBits];
+  const int buckets = 1 << bits;
+  const uint32_t mask = static_cast<uint32_t>(buckets - 1);
+  for (int d = threadIdx.x; d < buckets; d += blockDim.x) cnt[d] = 0;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
+  for (int e = 0; e < kTile / 256; ++e) {
+    const int64_t p = base + e * 256 + threadIdx.x;
+    if (p < np) atomicAdd(&cnt[(keys[p] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < buckets; d += blockDim.x)
+    hist[static_cast<int64_t>(d) * tiles + blockIdx.x] = cnt[d];
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads = 4 waves).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total,
+                                                         uint32_t* s_wave) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t wave_off = 0;
+  total = 0;
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t s = s_wave[w];
+    if (w < wave) wave_off += s;
+    total += s;
+  }
+  __syncthreads();
+  return wave_off + incl - v;
+}
+
+// One workgroup per digit: exclusive scan of hist[d][*] in place; digit_tot[d] = total.
+__global__ void __launch_bounds__(256) radix_scan_kernel(uint32_t* __restrict__ hist, int tiles,
+                                                         uint32_t* __restrict__ digit_tot) {
+  __shared__ uint32_t s_wave[4];
+  uint32_t* row = hist + static_cast<int64_t>(blockIdx.x) * tiles;
+  uint32_t carry = 0;
+  for (int base = 0; base < tiles; base += 256 * 4) {
+    uint32_t v[4];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = base + threadIdx.x * 4 + q;
+      v[q] = t < tiles ? row[t] : 0u;
+      sum += v[q];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, total, s_wave) + carry;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = base + threadIdx.x * 4 + q;
+      if (t < tiles) row[t] = run;
+      run += v[q];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) digit_tot[blockIdx.x] = carry;
+}
+
+// digit_tot -> exclusive prefix (digit_start), one workgroup.
+__global__ void __launch_bounds__(256) radix_digit_start_kernel(uint32_t* __restrict__ digit_tot,
+                                                                int buckets) {
+  __shared__ uint32_t s_wave[4];
+  const uint32_t v = threadIdx.x < static_cast<unsigned>(buckets) ? digit_tot[threadIdx.x] : 0u;
+  uint32_t total;
+  const uint32_t ex = block_exclusive_scan(v, total, s_wave);
+  if (threadIdx.x < static_cast<unsigned>(buckets)) digit_tot[threadIdx.x] = ex;
+}
+
+// Stable scatter of one tile. Lane l of wave w, round r holds tile position
+// w*64*kSortItems + r*64 + l, so (wave, round, lane) order is input order.
+__global__ void __launch_bounds__(256) radix_scatter_kernel(
+    const uint32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_start, int64_t np,
+    int tiles, int shift, int bits) {
+  __shared__ uint32_t wave_cnt[kSortWaves][1 << kMaxDigitBits];
+  __shared__ uint32_t tile_base[1 << kMaxDigitBits];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int buckets = 1 << bits;
+  const uint32_t mask = static_cast<uint32_t>(buckets - 1);
+  for (int d = threadIdx.x; d < kSortWaves * (1 << kMaxDigitBits); d += blockDim.x)
+    (&wave_cnt[0][0])[d] = 0;
+  for (int d = threadIdx.x; d < buckets; d += blockDim.x)
+    tile_base[d] = digit_start[d] + hist[static_cast<int64_t>(d) * tiles + blockIdx.x];
+  __syncthreads();
+
+  const uint64_t lanes_below = (1ull << lane) - 1ull;
+  const int64_t wave_start =
+      static_cast<int64_t>(blockIdx.x) * kTile + static_cast<int64_t>(wave) * 64 * kSortItems;
+  uint32_t k[kSortItems];
+  int32_t v[kSortItems];
+  uint32_t rank[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t p = wave_start + r * 64 + lane;
+    const bool valid = p < np;
+    const uint32_t key = valid ? keys_in[p] : 0u;
+    k[r] = key;
+    v[r] = valid ? vals_in[p] : 0;
+    const uint32_t d = (key >> shift) & mask;
+    uint64_t match = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bm = __ballot(bit);
+      match &= bit ? bm : ~bm;
+    }
+    const uint32_t before = static_cast<uint32_t>(__popcll(match & lanes_below));
+    const uint32_t base = wave_cnt[wave][d];
+    rank[r] = base + before;
+    if (valid && before == 0) wave_cnt[wave][d] = base + static_cast<uint32_t>(__popcll(match));
+  }
+  __syncthreads();
+  // exclusive prefix of the per-wave digit counts, in wave order
+  for (int d = threadIdx.x; d < buckets; d += blockDim.x) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+      const uint32_t c = wave_cnt[w][d];
+      wave_cnt[w][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t p = wave_start + r * 64 + lane;
+    if (p < np) {
+      const uint32_t d = (k[r] >> shift) & mask;
+      const int64_t dst = static_cast<int64_t>(tile_base[d]) + wave_cnt[wave][d] + rank[r];
+      keys_out[dst] = k[r];
+      vals_out[dst] = v[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// 3. segmented reduction
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void corner_terms(const float4 r, float t[8]) {
+  const float wx0 = 1.0f - r.x, wy0 = 1.0f - r.y, wz0 = 1.0f - r.z;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float wx = (c & 4) ? r.x : wx0;
+    const float wy = (c & 2) ? r.y : wy0;
+    const float wz = (c & 1) ? r.z : wz0;
+    t[c] = r.w * wx * wy * wz;
+  }
+}
+
+// One thread per chunk: sequential piece sums. Complete cells are written straight to
+// sums/map; a chunk's continuation piece goes to head[chunk], its open last piece to
+// tail[chunk] (with tail_slot = sorted position of that cell's first particle).
+__global__ void __launch_bounds__(256) piece_kernel(
+    const uint32_t* __restrict__ skey, const int32_t* __restrict__ sidx,
+    const float4* __restrict__ rec, int64_t np, uint32_t sentinel, int64_t chunks,
+    float* __restrict__ sums, int64_t cap, int32_t* __restrict__ map, float* __restrict__ head,
+    float* __restrict__ tail, int64_t* __restrict__ tail_slot) {
+  const int64_t ch = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ch >= chunks) return;
+  const int64_t s = ch * kChunk;
+  const int64_t e = s + kChunk < np ? s + kChunk : np;
+  const uint32_t prev = s > 0 ? skey[s - 1] : 0xFFFFFFFFu;
+  int64_t open_slot = -1;
+  int64_t i = s;
+  while (i < e) {
+    const uint32_t key = skey[i];
+    if (key >= sentinel) break;
+    float acc[8];
+    corner_terms(rec[sidx[i]], acc);
+    int64_t j = i + 1;
+    while (j < e && skey[j] == key) {
+      float t[8];
+      corner_terms(rec[sidx[j]], t);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = acc[c] + t[c];
+      ++j;
+    }
+    const bool starts = (i > s) || (prev != key);
+    const bool ends = (j < e) || (j >= np) || (skey[j] != key);
+    if (starts && ends) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) sums[c * cap + i] = acc[c];
+      map[key] = static_cast<int32_t>(i);
+    } else if (starts) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) tail[ch * 8 + c] = acc[c];
+      open_slot = i;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) head[ch * 8 + c] = acc[c];
+    }
+    i = j;
+  }
+  tail_slot[ch] = open_slot;
+}
+
+// First sorted position after `from` whose key differs from `key` (keys are sorted).
+__device__ __forceinline__ int64_t cell_end(const uint32_t* __restrict__ skey, int64_t from,
+                                            int64_t np, uint32_t key) {
+  int64_t lo = from, hi = np;  // invariant: skey[lo] == key, answer in (lo, hi]
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (skey[mid] == key) lo = mid; else hi = mid;
+  }
+  return hi;
+}
+
+// Cells whose pieces span chunks: small ones (<= kSmallPieces pieces) finish here with an
+// unrolled aligned pairwise tree; larger ones are listed for big_cell_kernel.
+__global__ void __launch_bounds__(256) join_kernel(
+    const uint32_t* __restrict__ skey, int64_t np, int64_t chunks,
+    const int64_t* __restrict__ tail_slot, const float* __restrict__ head,
+    const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
+    int32_t* __restrict__ map, BigCell* __restrict__ big, uint32_t* __restrict__ big_count) {
+  const int64_t ch = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ch >= chunks) return;
+  const int64_t slot = tail_slot[ch];
+  if (slot < 0) return;
+  const uint32_t key = skey[slot];
+  const int64_t end = cell_end(skey, slot, np, key);
+  const int64_t last = (end - 1) / kChunk;
+  const int64_t pieces = last - ch + 1;
+  if (pieces > kSmallPieces) {
+    const uint32_t at = atomicAdd(big_count, 1u);
+    big[at] = BigCell{slot, ch, pieces};
+    return;
+  }
+  float p[kSmallPieces][8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) p[0][c] = tail[ch * 8 + c];
+#pragma unroll
+  for (int q = 1; q < kSmallPieces; ++q) {
+    if (q < pieces) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) p[q][c] = head[(ch + q) * 8 + c];
+    }
+  }
+#pragma unroll
+  for (int s = 1; s < kSmallPieces; s <<= 1) {
+#pragma unroll
+    for (int l = 0; l + s < kSmallPieces; l += 2 * s) {
+      if (l + s < pieces) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) p[l][c] = p[l][c] + p[l + s][c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) sums[c * cap + slot] = p[0][c];
+  map[key] = static_cast<int32_t>(slot);
+}
+
+// One workgroup per large cell: aligned pairwise tree over batches of kBigBatch pieces in
+// LDS; batch results are merged by a binary-counter cascade (push, merge while the two
+// top partials cover equal piece counts) and a final right-nested combine, which is the
+// same aligned tree one level up. All control flow is uniform across the workgroup.
+__global__ void __launch_bounds__(256) big_cell_kernel(
+    const uint32_t* __restrict__ skey, const BigCell* __restrict__ big,
+    const uint32_t* __restrict__ big_count, const float* __restrict__ head,
+    const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
+    int32_t* __restrict__ map) {
+  constexpr int kMaxDepth = 40;
+  __shared__ float buf[kBigBatch][9];  // +1 pad against bank conflicts
+  __shared__ float stack[kMaxDepth][8];
+  __shared__ int64_t stack_size[kMaxDepth];
+  const int tid = threadIdx.x;
+  const uint32_t count = *big_count;
+  for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
+    const BigCell cell = big[b];
+    int depth = 0;
+    for (int64_t base = 0; base < cell.pieces; base += kBigBatch) {
+      const int64_t m = cell.pieces - base < kBigBatch ? cell.pieces - base : kBigBatch;
+      // load pieces [base, base+m): piece 0 is the tail of the cell's first chunk
+      for (int t = tid; t < kBigBatch * 8; t += blockDim.x) {
+        const int q = t >> 3, c = t & 7;
+        float v = 0.0f;
+        if (q < m) {
+          const int64_t g = base + q;
+          v = g == 0 ? tail[cell.chunk * 8 + c] : head[(cell.chunk + g) * 8 + c];
+        }
+        buf[q][c] = v;
+      }
+      __syncthreads();
+      for (int s = 1; s < kBigBatch; s <<= 1) {
+        for (int t = tid; t < kBigBatch * 8; t += blockDim.x) {
+          const int l = t >> 3, c = t & 7;
+          if ((l % (2 * s)) == 0 && l + s < m) buf[l][c] = buf[l][c] + buf[l + s][c];
+        }
+        __syncthreads();
+      }
+      int d = depth;
+      int64_t sz = m;
+      float val = tid < 8 ? buf[0][tid] : 0.0f;
+      while (d > 0 && stack_size[d - 1] == sz) {
+        if (tid < 8) val = stack[d - 1][tid] + val;
+        sz *= 2;
+        --d;
+      }
+      if (tid < 8) stack[d][tid] = val;
+      __syncthreads();  // every thread has read stack_size
+      if (tid == 0) stack_size[d] = sz;
+      depth = d + 1;
+      __syncthreads();
+    }
+    if (tid < 8) {
+      float acc = stack[depth - 1][tid];
+      for (int d = depth - 2; d >= 0; --d) acc = stack[d][tid] + acc;
+      sums[tid * cap + cell.slot] = acc;
+    }
+    if (tid == 0) map[skey[cell.slot]] = static_cast<int32_t>(cell.slot);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// 4. node assembly
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ map,
+                                                       const float* __restrict__ sums,
+                                                       int64_t cap, int n,
+                                                       float* __restrict__ rho) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n2 * n) return;
+  const int k = static_cast<int>(idx % n);
+  const int j = static_cast<int>((idx / n) % n);
+  const int i = static_cast<int>(idx / n2);
+  const int64_t m = n + 1;
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int ex = i + 1 - (c >> 2);
+    const int ey = j + 1 - ((c >> 1) & 1);
+    const int ez = k + 1 - (c & 1);
+    const int32_t slot = map[(ex * m + ey) * m + ez];
+    if (slot >= 0) acc = acc + sums[c * cap + slot];
+  }
+  rho[idx] = acc;
+}
+
+__global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* __restrict__ p, int64_t count,
+                                                       int32_t value) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = value;
+}
+
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
+}  // namespace
+
+struct pm_deposit_plan {
+  int n = 0;
+  int64_t cap = 0;
+  uint32_t sentinel = 0;
+  int passes = 0;
+  int digit_bits[4] = {0, 0, 0, 0};
+  uint32_t* keys[2] = {nullptr, nullptr};
+  int32_t* vals[2] = {nullptr, nullptr};
+  float4* rec = nullptr;
+  uint32_t* hist = nullptr;
+  uint32_t* digit = nullptr;
+  float* sums = nullptr;
+  float* head = nullptr;
+  float* tail = nullptr;
+  int64_t* tail_slot = nullptr;
+  int32_t* map = nullptr;
+  BigCell* big = nullptr;
+  uint32_t* big_count = nullptr;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  float last_ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+void free_plan(pm_deposit_plan* p) {
+  if (!p) return;
+  void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->rec, p->hist, p->digit,
+                  p->sums, p->head, p->tail, p->tail_slot, p->map, p->big, p->big_count};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto& e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete p;
+}
+
+template <typename T>
+pm_status alloc(T** ptr, int64_t count, const char* what) {
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), sizeof(T) * (count > 0 ? count : 1));
+  if (e != hipSuccess) return pm::hip_fail(what, e);
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) {
+  PM_REQUIRE(out, "pm_deposit_plan_create: null output");
+  *out = nullptr;
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_deposit_plan_create: grid_size must be in [1, 1024]");
+  PM_REQUIRE(cap >= 1 && cap < (int64_t(1) << 31),
+             "pm_deposit_plan_create: max_particles must be in [1, 2^31)");
+  auto* p = new pm_deposit_plan();
+  p->n = n;
+  p->cap = cap;
+  const int64_t m = static_cast<int64_t>(n) + 1;
+  p->sentinel = static_cast<uint32_t>(m * m * m);
+  int key_bits = 1;
+  while ((int64_t(1) << key_bits) <= static_cast<int64_t>(p->sentinel)) ++key_bits;
+  p->passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  for (int s = 0; s < p->passes; ++s)
+    p->digit_bits[s] = key_bits / p->passes + (s < key_bits % p->passes ? 1 : 0);
+  const int64_t tiles = (cap + kTile - 1) / kTile;
+  const int64_t chunks = (cap + kChunk - 1) / kChunk;
+  pm_status st = PM_OK;
+  if ((st = alloc(&p->keys[0], cap, "hipMalloc keys")) ||
+      (st = alloc(&p->keys[1], cap, "hipMalloc keys")) ||
+      (st = alloc(&p->vals[0], cap, "hipMalloc vals")) ||
+      (st = alloc(&p->vals[1], cap, "hipMalloc vals")) ||
+      (st = alloc(&p->rec, cap, "hipMalloc rec")) ||
+      (st = alloc(&p->hist, tiles << kMaxDigitBits, "hipMalloc hist")) ||
+      (st = alloc(&p->digit, int64_t(1) << kMaxDigitBits, "hipMalloc digit")) ||
+      (st = alloc(&p->sums, 8 * cap, "hipMalloc sums")) ||
+      (st = alloc(&p->head, 8 * chunks, "hipMalloc head")) ||
+      (st = alloc(&p->tail, 8 * chunks, "hipMalloc tail")) ||
+      (st = alloc(&p->tail_slot, chunks, "hipMalloc tail_slot")) ||
+      (st = alloc(&p->map, m * m * m, "hipMalloc map")) ||
+      (st = alloc(&p->big, chunks, "hipMalloc big")) ||
+      (st = alloc(&p->big_count, 1, "hipMalloc big_count"))) {
+    free_plan(p);
+    return st;
+  }
+  for (auto& e : p->ev) {
+    hipError_t he = hipEventCreate(&e);
+    if (he != hipSuccess) {
+      free_plan(p);
+      return pm::hip_fail("hipEventCreate", he);
+    }
+  }
+  *out = p;
+  return PM_OK;
+}
+
+pm_status pm_deposit_plan_destroy(pm_deposit_plan* plan) {
+  free_plan(plan);
+  return PM_OK;
+}
+
+pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable) {
+  PM_REQUIRE(plan, "pm_deposit_plan_set_timing: null plan");
+  plan->timing = enable != 0;
+  return PM_OK;
+}
+
+pm_status pm_deposit_last_timing(pm_deposit_plan* plan, float out_ms[4]) {
+  PM_REQUIRE(plan && out_ms, "pm_deposit_last_timing: null pointer");
+  if (plan->timing) {
+    PM_TRY_HIP(hipEventSynchronize(plan->ev[4]));
+    for (int q = 0; q < 4; ++q)
+      PM_TRY_HIP(hipEventElapsedTime(&plan->last_ms[q], plan->ev[q], plan->ev[q + 1]));
+  }
+  for (int q = 0; q < 4; ++q) out_ms[q] = plan->last_ms[q];
+  return PM_OK;
+}
+
+pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py,
+                         const float* pz, const float* mass, int64_t np, float dx, float* rho,
+                         uint32_t* keys_out, int32_t* perm_out, hipStream_t stream) {
+  PM_REQUIRE(plan, "pm_deposit_cic: null plan");
+  PM_REQUIRE(rho, "pm_deposit_cic: null rho");
+  PM_REQUIRE(np >= 0, "pm_deposit_cic: negative particle count");
+  PM_REQUIRE(dx > 0.0f, "pm_deposit_cic: dx must be positive");
+  if (np > plan->cap) {
+    pm::set_error("pm_deposit_cic: " + std::to_string(np) + " particles exceed the plan capacity " +
+                  std::to_string(plan->cap));
+    return PM_ERR_CAPACITY;
+  }
+  PM_REQUIRE(np == 0 || (px && py && pz && mass), "pm_deposit_cic: null particle array");
+  const int n = plan->n;
+  const int64_t nodes = static_cast<int64_t>(n) * n * n;
+  const int64_t m = static_cast<int64_t>(n) + 1;
+  const int64_t cells = m * m * m;
+  const bool timing = plan->timing;
+  if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[0], stream));
+
+  fill_i32_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(plan->map, cells, -1);
+  PM_CHECK_LAUNCH("fill_i32_kernel");
+  if (np == 0) {
+    if (timing) {
+      for (int q = 1; q < 5; ++q) PM_TRY_HIP(hipEventRecord(plan->ev[q], stream));
+    }
+    assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
+                                                                       plan->cap, n, rho);
+    PM_CHECK_LAUNCH("assemble_kernel");
+    return PM_OK;
+  }
+
+  // 1. keys
+  const float inv_vol = 1.0f / ((dx * dx) * dx);
+  key_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+      px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel, plan->keys[0], plan->vals[0],
+      plan->rec, keys_out);
+  PM_CHECK_LAUNCH("key_kernel");
+  if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[1], stream));
+
+  // 2. sort
+  const int tiles = static_cast<int>((np + kTile - 1) / kTile);
+  int src = 0, shift = 0;
+  for (int s = 0; s < plan->passes; ++s) {
+    const int bits = plan->digit_bits[s];
+    const int buckets = 1 << bits;
+    radix_hist_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], np, tiles, shift, bits,
+                                                  plan->hist);
+    PM_CHECK_LAUNCH("radix_hist_kernel");
+    radix_scan_kernel<<<buckets, 256, 0, stream>>>(plan->hist, tiles, plan->digit);
+    PM_CHECK_LAUNCH("radix_scan_kernel");
+    radix_digit_start_kernel<<<1, 256, 0, stream>>>(plan->digit, buckets);
+    PM_CHECK_LAUNCH("radix_digit_start_kernel");
+    radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], plan->vals[src],
+                                                     plan->keys[1 - src], plan->vals[1 - src],
+                                                     plan->hist, plan->digit, np, tiles, shift,
+                                                     bits);
+    PM_CHECK_LAUNCH("radix_scatter_kernel");
+    src = 1 - src;
+    shift += bits;
+  }
+  const uint32_t* skey = plan->keys[src];
+  const int32_t* sidx = plan->vals[src];
+  if (perm_out)
+    PM_TRY_HIP(hipMemcpyAsync(perm_out, sidx, sizeof(int32_t) * np, hipMemcpyDeviceToDevice,
+                              stream));
+  if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[2], stream));
+
+  // 3. segmented reduction
+  const int64_t chunks = (np + kChunk - 1) / kChunk;
+  piece_kernel<<<pm::blocks_for(chunks), pm::kBlock, 0, stream>>>(
+      skey, sidx, plan->rec, np, plan->sentinel, chunks, plan->sums, plan->cap, plan->map,
+      plan->head, plan->tail, plan->tail_slot);
+  PM_CHECK_LAUNCH("piece_kernel");
+  zero_u32_kernel<<<1, 1, 0, stream>>>(plan->big_count);
+  PM_CHECK_LAUNCH("zero_u32_kernel");
+  join_kernel<<<pm::blocks_for(chunks), pm::kBlock, 0, stream>>>(
+      skey, np, chunks, plan->tail_slot, plan->head, plan->tail, plan->sums, plan->cap,
+      plan->map, plan->big, plan->big_count);
+  PM_CHECK_LAUNCH("join_kernel");
+  big_cell_kernel<<<256, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
+                                           plan->tail, plan->sums, plan->cap, plan->map);
+  PM_CHECK_LAUNCH("big_cell_kernel");
+  if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[3], stream));
+
+  // 4. nodes
+  assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
+                                                                     plan->cap, n, rho);
+  PM_CHECK_LAUNCH("assemble_kernel");
+  if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[4], stream));
+  return PM_OK;
+}
+
+}  // extern "C"

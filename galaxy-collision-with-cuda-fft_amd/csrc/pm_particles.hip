@@ -1,0 +1,317 @@
+// Streaming particle and node kernels: -grad(phi), CIC gather, kick, drift, their fused
+// forms, and the small-N direct force.
+//
+// Every kernel keeps the reference's per-element operation order with one rounding per
+// operation, so the fused variants are bitwise equal to the unfused sequence and to a
+// numpy float32 restatement.
+#include "pm_cic.h"
+#include "pm_common.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// g = -grad(phi) (reference deposit.py:132-174). One thread per node, k fastest, so each
+// wavefront reads and writes 64 consecutive floats of a z-row.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi,
+                                                   float* __restrict__ gx,
+                                                   float* __restrict__ gy,
+                                                   float* __restrict__ gz, int n,
+                                                   float inv_dx, float inv_2dx) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t total = n2 * n;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int k = static_cast<int>(idx % n);
+  const int j = static_cast<int>((idx / n) % n);
+  const int i = static_cast<int>(idx / n2);
+  float a, b, s;
+  // x
+  if (i == 0) {
+    a = phi[idx + n2]; b = phi[idx]; s = inv_dx;
+  } else if (i == n - 1) {
+    a = phi[idx]; b = phi[idx - n2]; s = inv_dx;
+  } else {
+    a = phi[idx + n2]; b = phi[idx - n2]; s = inv_2dx;
+  }
+  const float vx = -(a - b) * s;
+  // y
+  if (j == 0) {
+    a = phi[idx + n]; b = phi[idx]; s = inv_dx;
+  } else if (j == n - 1) {
+    a = phi[idx]; b = phi[idx - n]; s = inv_dx;
+  } else {
+    a = phi[idx + n]; b = phi[idx - n]; s = inv_2dx;
+  }
+  const float vy = -(a - b) * s;
+  // z
+  if (k == 0) {
+    a = phi[idx + 1]; b = phi[idx]; s = inv_dx;
+  } else if (k == n - 1) {
+    a = phi[idx]; b = phi[idx - 1]; s = inv_dx;
+  } else {
+    a = phi[idx + 1]; b = phi[idx - 1]; s = inv_2dx;
+  }
+  const float vz = -(a - b) * s;
+  gx[idx] = vx;
+  gy[idx] = vy;
+  gz[idx] = vz;
+}
+
+// CIC interpolation of the three node fields at one particle (reference deposit.py:100-129).
+__device__ __forceinline__ void cic_gather(float x, float y, float z,
+                                           const float* __restrict__ gx,
+                                           const float* __restrict__ gy,
+                                           const float* __restrict__ gz, int n, float dx,
+                                           float& ax, float& ay, float& az) {
+  ax = 0.0f;
+  ay = 0.0f;
+  az = 0.0f;
+  int ix, iy, iz;
+  float fx, fy, fz;
+  if (!pm::cic_axis(x, dx, n, ix, fx) || !pm::cic_axis(y, dx, n, iy, fy) ||
+      !pm::cic_axis(z, dx, n, iz, fz))
+    return;
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+#pragma unroll
+  for (int cx = 0; cx < 2; ++cx) {
+    const int jx = ix + cx;
+    const float wx = cx ? fx : 1.0f - fx;
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy) {
+      const int jy = iy + cy;
+      const float wy = cy ? fy : 1.0f - fy;
+#pragma unroll
+      for (int cz = 0; cz < 2; ++cz) {
+        const int jz = iz + cz;
+        const float wz = cz ? fz : 1.0f - fz;
+        if (jx >= 0 && jx < n && jy >= 0 && jy < n && jz >= 0 && jz < n) {
+          const float w = wx * wy * wz;
+          const int64_t node = jx * n2 + static_cast<int64_t>(jy) * n + jz;
+          ax += w * gx[node];
+          ay += w * gy[node];
+          az += w * gz[node];
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) gather_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gz,
+    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, int64_t np, int n,
+    float dx) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  float a0, a1, a2;
+  cic_gather(px[p], py[p], pz[p], gx, gy, gz, n, dx, a0, a1, a2);
+  ax[p] = a0;
+  ay[p] = a1;
+  az[p] = a2;
+}
+
+__global__ void __launch_bounds__(256) gather_kick_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gz,
+    float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az,
+    float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz, int64_t np, int n,
+    float dx, float h) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  float a0, a1, a2;
+  cic_gather(px[p], py[p], pz[p], gx, gy, gz, n, dx, a0, a1, a2);
+  ax[p] = a0;
+  ay[p] = a1;
+  az[p] = a2;
+  vx[p] = vx[p] + a0 * h;
+  vy[p] = vy[p] + a1 * h;
+  vz[p] = vz[p] + a2 * h;
+}
+
+// v += a*dt (reference integrator.py:27-42)
+__global__ void __launch_bounds__(256) kick_kernel(float* __restrict__ vx,
+                                                   float* __restrict__ vy,
+                                                   float* __restrict__ vz,
+                                                   const float* __restrict__ ax,
+                                                   const float* __restrict__ ay,
+                                                   const float* __restrict__ az, int64_t np,
+                                                   float dt) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  vx[p] = vx[p] + ax[p] * dt;
+  vy[p] = vy[p] + ay[p] * dt;
+  vz[p] = vz[p] + az[p] * dt;
+}
+
+// x += v*dt (reference integrator.py:45-60)
+__global__ void __launch_bounds__(256) drift_kernel(float* __restrict__ px,
+                                                    float* __restrict__ py,
+                                                    float* __restrict__ pz,
+                                                    const float* __restrict__ vx,
+                                                    const float* __restrict__ vy,
+                                                    const float* __restrict__ vz, int64_t np,
+                                                    float dt) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  px[p] = px[p] + vx[p] * dt;
+  py[p] = py[p] + vy[p] * dt;
+  pz[p] = pz[p] + vz[p] * dt;
+}
+
+__global__ void __launch_bounds__(256) kick_drift_kernel(
+    float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
+    float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+    const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
+    int64_t np, float h, float dt) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const float v0 = vx[p] + ax[p] * h;
+  const float v1 = vy[p] + ay[p] * h;
+  const float v2 = vz[p] + az[p] * h;
+  vx[p] = v0;
+  vy[p] = v1;
+  vz[p] = v2;
+  px[p] = px[p] + v0 * dt;
+  py[p] = py[p] + v1 * dt;
+  pz[p] = pz[p] + v2 * dt;
+}
+
+// a_i = G sum_{j != i} m_j (x_j - x_i) / (|x_i - x_j|^2 + eps^2)^{3/2}
+// (reference integrator.py:63-97); positions of a tile of j staged through LDS.
+__global__ void __launch_bounds__(256) direct_accel_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float* __restrict__ m, float* __restrict__ ax, float* __restrict__ ay,
+    float* __restrict__ az, int64_t np, float grav, float eps2) {
+  __shared__ float sx[256], sy[256], sz[256], sm[256];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = i < np;
+  const float xi = active ? px[i] : 0.0f;
+  const float yi = active ? py[i] : 0.0f;
+  const float zi = active ? pz[i] : 0.0f;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (int64_t base = 0; base < np; base += 256) {
+    const int64_t j = base + threadIdx.x;
+    __syncthreads();
+    if (j < np) {
+      sx[threadIdx.x] = px[j];
+      sy[threadIdx.x] = py[j];
+      sz[threadIdx.x] = pz[j];
+      sm[threadIdx.x] = m[j];
+    }
+    __syncthreads();
+    const int cnt = static_cast<int>(np - base < 256 ? np - base : 256);
+    for (int t = 0; t < cnt; ++t) {
+      if (base + t == i) continue;
+      const float rx = sx[t] - xi;
+      const float ry = sy[t] - yi;
+      const float rz = sz[t] - zi;
+      const float r2 = rx * rx + ry * ry + rz * rz + eps2;
+      const float inv_r3 = 1.0f / (r2 * pm::sqrt_rn(r2));
+      const float f = grav * sm[t] * inv_r3;
+      a0 += f * rx;
+      a1 += f * ry;
+      a2 += f * rz;
+    }
+  }
+  if (active) {
+    ax[i] = a0;
+    ay[i] = a1;
+    az[i] = a2;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+pm_status pm_grad(const float* phi, float* gx, float* gy, float* gz, int32_t n, float dx,
+                  hipStream_t stream) {
+  PM_REQUIRE(phi && gx && gy && gz, "pm_grad: null pointer");
+  PM_REQUIRE(n >= 2 && n <= 2048, "pm_grad: grid_size must be in [2, 2048]");
+  PM_REQUIRE(dx > 0.0f, "pm_grad: dx must be positive");
+  const float inv_dx = 1.0f / dx;
+  const float inv_2dx = 1.0f / (2.0f * dx);
+  const int64_t total = static_cast<int64_t>(n) * n * n;
+  grad_kernel<<<pm::blocks_for(total), pm::kBlock, 0, stream>>>(phi, gx, gy, gz, n, inv_dx,
+                                                                 inv_2dx);
+  PM_CHECK_LAUNCH("grad_kernel");
+  return PM_OK;
+}
+
+pm_status pm_gather(const float* px, const float* py, const float* pz, const float* gx,
+                    const float* gy, const float* gz, float* ax, float* ay, float* az,
+                    int64_t np, int32_t n, float dx, hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_gather: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(px && py && pz && gx && gy && gz && ax && ay && az, "pm_gather: null pointer");
+  PM_REQUIRE(n >= 1 && n <= 2048, "pm_gather: grid_size must be in [1, 2048]");
+  PM_REQUIRE(dx > 0.0f, "pm_gather: dx must be positive");
+  gather_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, gx, gy, gz, ax, ay,
+                                                                az, np, n, dx);
+  PM_CHECK_LAUNCH("gather_kernel");
+  return PM_OK;
+}
+
+pm_status pm_gather_kick(const float* px, const float* py, const float* pz, const float* gx,
+                         const float* gy, const float* gz, float* ax, float* ay, float* az,
+                         float* vx, float* vy, float* vz, int64_t np, int32_t n, float dx,
+                         float h, hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_gather_kick: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(px && py && pz && gx && gy && gz && ax && ay && az && vx && vy && vz,
+             "pm_gather_kick: null pointer");
+  PM_REQUIRE(n >= 1 && n <= 2048, "pm_gather_kick: grid_size must be in [1, 2048]");
+  PM_REQUIRE(dx > 0.0f, "pm_gather_kick: dx must be positive");
+  gather_kick_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+      px, py, pz, gx, gy, gz, ax, ay, az, vx, vy, vz, np, n, dx, h);
+  PM_CHECK_LAUNCH("gather_kick_kernel");
+  return PM_OK;
+}
+
+pm_status pm_kick(float* vx, float* vy, float* vz, const float* ax, const float* ay,
+                  const float* az, int64_t np, float dt, hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_kick: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(vx && vy && vz && ax && ay && az, "pm_kick: null pointer");
+  kick_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(vx, vy, vz, ax, ay, az, np, dt);
+  PM_CHECK_LAUNCH("kick_kernel");
+  return PM_OK;
+}
+
+pm_status pm_drift(float* px, float* py, float* pz, const float* vx, const float* vy,
+                   const float* vz, int64_t np, float dt, hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_drift: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(px && py && pz && vx && vy && vz, "pm_drift: null pointer");
+  drift_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, vx, vy, vz, np, dt);
+  PM_CHECK_LAUNCH("drift_kernel");
+  return PM_OK;
+}
+
+pm_status pm_kick_drift(float* px, float* py, float* pz, float* vx, float* vy, float* vz,
+                        const float* ax, const float* ay, const float* az, int64_t np, float h,
+                        float dt, hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_kick_drift: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(px && py && pz && vx && vy && vz && ax && ay && az, "pm_kick_drift: null pointer");
+  kick_drift_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, vx, vy, vz, ax,
+                                                                    ay, az, np, h, dt);
+  PM_CHECK_LAUNCH("kick_drift_kernel");
+  return PM_OK;
+}
+
+pm_status pm_direct_accel(const float* px, const float* py, const float* pz, const float* m,
+                          float* ax, float* ay, float* az, int64_t np, float grav, float eps2,
+                          hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_direct_accel: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(np <= (int64_t(1) << 24), "pm_direct_accel: O(N^2) path limited to 2^24 particles");
+  PM_REQUIRE(px && py && pz && m && ax && ay && az, "pm_direct_accel: null pointer");
+  direct_accel_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, m, ax, ay, az,
+                                                                      np, grav, eps2);
+  PM_CHECK_LAUNCH("direct_accel_kernel");
+  return PM_OK;
+}
+
+}  // extern "C"

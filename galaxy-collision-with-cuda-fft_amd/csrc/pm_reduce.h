@@ -1,0 +1,92 @@
+// Deterministic float64 reductions shared by the multigrid solver and the diagnostics.
+//
+// Every reduction is two-stage with a fixed tree: stage 1 runs min(kRedBlocks,
+// ceil(count/256)) workgroups; each thread sums its grid-stride elements sequentially,
+// then the workgroup combines lanes with a shuffle-down tree and the four waves in order.
+// Stage 2 is one workgroup that folds the partials the same way. For a given element
+// count the result is therefore identical run to run.
+#pragma once
+
+#include "pm_common.h"
+
+namespace pm {
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedMaxQ = 16;  // quantities per reduction (partials stride)
+
+inline unsigned red_blocks(int64_t count) {
+  int64_t b = (count + kBlock - 1) / kBlock;
+  if (b > kRedBlocks) b = kRedBlocks;
+  if (b < 1) b = 1;
+  return static_cast<unsigned>(b);
+}
+
+// Sum Q doubles across a 256-thread workgroup; the result is valid on thread 0.
+// s_red must hold 4*Q doubles. Contains __syncthreads(): call from uniform control flow.
+template <int Q>
+__device__ __forceinline__ void block_sum(double (&v)[Q], double* s_red) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] += __shfl_down(v[q], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) s_red[wave * Q + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      double t = s_red[q];
+      for (int w = 1; w < 4; ++w) t = t + s_red[w * Q + q];
+      v[q] = t;
+    }
+  }
+  __syncthreads();
+}
+
+// Stage 2: fold `blocks` partials of Q quantities (partials[b*kRedMaxQ + q]) into out[q].
+template <int Q>
+__global__ void __launch_bounds__(256) fold_partials_kernel(const double* __restrict__ partials,
+                                                            int blocks,
+                                                            double* __restrict__ out) {
+  __shared__ double s_red[4 * Q];
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = v[q] + partials[b * kRedMaxQ + q];
+  }
+  block_sum<Q>(v, s_red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) out[q] = v[q];
+  }
+}
+
+// Scratch for reductions issued without a handle: kRedBlocks*kRedMaxQ partial doubles and
+// kRedMaxQ result doubles on the device, plus a pinned host mirror of the result.
+struct RedScratch {
+  double* partials = nullptr;
+  double* result = nullptr;
+  double* host = nullptr;
+};
+pm_status red_scratch(RedScratch** out);
+
+// sum of x^2 over `count` float32 values -> out_dev[0] (float64). Enqueued only.
+pm_status launch_sumsq_f32(const float* x, int64_t count, double* partials, double* out_dev,
+                           hipStream_t stream);
+// sum of x^2 over `count` float64 values -> out_dev[0]. Enqueued only.
+pm_status launch_sumsq_f64(const double* x, int64_t count, double* partials, double* out_dev,
+                           hipStream_t stream);
+// Ten raw moments of rho -> sums_dev[0..9]; if face_params is non-null also writes the
+// eleven float32 face parameters [G, M, cx, cy, cz, sxx, syy, szz, sxy, sxz, syz]
+// computed from them exactly as reference multigrid.py:364-389 does on the host.
+pm_status launch_moments(const float* rho, int n, float dx, double grav, double* partials,
+                         double* sums_dev, float* face_params, hipStream_t stream);
+
+}  // namespace pm

@@ -1,0 +1,184 @@
+"""ctypes binding of libpmchain.so (declared in include/pmchain.h).
+
+The library is required: importing the package on a machine without the built library, or
+calling a kernel without a HIP device, raises instead of falling back to any CPU path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(os.environ.get("PMCHAIN_LIB", Path(__file__).with_name("libpmchain.so")))
+
+PM_OK = 0
+PM_ERR_CAPACITY = 4
+PM_MG_PLAIN = 0
+PM_MG_MIXED = 1
+
+_vp = ctypes.c_void_p
+_f = ctypes.c_float
+_d = ctypes.c_double
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+
+
+class PMChainError(RuntimeError):
+    """An error status returned by libpmchain."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class MGDesc(ctypes.Structure):
+    _fields_ = [
+        ("grid_size", _i32),
+        ("dx", _f),
+        ("grav", _d),
+        ("n_cycles", _i32),
+        ("pre_sweeps", _i32),
+        ("post_sweeps", _i32),
+        ("coarse_sweeps", _i32),
+        ("min_cycles", _i32),
+        ("tol", _d),
+        ("mode", _i32),
+    ]
+
+
+class MGStats(ctypes.Structure):
+    _fields_ = [
+        ("cycles", _i32),
+        ("levels", _i32),
+        ("resid_norm", _d),
+        ("rhs_norm", _d),
+        ("err_est", _d),
+    ]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 48),
+        ("launches", _i64),
+        ("total_ms", _d),
+        ("bytes", _d),
+    ]
+
+
+def timing_enable(on: bool) -> None:
+    call("pm_timing_enable", 1 if on else 0)
+
+
+def timing_reset() -> None:
+    call("pm_timing_reset")
+
+
+def timing_report() -> dict:
+    """{kernel name: {"launches", "total_ms", "bytes"}} accumulated since the last reset."""
+    count = _i32()
+    call("pm_timing_report", 0, None, ctypes.byref(count))
+    buf = (KernelTime * max(count.value, 1))()
+    call("pm_timing_report", count.value, buf, ctypes.byref(count))
+    return {k.name.decode(): {"launches": int(k.launches), "total_ms": float(k.total_ms),
+                              "bytes": float(k.bytes)} for k in buf[: count.value]}
+
+
+# name -> argtypes; every function returns pm_status (int)
+_SIGNATURES = {
+    "pm_deposit_plan_create": [_i32, _i64, ctypes.POINTER(_vp)],
+    "pm_deposit_plan_destroy": [_vp],
+    "pm_deposit_cic": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
+    "pm_deposit_plan_set_timing": [_vp, _i32],
+    "pm_deposit_last_timing": [_vp, ctypes.POINTER(_f)],
+    "pm_grad": [_vp, _vp, _vp, _vp, _i32, _f, _vp],
+    "pm_gather": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _vp],
+    "pm_kick": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _vp],
+    "pm_drift": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _vp],
+    "pm_kick_drift": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _vp],
+    "pm_gather_kick": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
+                       _f, _f, _vp],
+    "pm_direct_accel": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _vp],
+    "pm_mg_create": [ctypes.POINTER(MGDesc), ctypes.POINTER(_vp)],
+    "pm_mg_destroy": [_vp],
+    "pm_mg_num_levels": [_vp, ctypes.POINTER(_i32)],
+    "pm_mg_solve": [_vp, _vp, _vp, _i32, ctypes.POINTER(MGStats), _vp],
+    "pm_mg_vcycle": [_vp, _vp],
+    "pm_mg_residual_norm": [_vp, ctypes.POINTER(_d), _vp],
+    "pm_mg_moments": [_vp, _vp, ctypes.POINTER(_d), _vp],
+    "pm_rb_pass": [_vp, _vp, _i32, _f, _i32, _vp],
+    "pm_residual": [_vp, _vp, _vp, _i32, _f, _vp],
+    "pm_restrict": [_vp, _vp, _i32, _i32, _vp],
+    "pm_prolong_add": [_vp, _vp, _i32, _i32, _vp],
+    "pm_set_faces": [_vp, _i32, _f, ctypes.POINTER(_f), _vp],
+    "pm_reduce_particles_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_d), _vp],
+    "pm_reduce_grid_pe_f64": [_vp, _vp, _i64, ctypes.POINTER(_d), _vp],
+    "pm_radial_histogram_f64": [_vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _i32, _vp, _vp],
+    "pm_timing_enable": [_i32],
+    "pm_timing_reset": [],
+    "pm_timing_report": [_i32, _vp, ctypes.POINTER(_i32)],
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libpmchain.so once; raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"libpmchain.so not found at {LIB_PATH}: build it with "
+                "`python galaxy-collision-with-cuda-fft_amd/build_native.py`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.pm_last_error.restype = ctypes.c_char_p
+        L.pm_last_error.argtypes = []
+        L.pm_abi_version.restype = _i32
+        for name, argtypes in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def exported_names() -> list[str]:
+    return ["pm_last_error", "pm_abi_version", *_SIGNATURES]
+
+
+def call(name: str, *args) -> None:
+    """Call a pm_* entry point and raise PMChainError on a non-zero status."""
+    L = lib()
+    status = getattr(L, name)(*args)
+    if status != PM_OK:
+        msg = L.pm_last_error().decode(errors="replace")
+        raise PMChainError(status, f"{name}: {msg}")
+
+
+def require_device() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("libpmchain kernels need a HIP device (torch.cuda.is_available() is False)")
+
+
+def ptr(t: torch.Tensor | None, dtype: torch.dtype = torch.float32) -> int | None:
+    """Device address of a contiguous HIP tensor (None passes through)."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        t = getattr(t, "tensor", None)
+        if t is None:
+            raise TypeError("expected a torch tensor or a DeviceField")
+    if not t.is_cuda:
+        raise ValueError("libpmchain works on device tensors; got a host tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("libpmchain needs contiguous tensors")
+    return t.data_ptr()
+
+
+def stream(device: torch.device | None = None) -> int:
+    """The HIP stream torch is currently issuing work on (hipStream_t as an integer)."""
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,168 @@
+"""Structure-of-arrays particle and grid state in HBM (reference data.py:84-138).
+
+Every field is one contiguous device tensor (float32, grids C-order with k fastest). A
+``DeviceField`` wraps it with the small field API the reference's callers use
+(``to_numpy``, ``from_numpy``, ``fill``, ``shape``, ``[i]``), so drivers written against the
+reference run unchanged; ``.tensor`` exposes the torch tensor.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+_F32 = 4
+_I32 = 4
+
+# Particle fields: pos(x,y,z) + vel(x,y,z) + mass f32, gid i32 (reference data.py:33-34).
+BYTES_PER_PARTICLE = 6 * _F32 + _F32 + _I32
+# rho + phi, the node acceleration field, and the multigrid hierarchy (phi/rhs/res per
+# level, 8/7 of the finest) in units of n^3 float32 arrays.
+_GRID_FIELDS = 2 + 3 + 3.0 * 8.0 / 7.0
+
+
+@dataclass(frozen=True)
+class MemoryEstimate:
+    n_particles: int
+    grid_size: int
+    particle_bytes: int
+    grid_bytes: int
+
+    @property
+    def total_bytes(self) -> int:
+        return self.particle_bytes + self.grid_bytes
+
+    @property
+    def total_gb(self) -> float:
+        return self.total_bytes / 1e9
+
+
+def estimate_memory(n_particles: int, grid_size: int = 256) -> MemoryEstimate:
+    """SoA footprint of the production path (same accounting as reference data.py:61-81)."""
+    if n_particles <= 0:
+        raise ValueError(f"n_particles must be positive, got {n_particles}")
+    if grid_size <= 0:
+        raise ValueError(f"grid_size must be positive, got {grid_size}")
+    return MemoryEstimate(n_particles, grid_size, n_particles * BYTES_PER_PARTICLE,
+                          int(_GRID_FIELDS * grid_size ** 3 * _F32))
+
+
+def default_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("a HIP device is required for device-resident state")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class DeviceField:
+    """A contiguous device tensor with the field methods reference callers use."""
+
+    __slots__ = ("tensor",)
+
+    def __init__(self, tensor: torch.Tensor):
+        if not tensor.is_contiguous():
+            raise ValueError("DeviceField needs a contiguous tensor")
+        self.tensor = tensor
+
+    @classmethod
+    def zeros(cls, shape, dtype=torch.float32, device=None) -> "DeviceField":
+        if isinstance(shape, int):
+            shape = (shape,)
+        return cls(torch.zeros(tuple(shape), dtype=dtype, device=device or default_device()))
+
+    @property
+    def shape(self) -> tuple[int, ...]:
+        return tuple(self.tensor.shape)
+
+    @property
+    def dtype(self):
+        return self.tensor.dtype
+
+    def to_numpy(self) -> np.ndarray:
+        return self.tensor.detach().cpu().numpy()
+
+    def from_numpy(self, arr: np.ndarray) -> None:
+        arr = np.asarray(arr)
+        if tuple(arr.shape) != self.shape:
+            raise ValueError(f"shape mismatch: field {self.shape}, array {tuple(arr.shape)}")
+        src = torch.from_numpy(np.ascontiguousarray(arr)).to(self.tensor.dtype)
+        self.tensor.copy_(src)
+
+    def fill(self, value) -> None:
+        self.tensor.fill_(value)
+
+    def __getitem__(self, idx):
+        v = self.tensor[idx]
+        return v.item() if v.dim() == 0 else v
+
+    def __setitem__(self, idx, value) -> None:
+        self.tensor[idx] = value
+
+    def __len__(self) -> int:
+        return self.tensor.shape[0]
+
+
+def as_tensor(f) -> torch.Tensor:
+    """The tensor behind a DeviceField (or the tensor itself)."""
+    return f.tensor if isinstance(f, DeviceField) else f
+
+
+class ParticleState:
+    """SoA particle fields on the device (reference data.py:84-113)."""
+
+    def __init__(self, n: int, device=None):
+        if n <= 0:
+            raise ValueError(f"particle count must be positive, got {n}")
+        dev = device or default_device()
+        self.n = n
+        mk = lambda dtype=torch.float32: DeviceField.zeros(n, dtype, dev)  # noqa: E731
+        self.pos_x, self.pos_y, self.pos_z = mk(), mk(), mk()
+        self.vel_x, self.vel_y, self.vel_z = mk(), mk(), mk()
+        self.mass = mk()
+        self.gid = mk(torch.int32)
+
+    def fields(self):
+        return (self.pos_x, self.pos_y, self.pos_z, self.vel_x, self.vel_y, self.vel_z,
+                self.mass, self.gid)
+
+    def load(self, pos: np.ndarray, vel: np.ndarray | None = None, mass: np.ndarray | None = None,
+             gid: np.ndarray | None = None) -> None:
+        """Copy host (N,3)/(N,) arrays into the device fields."""
+        pos = np.asarray(pos, dtype=np.float32)
+        for a, f in enumerate((self.pos_x, self.pos_y, self.pos_z)):
+            f.from_numpy(np.ascontiguousarray(pos[:, a]))
+        if vel is not None:
+            vel = np.asarray(vel, dtype=np.float32)
+            for a, f in enumerate((self.vel_x, self.vel_y, self.vel_z)):
+                f.from_numpy(np.ascontiguousarray(vel[:, a]))
+        if mass is not None:
+            self.mass.from_numpy(np.asarray(mass, dtype=np.float32))
+        if gid is not None:
+            self.gid.from_numpy(np.asarray(gid, dtype=np.int32))
+
+    def positions(self) -> np.ndarray:
+        return np.stack([f.to_numpy() for f in (self.pos_x, self.pos_y, self.pos_z)], axis=1)
+
+    def velocities(self) -> np.ndarray:
+        return np.stack([f.to_numpy() for f in (self.vel_x, self.vel_y, self.vel_z)], axis=1)
+
+
+class GridState:
+    """Device grids rho, phi and g = -grad(phi) (reference data.py:116-138)."""
+
+    def __init__(self, grid_size: int = 256, device=None):
+        if grid_size <= 0:
+            raise ValueError(f"grid_size must be positive, got {grid_size}")
+        dev = device or default_device()
+        self.grid_size = grid_size
+        shape = (grid_size,) * 3
+        self.rho = DeviceField.zeros(shape, device=dev)
+        self.phi = DeviceField.zeros(shape, device=dev)
+        self.ax = DeviceField.zeros(shape, device=dev)
+        self.ay = DeviceField.zeros(shape, device=dev)
+        self.az = DeviceField.zeros(shape, device=dev)
+
+
+__all__ = ["BYTES_PER_PARTICLE", "MemoryEstimate", "estimate_memory", "DeviceField",
+           "ParticleState", "GridState", "as_tensor"]

@@ -1,0 +1,192 @@
+"""Seeded synthetic initial conditions (host numpy), the inputs of the PM benchmarks.
+
+- ``plummer_ic``: an isolated Plummer sphere in equilibrium — positions from the inverse of
+  the cumulative mass M(<r)/M = r^3/(r^2+a^2)^{3/2}, speeds q*v_esc with q drawn by
+  rejection from q^2(1-q^2)^{7/2} under a 0.1 envelope, isotropic directions, centre of
+  mass and mean velocity removed (the recipe of reference ic.py:296-380).
+- ``two_galaxy_ic``: the collision recipe of the benchmark configurations — per galaxy an
+  exponential disk (scale length r_d, sech^2 vertical profile of height z0, 80% of the
+  stellar mass), a Plummer bulge (scale a_b, 20%) and a central black-hole particle (1% of
+  the stellar mass); circular velocities from the enclosed mass plus Gaussian dispersion;
+  disks inclined about the collision (x) axis; galaxies placed symmetrically about the box
+  centre along x and closing at a fraction of the mutual escape speed.
+
+Distances are in the box's length unit with G = 1 unless given.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+COMPONENT_DISK, COMPONENT_BULGE, COMPONENT_BH = 0, 1, 2
+
+
+@dataclass
+class ICResult:
+    pos: np.ndarray        # (N, 3) float32
+    vel: np.ndarray        # (N, 3) float32
+    mass: np.ndarray       # (N,) float32
+    gid: np.ndarray        # (N,) int32 galaxy id
+    component: np.ndarray  # (N,) int8
+    preset: str
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+
+def _unit_vectors(rng, n: int) -> np.ndarray:
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v
+
+
+def _plummer_speed_ratio(rng, n: int) -> np.ndarray:
+    out = np.empty(n)
+    filled = 0
+    while filled < n:
+        m = n - filled
+        x = rng.random(m)
+        y = rng.random(m)
+        ok = 0.1 * y < x ** 2 * (1.0 - x ** 2) ** 3.5
+        k = int(ok.sum())
+        out[filled:filled + k] = x[ok]
+        filled += k
+    return out
+
+
+def _plummer_sample(rng, n: int, mass: float, a: float, grav: float, rmax_factor: float = 20.0):
+    u = np.clip(rng.random(n), 1e-12, 1.0)
+    r = np.minimum(a / np.sqrt(u ** (-2.0 / 3.0) - 1.0), rmax_factor * a)
+    pos = r[:, None] * _unit_vectors(rng, n)
+    q = _plummer_speed_ratio(rng, n)
+    v_esc = np.sqrt(2.0 * grav * mass) * (r ** 2 + a ** 2) ** -0.25
+    vel = (q * v_esc)[:, None] * _unit_vectors(rng, n)
+    return pos, vel
+
+
+def plummer_ic(n: int, *, total_mass: float = 1.0, scale_a: float = 1.0, grav: float = 1.0,
+               centre=(0.0, 0.0, 0.0), seed: int = 0) -> ICResult:
+    """Equal-mass Plummer sphere at `centre`, at rest."""
+    rng = np.random.default_rng(seed)
+    pos, vel = _plummer_sample(rng, n, total_mass, scale_a, grav)
+    pos -= pos.mean(axis=0)
+    vel -= vel.mean(axis=0)
+    pos += np.asarray(centre, dtype=np.float64)
+    return ICResult(pos.astype(np.float32), vel.astype(np.float32),
+                    np.full(n, total_mass / n, np.float32), np.zeros(n, np.int32),
+                    np.full(n, COMPONENT_BULGE, np.int8), "plummer")
+
+
+def _exp_disk_radius(u: np.ndarray, r_d: float) -> np.ndarray:
+    """Inverse of M(<R)/M_d = 1 - (1 + R/r_d) exp(-R/r_d) via the W_{-1} branch."""
+    from scipy.special import lambertw
+
+    w = lambertw(-(1.0 - u) / np.e, k=-1).real
+    return r_d * (-1.0 - w)
+
+
+def _rotate_x(v: np.ndarray, deg: float) -> np.ndarray:
+    t = np.deg2rad(deg)
+    c, s = np.cos(t), np.sin(t)
+    out = v.copy()
+    out[:, 1] = c * v[:, 1] - s * v[:, 2]
+    out[:, 2] = s * v[:, 1] + c * v[:, 2]
+    return out
+
+
+def galaxy(rng, n: int, *, mass: float = 1.0, disk_frac: float = 0.8, r_d: float = 1.0,
+           z0: float = 0.1, bulge_a: float = 0.2, bh_frac: float = 0.01,
+           dispersion: float = 0.1, grav: float = 1.0, r_max: float = 9.0,
+           softening: float = 0.05):
+    """One galaxy centred at the origin in its own disk plane: (pos, vel, mass, component).
+
+    n stellar particles (disk + bulge, equal masses) plus one black-hole particle.
+    """
+    n_disk = int(round(disk_frac * n))
+    n_bulge = n - n_disk
+    m_disk, m_bulge = mass * disk_frac, mass * (1.0 - disk_frac)
+    m_bh = bh_frac * mass
+    # disk: radius from the exponential profile truncated at r_max, sech^2 heights
+    u_max = 1.0 - (1.0 + r_max / r_d) * np.exp(-r_max / r_d)
+    R = _exp_disk_radius(rng.random(n_disk) * u_max, r_d)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n_disk)
+    z = z0 * np.arctanh(np.clip(2.0 * rng.random(n_disk) - 1.0, -1 + 1e-12, 1 - 1e-12))
+    d_pos = np.stack([R * np.cos(phi), R * np.sin(phi), z], axis=1)
+    x = R / r_d
+    m_enc = (m_disk * (1.0 - (1.0 + x) * np.exp(-x)) / u_max
+             + m_bulge * R ** 3 / (R ** 2 + bulge_a ** 2) ** 1.5 + m_bh)
+    r_eff = np.sqrt(R ** 2 + softening ** 2)
+    v_c = np.sqrt(grav * m_enc * R ** 2 / r_eff ** 3)
+    tang = np.stack([-np.sin(phi), np.cos(phi), np.zeros_like(phi)], axis=1)
+    d_vel = v_c[:, None] * tang + dispersion * v_c[:, None] * rng.normal(size=(n_disk, 3))
+    # bulge: Plummer sphere with its isotropic dispersion sigma^2 = G M_b / (6 sqrt(r^2+a^2))
+    b_pos, _ = _plummer_sample(rng, n_bulge, m_bulge, bulge_a, grav, rmax_factor=r_max / bulge_a)
+    rb = np.linalg.norm(b_pos, axis=1)
+    sigma = np.sqrt(grav * (m_bulge / (6.0 * np.sqrt(rb ** 2 + bulge_a ** 2))
+                            + m_bh / (3.0 * np.sqrt(rb ** 2 + softening ** 2))))
+    b_vel = sigma[:, None] * rng.normal(size=(n_bulge, 3))
+    pos = np.concatenate([d_pos, b_pos, np.zeros((1, 3))])
+    vel = np.concatenate([d_vel, b_vel, np.zeros((1, 3))])
+    m = np.concatenate([np.full(n_disk, m_disk / n_disk), np.full(n_bulge, m_bulge / n_bulge), [m_bh]])
+    comp = np.concatenate([np.full(n_disk, COMPONENT_DISK), np.full(n_bulge, COMPONENT_BULGE),
+                           [COMPONENT_BH]]).astype(np.int8)
+    return pos, vel, m, comp
+
+
+def two_galaxy_ic(n_per_galaxy, *, box_centre, separation: float = 20.0,
+                  approach: float = 0.5, inclinations=(30.0, -45.0), masses=(1.0, 1.0),
+                  grav: float = 1.0, seed: int = 0, **galaxy_kw) -> ICResult:
+    """Two galaxies on a head-on approach along x (see the module docstring).
+
+    n_per_galaxy: int or (n1, n2) stellar particles per galaxy (each also gets a BH).
+    masses: stellar masses; the closing speed is `approach` times sqrt(2 G M_tot / sep)
+    with M_tot including the black holes.
+    """
+    rng = np.random.default_rng(seed)
+    counts = (n_per_galaxy, n_per_galaxy) if np.isscalar(n_per_galaxy) else tuple(n_per_galaxy)
+    bh_frac = galaxy_kw.get("bh_frac", 0.01)
+    m_tot = sum(masses) * (1.0 + bh_frac)
+    v_rel = approach * np.sqrt(2.0 * grav * m_tot / separation)
+    centre = np.asarray(box_centre, dtype=np.float64)
+    parts = []
+    for g, (n, m, inc) in enumerate(zip(counts, masses, inclinations)):
+        pos, vel, mm, comp = galaxy(rng, int(n), mass=m, grav=grav, **galaxy_kw)
+        pos, vel = _rotate_x(pos, inc), _rotate_x(vel, inc)
+        # galaxy 0 on the left moving +x, galaxy 1 on the right moving -x; the bulk speeds
+        # split v_rel so that the total momentum is zero
+        m_this, m_other = m, masses[1 - g]
+        side = -1.0 if g == 0 else 1.0
+        offset = separation * m_other / (m_this + m_other)
+        speed = v_rel * m_other / (m_this + m_other)
+        pos = pos + centre + np.array([side * offset, 0.0, 0.0])
+        vel = vel + np.array([-side * speed, 0.0, 0.0])
+        parts.append((pos, vel, mm, np.full(len(mm), g, np.int32), comp))
+    pos, vel, mass, gid, comp = (np.concatenate([p[i] for p in parts]) for i in range(5))
+    return ICResult(pos.astype(np.float32), vel.astype(np.float32), mass.astype(np.float32),
+                    gid, comp, "two_galaxy")
+
+
+def sort_by_cell(ic: ICResult, grid_size: int, dx: float) -> ICResult:
+    """Reorders particles by their CIC cell (C order): the memory layout the fused step
+    prefers (neighbouring threads touch neighbouring nodes). Particle identity is kept in
+    ``gid``/``component``; the physics does not depend on the order."""
+    g = np.floor(ic.pos.astype(np.float64) / dx).astype(np.int64)
+    g = np.clip(g, -1, grid_size)
+    m = grid_size + 2
+    key = ((g[:, 0] + 1) * m + (g[:, 1] + 1)) * m + (g[:, 2] + 1)
+    order = np.argsort(key, kind="stable")
+    return ICResult(ic.pos[order], ic.vel[order], ic.mass[order], ic.gid[order],
+                    ic.component[order], ic.preset)
+
+
+def box_fit_fraction(ic: ICResult, grid_size: int, dx: float) -> float:
+    """Mass fraction outside the node span [0, (grid_size-1)*dx] (reference sim.py:173-191)."""
+    hi = (grid_size - 1) * dx
+    outside = np.any((ic.pos < 0.0) | (ic.pos > hi), axis=1)
+    return float(ic.mass[outside].astype(np.float64).sum() / ic.mass.astype(np.float64).sum())
+
+
+__all__ = ["ICResult", "plummer_ic", "two_galaxy_ic", "galaxy", "sort_by_cell", "box_fit_fraction"]

@@ -1,0 +1,103 @@
+"""GPU parity of the sort-based CIC deposit (replaces reference deposit.py:47-82).
+
+Cell keys and the stable permutation are integer work: bit-exact against np.floor and
+np.argsort(kind="stable"). rho is bit-exact against the oracle's restatement of the same
+summation order (oracle/pm_oracle.c), and the reference's closed forms hold.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import pm_oracle as O
+from conftest import announce, assert_bitwise
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _run(device, pos, mass, n, dx):
+    from galaxy_collision_amd.data import ParticleState
+    from galaxy_collision_amd.deposit import deposit_keys
+
+    parts = ParticleState(max(len(mass), 1))
+    if len(mass):
+        parts.load(pos, None, mass)
+    parts.n = len(mass)
+    rho = torch.full((n, n, n), 7.0, device=device)  # deposit must overwrite everything
+    keys, perm = deposit_keys(parts, rho, dx)
+    return rho.cpu().numpy(), keys.cpu().numpy().view(np.uint32), perm.cpu().numpy()
+
+
+def _check(device, pos, mass, n, dx):
+    rho, keys, perm = _run(device, pos, mass, n, dx)
+    ref_rho, ref_keys, ref_perm = O.deposit_cic(pos, mass, n, dx)
+    assert_bitwise(keys, O.cell_keys(pos, n, dx))
+    assert_bitwise(keys, ref_keys)
+    assert_bitwise(perm, O.stable_perm(keys))
+    assert_bitwise(rho, ref_rho)
+    return rho
+
+
+def test_closed_forms(device):
+    announce("deposit closed forms")
+    rho, _, _ = _run(device, np.array([[8.0, 8, 8]], F32), np.array([3.0], F32), 16, 1.0)
+    assert rho[8, 8, 8] == 3.0 and rho.sum() == 3.0
+    rho, _, _ = _run(device, np.array([[8.5, 8.5, 8.5]], F32), np.array([8.0], F32), 16, 1.0)
+    assert np.all(rho[8:10, 8:10, 8:10] == 1.0) and rho.sum() == 8.0
+    rho, _, _ = _run(device, np.array([[-3.0, 8, 8], [40.0, 8, 8], [15.5, 8, 8]], F32),
+                     np.ones(3, F32), 16, 1.0)
+    assert rho[15, 8, 8] == 0.5 and rho[0].sum() == 0.0 and rho.sum() == 0.5
+    dx = 2.5
+    rho, _, _ = _run(device, np.array([[2 * dx, 3 * dx, 4 * dx]], F32), np.array([5.0], F32), 8, dx)
+    assert rho[2, 3, 4] == pytest.approx(5.0 / dx ** 3, rel=1e-6)
+
+
+@pytest.mark.parametrize("seed,n,dx", [(0, 64, 1.0), (3, 32, 2.5)])
+def test_mass_conservation(device, seed, n, dx):
+    announce(f"deposit mass conservation seed={seed}")
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(2.0 * dx, (n - 3) * dx, size=(5000, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2.0, size=5000).astype(F32)
+    rho = _check(device, pos, mass, n, dx)
+    assert rho.astype(np.float64).sum() * dx ** 3 == pytest.approx(mass.astype(np.float64).sum(), rel=1e-4)
+
+
+def test_edge_cases(device):
+    announce("deposit edge cases")
+    # empty
+    rho, keys, perm = _run(device, np.zeros((0, 3), F32), np.zeros(0, F32), 8, 1.0)
+    assert not rho.any() and keys.size == 0
+    # everything outside the box
+    pos = np.array([[-5.0, 1, 1], [100.0, 2, 2], [np.nan, 3, 3], [3, np.inf, 3]], F32)
+    _check(device, pos, np.ones(4, F32), 8, 1.0)
+    # exactly on the -1 and n boundaries of the key range
+    pos = np.array([[-1.0, 0, 0], [7.999, 7.999, 7.999], [8.0, 1, 1], [-1.0001, 2, 2]], F32)
+    _check(device, pos, np.ones(4, F32), 8, 1.0)
+
+
+@pytest.mark.parametrize("npart,n,dx,dense", [
+    (1, 4, 1.0, 0), (17, 4, 1.0, 17), (4097, 8, 1.0, 0), (50_000, 32, 0.3125, 20_000),
+    (300_000, 64, 1.0, 150_000), (1_000_003, 128, 0.5, 0)])
+def test_random_bitwise(device, npart, n, dx, dense):
+    """Includes a cell holding `dense` particles: thousands of pieces, the multi-batch tree."""
+    announce(f"deposit random npart={npart} n={n} dense={dense}")
+    rng = np.random.default_rng(npart)
+    pos = rng.normal(n * dx / 2, n * dx / 5, size=(npart, 3)).astype(F32)
+    if dense:
+        pos[:dense] = rng.uniform(0.3 * n * dx, 0.3 * n * dx + 0.9 * dx, size=(dense, 3))
+        pos = pos[rng.permutation(npart)]
+    mass = rng.uniform(0.5, 2.0, npart).astype(F32)
+    _check(device, pos, mass, n, dx)
+
+
+def test_deterministic_repeat(device):
+    announce("deposit determinism")
+    rng = np.random.default_rng(11)
+    pos = rng.normal(32, 4, size=(400_000, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2.0, 400_000).astype(F32)
+    a, _, _ = _run(device, pos, mass, 64, 1.0)
+    b, _, _ = _run(device, pos, mass, 64, 1.0)
+    assert_bitwise(a, b)

@@ -1,0 +1,375 @@
+"""Pins the CPU oracle to the reference's closed-form tests and to mathematics.
+
+Each restatement in oracle/pm_oracle.py is checked against something that does not
+share its formula: the reference test suite's closed forms (tests/test_deposit.py,
+test_integrator.py, test_solver_multigrid.py, test_solver_fft.py), float64 ground truths
+(np.add.at scatter, the DST-I exact Dirichlet solve, brute-force dense linear algebra and
+direct convolution on tiny grids), and exactness cases where float32 rounding vanishes.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import pm_oracle as O
+
+F32 = np.float32
+
+
+# ---------------------------------------------------------------- deposit (deposit.py)
+
+def test_deposit_on_node_and_midpoint():
+    rho, _, _ = O.deposit_cic(np.array([[8.0, 8.0, 8.0]]), np.array([3.0]), 16, 1.0)
+    assert rho[8, 8, 8] == 3.0 and rho.sum() == 3.0
+    rho, _, _ = O.deposit_cic(np.array([[8.5, 8.5, 8.5]]), np.array([8.0]), 16, 1.0)
+    assert np.all(rho[8:10, 8:10, 8:10] == 1.0) and rho.sum() == 8.0
+
+
+def test_deposit_on_node_dx_not_one():
+    dx = 2.5
+    rho, _, _ = O.deposit_cic(np.array([[2 * dx, 3 * dx, 4 * dx]]), np.array([5.0]), 8, dx)
+    assert rho[2, 3, 4] == pytest.approx(5.0 / dx ** 3, rel=1e-6)
+    assert rho.sum() == pytest.approx(5.0 / dx ** 3, rel=1e-6)
+
+
+def test_deposit_open_boundary_drops_outside_weight():
+    rho, keys, _ = O.deposit_cic(np.array([[-3.0, 8, 8], [40.0, 8, 8], [15.5, 8, 8]]),
+                                 np.ones(3), 16, 1.0)
+    assert rho[15, 8, 8] == pytest.approx(0.5)
+    assert rho[0].sum() == 0.0
+    assert rho.sum() == pytest.approx(0.5)
+    assert keys[0] == keys[1] == 17 ** 3  # no corner in the box
+
+
+@pytest.mark.parametrize("seed,n,dx", [(0, 64, 1.0), (3, 32, 2.5)])
+def test_deposit_mass_conservation(seed, n, dx):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(2.0 * dx, (n - 3) * dx, size=(5000, 3))
+    mass = rng.uniform(0.5, 2.0, size=5000)
+    rho, _, _ = O.deposit_cic(pos, mass, n, dx)
+    assert rho.astype(np.float64).sum() * dx ** 3 == pytest.approx(mass.sum(), rel=1e-4)
+
+
+def test_deposit_exact_when_rounding_vanishes():
+    """Power-of-two masses at quarter-integer positions: every product and sum is exact in
+    float32, so the fixed-order deposit must equal the float64 scatter bit for bit."""
+    rng = np.random.default_rng(7)
+    n = 12
+    pos = rng.integers(-4, 4 * n + 4, size=(3000, 3)).astype(np.float64) / 4.0
+    pos[:800] = 5.25  # one heavily populated cell: many pieces
+    mass = 2.0 ** rng.integers(-2, 3, size=3000)
+    rho, _, _ = O.deposit_cic(pos, mass, n, 1.0)
+    ref = O.deposit_cic_f64(pos, mass, n, 1.0)
+    assert np.array_equal(rho.astype(np.float64), ref)
+
+
+def test_deposit_random_matches_float64_and_sort():
+    rng = np.random.default_rng(1)
+    n = 24
+    pos = rng.normal(n / 2, n / 6, size=(40000, 3)).astype(F32)
+    pos[:6000] = rng.uniform(11.0, 11.999, size=(6000, 3))  # dense cell with many pieces
+    mass = rng.uniform(0.5, 2.0, 40000).astype(F32)
+    rho, keys, perm = O.deposit_cic(pos, mass, n, 0.75)
+    ref = O.deposit_cic_f64(pos, mass, n, 0.75)
+    assert np.abs(rho - ref).max() <= 2e-6 * np.abs(ref).max()
+    # keys from an independent float64-floor computation (positions exactly representable)
+    g = pos.astype(np.float64) / np.float64(F32(0.75))
+    i = np.floor(g.astype(F32)).astype(np.int64)
+    ok = np.all((g.astype(F32) >= -1) & (g.astype(F32) < n), axis=1)
+    m = n + 1
+    k2 = np.where(ok, ((i[:, 0] + 1) * m + i[:, 1] + 1) * m + i[:, 2] + 1, m ** 3)
+    assert np.array_equal(keys.astype(np.int64), k2)
+    assert np.array_equal(perm, np.argsort(keys, kind="stable"))
+
+
+def test_deposit_empty():
+    rho, keys, perm = O.deposit_cic(np.zeros((0, 3)), np.zeros(0), 8, 1.0)
+    assert rho.shape == (8, 8, 8) and not rho.any() and keys.size == 0 and perm.size == 0
+
+
+# ---------------------------------------------------------------- grad / gather / kick
+
+def test_grad_linear_phi_and_faces():
+    n = 16
+    ii, jj, kk = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    gx, gy, gz = O.grad((2.0 * ii + 3.0 * jj - 1.0 * kk).astype(F32), 1.0)
+    assert np.all(gx == -2.0) and np.all(gy == -3.0) and np.all(gz == 1.0)  # faces too
+    dx = 2.0
+    gx, gy, _ = O.grad((2.0 * ii * dx + 3.0 * jj * dx).astype(F32), dx)
+    assert np.allclose(gx, -2.0, atol=1e-4) and np.allclose(gy, -3.0, atol=1e-4)
+
+
+def test_gather_constant_and_linear_fields():
+    rng = np.random.default_rng(2)
+    n = 32
+    pos = rng.uniform(1.0, n - 2.0, size=(500, 3)).astype(F32)
+    ones = np.ones((n, n, n), F32)
+    ax, ay, az = O.gather(pos, 2 * ones, -3 * ones, 0.5 * ones, 1.0)
+    assert np.allclose(ax, 2.0, atol=1e-5) and np.allclose(ay, -3.0, atol=1e-5)
+    ii, jj, kk = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    lin = (0.3 * ii + 0.1 * jj - 0.2 * kk).astype(F32)
+    ax, _, _ = O.gather(pos, lin, 0 * lin, 0 * lin, 1.0)
+    exp = 0.3 * pos[:, 0] + 0.1 * pos[:, 1] - 0.2 * pos[:, 2]
+    assert np.allclose(ax, exp, atol=1e-3)
+
+
+def test_force_near_boundary_not_halved():
+    n = 16
+    ii, _, _ = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    gx, gy, gz = O.grad((2.0 * ii).astype(F32), 1.0)
+    ax, _, _ = O.gather(np.array([[14.5, 8.0, 8.0]], F32), gx, gy, gz, 1.0)
+    assert ax[0] == pytest.approx(-2.0, abs=1e-4)
+
+
+def test_kick_drift_uniform_acceleration():
+    dt = 0.2
+    v = np.zeros(4, F32)
+    x = np.zeros(4, F32)
+    a = np.full(4, 0.5, F32)
+    v = O.kick(v, a, dt / 2)
+    x = O.drift(x, v, dt)
+    v = O.kick(v, a, dt / 2)
+    assert v[0] == pytest.approx(0.5 * dt, rel=1e-5)
+    assert x[0] == pytest.approx(0.5 * 0.5 * dt ** 2, rel=1e-5)
+
+
+def test_direct_accel_two_body():
+    pos = np.array([[0.0, 0, 0], [10.0, 0, 0]])
+    m = np.array([1.0e10, 5.0e10])
+    G = 4.498e-12
+    a = O.direct_accel_f64(pos, m, G, 0.0)
+    assert a[0, 0] == pytest.approx(G * 5e10 / 100, rel=1e-12)
+    assert a[1, 0] == pytest.approx(-G * 1e10 / 100, rel=1e-12)
+
+
+# ---------------------------------------------------------------- multigrid pieces
+
+def test_dirichlet_solve_matches_dense_solve():
+    n, dx = 6, 0.7
+    rng = np.random.default_rng(4)
+    f = rng.normal(size=(n, n, n))
+    bnd = rng.normal(size=(n, n, n))
+    phi = O.dirichlet_solve_f64(f, bnd, dx)
+    m = n - 2
+    idx = lambda i, j, k: ((i - 1) * m + (j - 1)) * m + (k - 1)  # noqa: E731
+    A = np.zeros((m ** 3, m ** 3))
+    b = np.zeros(m ** 3)
+    for i in range(1, n - 1):
+        for j in range(1, n - 1):
+            for k in range(1, n - 1):
+                r = idx(i, j, k)
+                A[r, r] = -6.0 / dx ** 2
+                b[r] = f[i, j, k]
+                for di, dj, dk in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                    a, bb, c = i + di, j + dj, k + dk
+                    if 1 <= a <= n - 2 and 1 <= bb <= n - 2 and 1 <= c <= n - 2:
+                        A[r, idx(a, bb, c)] = 1.0 / dx ** 2
+                    else:
+                        b[r] -= bnd[a, bb, c] / dx ** 2
+    u = np.linalg.solve(A, b)
+    assert np.allclose(phi[1:-1, 1:-1, 1:-1].ravel(), u, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(phi[0], bnd[0])
+
+
+def test_restrict_and_prolong_reproduce_linear_functions():
+    nf = 16
+    nc = nf // 2
+    ii, jj, kk = np.meshgrid(*(np.arange(nf),) * 3, indexing="ij")
+    lin = (0.25 * ii - 0.5 * jj + 0.125 * kk).astype(F32)
+    coarse = O.restrict(lin)
+    ci, cj, ck = np.meshgrid(*(np.arange(nc),) * 3, indexing="ij")
+    exp = (0.25 * 2 * ci - 0.5 * 2 * cj + 0.125 * 2 * ck).astype(F32)
+    assert np.array_equal(coarse[1:-1, 1:-1, 1:-1], exp[1:-1, 1:-1, 1:-1])
+    assert not coarse[0].any() and not coarse[-1].any()
+    fine = np.zeros((nf,) * 3, F32)
+    O.prolong_add(exp, fine)
+    inner = (slice(1, nf - 2),) * 3  # fine nodes whose coarse stencil stays inside [0, nc)
+    assert np.array_equal(fine[inner], lin[inner])
+
+
+def test_rb_sweeps_and_residual_converge_to_dirichlet_solution():
+    n, dx = 10, 1.0
+    rng = np.random.default_rng(5)
+    rhs = rng.normal(size=(n, n, n)).astype(F32)
+    phi = rng.normal(size=(n, n, n)).astype(F32)
+    exact = O.dirichlet_solve_f64(rhs, phi, dx)
+    for _ in range(300):
+        O.rb_pass(phi, rhs, 1.0, 0)
+        O.rb_pass(phi, rhs, 1.0, 1)
+    assert np.abs(phi - exact).max() < 1e-5 * np.abs(exact).max()
+    r = O.residual(exact.astype(F32), rhs, 1.0)
+    assert np.abs(r).max() < 1e-4 * np.abs(rhs).max() and not r[0].any()
+
+
+def test_face_values_match_float64_multipole():
+    n, dx = 12, 1.5
+    p = np.array([2.0, 3.0e3, 7.1, 8.2, 6.3, 40.0, 30.0, 20.0, 2.0, -3.0, 1.0], F32)
+    i, j, k = np.array([0, 11, 5]), np.array([3, 0, 11]), np.array([11, 4, 0])
+    got = O.face_value(i, j, k, dx, p)
+    pd = p.astype(np.float64)
+    r = np.stack([i * dx - pd[2], j * dx - pd[3], k * dx - pd[4]], 1)
+    d = np.linalg.norm(r, axis=1)
+    nv = r / d[:, None]
+    S = np.array([[pd[5], pd[8], pd[9]], [pd[8], pd[6], pd[10]], [pd[9], pd[10], pd[7]]])
+    nsn = np.einsum("ia,ab,ib->i", nv, S, nv)
+    exp = -pd[0] * (pd[1] / d + (3 * nsn - np.trace(S)) / (2 * d ** 3))
+    assert np.allclose(got, exp, rtol=2e-6)
+
+
+def test_moments_match_direct_sums():
+    n, dx = 10, 0.5
+    rng = np.random.default_rng(6)
+    rho = rng.uniform(0, 1, size=(n, n, n))
+    s = O.moments(rho, dx)
+    ii, jj, kk = (a * dx for a in np.meshgrid(*(np.arange(n),) * 3, indexing="ij"))
+    m = rho * dx ** 3
+    direct = [m.sum(), (m * ii).sum(), (m * jj).sum(), (m * kk).sum(), (m * ii * ii).sum(),
+              (m * jj * jj).sum(), (m * kk * kk).sum(), (m * ii * jj).sum(), (m * ii * kk).sum(),
+              (m * jj * kk).sum()]
+    assert np.allclose(s, direct, rtol=1e-13)
+
+
+def _gaussian_rho(n, dx, sigma, total_mass, centre=None):
+    axis = np.arange(n) * dx
+    c = np.full(3, (n - 1) / 2.0 * dx) if centre is None else np.asarray(centre)
+    xx, yy, zz = np.meshgrid(axis - c[0], axis - c[1], axis - c[2], indexing="ij")
+    g = np.exp(-(xx ** 2 + yy ** 2 + zz ** 2) / (2.0 * sigma ** 2))
+    return (g / g.sum() * total_mass / dx ** 3).astype(F32)
+
+
+def test_mg_solve_converges_to_dirichlet_problem():
+    n, dx, G = 32, 1.0, 1.0
+    rho = _gaussian_rho(n, dx, 3.0, 1.0e3)
+    phi, cycles, rhs_norm, resid = O.mg_solve(rho, dx, G, n_cycles=30, tol=0.0)
+    f = (F32(4.0 * math.pi * G) * rho).astype(F32)
+    exact = O.dirichlet_solve_f64(f, phi, dx)
+    assert cycles == 30 and resid < 1e-4 * rhs_norm
+    assert np.linalg.norm(phi - exact) / np.linalg.norm(exact) < 1e-5
+
+
+def test_mg_vcycles_reduce_residual():
+    n = 32
+    rho = _gaussian_rho(n, 1.0, 3.0, 1.0e10)
+    _, _, rn, r1 = O.mg_solve(rho, 1.0, 4.498e-12, n_cycles=1, tol=0.0)
+    _, _, _, r7 = O.mg_solve(rho, 1.0, 4.498e-12, n_cycles=7, tol=0.0)
+    assert r7 < 0.1 * r1
+
+
+def test_mg_point_mass_far_field_law():
+    n, G, mass = 48, 1.0, 1.0
+    c = n // 2
+    rho = np.zeros((n, n, n), F32)
+    rho[c, c, c] = mass
+    phi, _, rn, resid = O.mg_solve(rho, 1.0, G, n_cycles=60, tol=0.0)
+    assert resid < 1e-4 * rn
+    prev = np.inf
+    for r in (4, 6, 8, 12):
+        analytic = -G * mass / r
+        vals = [phi[c + r, c, c], phi[c - r, c, c], phi[c, c + r, c], phi[c, c - r, c],
+                phi[c, c, c + r], phi[c, c, c - r]]
+        rel = abs(float(np.mean(vals)) - analytic) / abs(analytic)
+        assert rel < 0.4 / r ** 2
+        assert rel < prev * (1.0 + 1e-4) + 1e-7
+        prev = rel
+
+
+def test_mg_matches_fft_oracle_loosely():
+    n, G = 48, 1.0
+    rho = _gaussian_rho(n, 1.0, 4.0, 2.0e3)
+    phi_mg, _, rn, resid = O.mg_solve(rho, 1.0, G, n_cycles=40)
+    phi_fft = O.fft_solve_f64(rho, 1.0, G)
+    assert resid < 1e-3 * rn
+    core = (slice(6, n - 6),) * 3
+    peak = np.abs(phi_fft).max()
+    assert np.abs(phi_mg[core] - phi_fft[core]).max() / peak < 0.03
+    assert np.abs(phi_mg[core] - phi_fft[core]).mean() / peak < 0.005
+
+
+# ---------------------------------------------------------------- FFT oracle
+
+def test_fft_point_mass_and_isotropy():
+    n, mass, G = 32, 1.0e10, 4.498e-12
+    c = n // 2
+    rho = np.zeros((n, n, n))
+    rho[c, c, c] = mass
+    phi = O.fft_solve_f64(rho, 1.0, G)
+    for r in (2, 4, 8):
+        assert phi[c + r, c, c] == pytest.approx(-G * mass / r, rel=2e-3)
+    assert phi[c + 6, c, c] == pytest.approx(phi[c, c + 6, c], rel=1e-12)
+
+
+def test_fft_matches_brute_force_convolution():
+    n, dx, G = 5, 0.8, 2.0
+    rho = np.random.default_rng(8).uniform(0, 1, size=(n, n, n))
+    phi = O.fft_solve_f64(rho, dx, G)
+    idx = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    d = np.linalg.norm(idx[:, None, :] - idx[None, :, :], axis=-1) * dx
+    with np.errstate(divide="ignore"):
+        k = np.where(d > 0, -G * dx ** 3 / d, 0.0)
+    assert np.allclose(phi.ravel(), k @ rho.ravel(), rtol=1e-12, atol=1e-12)
+
+
+def test_fft_gaussian_self_energy():
+    n, dx, sigma, mass, G = 48, 2.0, 8.0, 1.0, 1.0
+    axis = (np.arange(n) - (n - 1) / 2.0) * dx
+    xx, yy, zz = np.meshgrid(axis, axis, axis, indexing="ij")
+    g = np.exp(-(xx ** 2 + yy ** 2 + zz ** 2) / (2 * sigma ** 2))
+    rho = g / (g.sum() * dx ** 3) * mass
+    phi = O.fft_solve_f64(rho, dx, G)
+    w = 0.5 * (rho * phi).sum() * dx ** 3
+    assert w == pytest.approx(-G * mass ** 2 / (2 * sigma * math.sqrt(math.pi)), rel=0.01)
+
+
+# ---------------------------------------------------------------- diagnostics
+
+def test_particle_scalars_known_values():
+    pos = np.array([[1.0, 0, 0], [0, 2.0, 0]])
+    vel = np.array([[0, 1.0, 0], [-1.0, 0, 0]])
+    m = np.array([2.0, 3.0])
+    s = O.particle_scalars(pos, vel, m)
+    assert s[0] == 5.0 and s[1] == pytest.approx(0.5 * 2 + 0.5 * 3)
+    assert np.allclose(s[2:5], [-3.0, 2.0, 0.0])
+    assert np.allclose(s[5:8], [0, 0, 2.0 * 1.0 + 3.0 * 2.0])
+    assert np.allclose(s[8:11], [2.0, 6.0, 0.0])
+
+
+# ---------------------------------------------------------------- C baseline port
+
+def test_cpu_port_kernels_match_numpy_restatement():
+    rng = np.random.default_rng(9)
+    n, dx, npart = 20, 0.9, 3000
+    pos = rng.uniform(-1, n * dx, size=(npart, 3)).astype(F32)
+    vel = rng.normal(size=(npart, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2, npart).astype(F32)
+    chain = O.CpuChain(pos, vel, mass, n, dx, 1.0, threads=2)
+    O.lib().cpu_deposit_atomic(O._fp(chain.x), O._fp(chain.y), O._fp(chain.z), O._fp(chain.m),
+                               npart, n, dx, O._fp(chain.rho))
+    ref = O.deposit_cic_f64(pos, mass, n, dx)
+    assert np.abs(chain.rho - ref).max() <= 2e-6 * np.abs(ref).max()
+    phi = rng.normal(size=(n, n, n)).astype(F32)
+    chain.phi[...] = phi
+    O.lib().cpu_grad(O._fp(chain.phi), O._fp(chain.gx), O._fp(chain.gy), O._fp(chain.gz), n, dx)
+    gx, gy, gz = O.grad(phi, dx)
+    assert np.array_equal(chain.gx, gx) and np.array_equal(chain.gz, gz)
+    O.lib().cpu_gather(O._fp(chain.x), O._fp(chain.y), O._fp(chain.z), O._fp(chain.gx),
+                       O._fp(chain.gy), O._fp(chain.gz), O._fp(chain.ax), O._fp(chain.ay),
+                       O._fp(chain.az), npart, n, dx)
+    ax, ay, az = O.gather(pos, gx, gy, gz, dx)
+    assert np.array_equal(chain.ax, ax) and np.array_equal(chain.ay, ay)
+    chain.close()
+
+
+def test_cpu_port_mg_matches_numpy_mg():
+    n, G = 32, 1.0
+    rho = _gaussian_rho(n, 1.0, 3.0, 1.0e3, centre=(14.0, 17.5, 15.2))
+    phi_np, cyc_np, _, _ = O.mg_solve(rho, 1.0, G)
+    L = O.lib()
+    h = L.cpu_mg_create(n, 1.0, G, 20, 2, 2, 40, 1, 1e-3)
+    phi_c = np.zeros((n, n, n), F32)
+    cyc_c = L.cpu_mg_solve(h, O._fp(np.ascontiguousarray(rho)), O._fp(phi_c), 0)
+    L.cpu_mg_destroy(h)
+    assert cyc_c == cyc_np
+    assert np.array_equal(phi_c, phi_np)
