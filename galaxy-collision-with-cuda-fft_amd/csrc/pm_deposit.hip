@@ -17,6 +17,7 @@
 
 #include "pm_cic.h"
 #include "pm_common.h"
+#include "pm_timing.h"
 
 namespace {
 
@@ -26,7 +27,7 @@ constexpr int kSortItems = 16;       // keys per lane
 constexpr int kTile = kSortWaves * 64 * kSortItems;  // 4096 keys per radix tile
 constexpr int kMaxDigitBits = 8;
 constexpr int kSmallPieces = 8;      // open cells with <= this many pieces finish in one thread
-constexpr int kBigBatch = 256;       // pieces per LDS batch for large cells
+constexpr int kBigBatch = 1024;      // pieces per batch for large cells
 
 struct BigCell {
   int64_t slot;   // sorted position of the cell's first particle
@@ -58,19 +59,32 @@ __global__ void __launch_bounds__(256) key_kernel(
 // ---------------------------------------------------------------------------------------
 // 2. stable LSD radix sort (reduce-then-scan; no inter-workgroup waiting)
 // ---------------------------------------------------------------------------------------
-// Per-tile digit histogram, stored digit-major: hist[d * tiles + tile].
+// Per-tile digit histogram, stored digit-major: hist[d * tiles + tile]. Lanes holding the
+// same digit are grouped with one ballot per digit bit and only the lowest such lane adds
+// the group's count, so nearly sorted keys do not serialize on one LDS counter.
 __global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restrict__ keys,
                                                          int64_t np, int tiles, int shift,
                                                          int bits, uint32_t* __restrict__ hist) {
   __shared__ uint32_t cnt[1 << kMaxDigitBits];
   const int buckets = 1 << bits;
   const uint32_t mask = static_cast<uint32_t>(buckets - 1);
+  const int lane = threadIdx.x & 63;
   for (int d = threadIdx.x; d < buckets; d += blockDim.x) cnt[d] = 0;
   __syncthreads();
+  const uint64_t lanes_below = (1ull << lane) - 1ull;
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
   for (int e = 0; e < kTile / 256; ++e) {
     const int64_t p = base + e * 256 + threadIdx.x;
-    if (p < np) atomicAdd(&cnt[(keys[p] >> shift) & mask], 1u);
+    const bool valid = p < np;
+    const uint32_t d = valid ? (keys[p] >> shift) & mask : 0u;
+    uint64_t match = __ballot(valid);
+    for (int bb = 0; bb < bits; ++bb) {
+      const bool bit = (d >> bb) & 1u;
+      const uint64_t bm = __ballot(bit);
+      match &= bit ? bm : ~bm;
+    }
+    if (valid && (match & lanes_below) == 0)
+      atomicAdd(&cnt[d], static_cast<uint32_t>(__popcll(match)));
   }
   __syncthreads();
   for (int d = threadIdx.x; d < buckets; d += blockDim.x)
@@ -221,44 +235,79 @@ __device__ __forceinline__ void corner_terms(const float4 r, float t[8]) {
   }
 }
 
-// One thread per chunk: sequential piece sums. Complete cells are written straight to
-// sums/map; a chunk's continuation piece goes to head[chunk], its open last piece to
-// tail[chunk] (with tail_slot = sorted position of that cell's first particle).
-__global__ void __launch_bounds__(256) piece_kernel(
+// One thread per chunk: sequential piece sums. A workgroup stages its kPieceThreads
+// chunks (keys and the gathered particle records) in LDS with coalesced loads; the records
+// are stored structure-of-arrays with one pad slot per chunk so that the 64 lanes of a
+// wave, each walking its own chunk, hit 64 different banks. Complete cells are written
+// straight to sums/map; a chunk's continuation piece goes to head[chunk], its open last
+// piece to tail[chunk] (tail_slot = sorted position of that cell's first particle).
+constexpr int kPieceThreads = 128;
+constexpr int kPieceSpan = kPieceThreads * kChunk;       // sorted positions per workgroup
+constexpr int kPiecePadded = kPieceThreads * (kChunk + 1);
+
+__global__ void __launch_bounds__(kPieceThreads) piece_kernel(
     const uint32_t* __restrict__ skey, const int32_t* __restrict__ sidx,
     const float4* __restrict__ rec, int64_t np, uint32_t sentinel, int64_t chunks,
     float* __restrict__ sums, int64_t cap, int32_t* __restrict__ map, float* __restrict__ head,
     float* __restrict__ tail, int64_t* __restrict__ tail_slot) {
-  const int64_t ch = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  __shared__ uint32_t s_key[kPiecePadded + 1];
+  __shared__ float s_fx[kPiecePadded], s_fy[kPiecePadded], s_fz[kPiecePadded], s_m[kPiecePadded];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kPieceSpan;
+  // stage: position base+q -> padded slot q + q/kChunk; s_key[kPiecePadded] = key after span
+  for (int q = threadIdx.x; q < kPieceSpan; q += blockDim.x) {
+    const int64_t g = base + q;
+    const int slot = q + q / kChunk;
+    if (g < np) {
+      s_key[slot] = skey[g];
+      const float4 r = rec[sidx[g]];
+      s_fx[slot] = r.x;
+      s_fy[slot] = r.y;
+      s_fz[slot] = r.z;
+      s_m[slot] = r.w;
+    } else {
+      s_key[slot] = 0xFFFFFFFFu;
+    }
+  }
+  if (threadIdx.x == 0)
+    s_key[kPiecePadded] = base + kPieceSpan < np ? skey[base + kPieceSpan] : 0xFFFFFFFFu;
+  __syncthreads();
+  const int64_t ch = static_cast<int64_t>(blockIdx.x) * kPieceThreads + threadIdx.x;
   if (ch >= chunks) return;
   const int64_t s = ch * kChunk;
-  const int64_t e = s + kChunk < np ? s + kChunk : np;
-  const uint32_t prev = s > 0 ? skey[s - 1] : 0xFFFFFFFFu;
+  const int e = static_cast<int>((s + kChunk < np ? s + kChunk : np) - s);  // positions here
+  const int row = threadIdx.x * (kChunk + 1);
+  const uint32_t prev = s == 0 ? 0xFFFFFFFFu
+                               : (threadIdx.x == 0 ? skey[s - 1] : s_key[row - 2]);
+  // key just after this chunk (next thread's first, or the staged one after the span)
+  const uint32_t after = e < kChunk ? 0xFFFFFFFFu
+                                    : (threadIdx.x + 1 < kPieceThreads ? s_key[row + kChunk + 1]
+                                                                       : s_key[kPiecePadded]);
   int64_t open_slot = -1;
-  int64_t i = s;
+  int i = 0;
   while (i < e) {
-    const uint32_t key = skey[i];
+    const uint32_t key = s_key[row + i];
     if (key >= sentinel) break;
     float acc[8];
-    corner_terms(rec[sidx[i]], acc);
-    int64_t j = i + 1;
-    while (j < e && skey[j] == key) {
+    corner_terms(make_float4(s_fx[row + i], s_fy[row + i], s_fz[row + i], s_m[row + i]), acc);
+    int j = i + 1;
+    while (j < e && s_key[row + j] == key) {
       float t[8];
-      corner_terms(rec[sidx[j]], t);
+      corner_terms(make_float4(s_fx[row + j], s_fy[row + j], s_fz[row + j], s_m[row + j]), t);
 #pragma unroll
       for (int c = 0; c < 8; ++c) acc[c] = acc[c] + t[c];
       ++j;
     }
-    const bool starts = (i > s) || (prev != key);
-    const bool ends = (j < e) || (j >= np) || (skey[j] != key);
+    const bool starts = (i > 0) || (prev != key);
+    const bool ends = (j < e) || (after != key);
+    const int64_t gi = s + i;
     if (starts && ends) {
 #pragma unroll
-      for (int c = 0; c < 8; ++c) sums[c * cap + i] = acc[c];
-      map[key] = static_cast<int32_t>(i);
+      for (int c = 0; c < 8; ++c) sums[c * cap + gi] = acc[c];
+      map[key] = static_cast<int32_t>(gi);
     } else if (starts) {
 #pragma unroll
       for (int c = 0; c < 8; ++c) tail[ch * 8 + c] = acc[c];
-      open_slot = i;
+      open_slot = gi;
     } else {
 #pragma unroll
       for (int c = 0; c < 8; ++c) head[ch * 8 + c] = acc[c];
@@ -324,54 +373,95 @@ __global__ void __launch_bounds__(256) join_kernel(
   map[key] = static_cast<int32_t>(slot);
 }
 
-// One workgroup per large cell: aligned pairwise tree over batches of kBigBatch pieces in
-// LDS; batch results are merged by a binary-counter cascade (push, merge while the two
+// One workgroup (4 waves) per cell with more than kSmallPieces pieces. Batches of
+// kBigBatch = 1024 pieces, aligned to the cell's first piece: each lane reduces 4
+// consecutive pieces, the wave continues the aligned pairwise tree with shuffles (lane l
+// absorbs lane l+s when that lane holds pieces), and the 4 wave results are combined the
+// same way. Batch results are merged by a binary-counter cascade (push; merge while the two
 // top partials cover equal piece counts) and a final right-nested combine, which is the
-// same aligned tree one level up. All control flow is uniform across the workgroup.
+// same aligned tree one level up.
+__device__ __forceinline__ void load_piece(const float* __restrict__ head,
+                                           const float* __restrict__ tail, int64_t chunk,
+                                           int64_t q, float v[8]) {
+  const float* src = q == 0 ? tail + chunk * 8 : head + (chunk + q) * 8;
+  const float4 a = *reinterpret_cast<const float4*>(src);
+  const float4 b = *reinterpret_cast<const float4*>(src + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
 __global__ void __launch_bounds__(256) big_cell_kernel(
     const uint32_t* __restrict__ skey, const BigCell* __restrict__ big,
     const uint32_t* __restrict__ big_count, const float* __restrict__ head,
     const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
     int32_t* __restrict__ map) {
   constexpr int kMaxDepth = 40;
-  __shared__ float buf[kBigBatch][9];  // +1 pad against bank conflicts
+  __shared__ float s_wave[4][8];
   __shared__ float stack[kMaxDepth][8];
   __shared__ int64_t stack_size[kMaxDepth];
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const uint32_t count = *big_count;
-  for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
-    const BigCell cell = big[b];
+  for (uint32_t bcell = blockIdx.x; bcell < count; bcell += gridDim.x) {
+    const BigCell cell = big[bcell];
     int depth = 0;
     for (int64_t base = 0; base < cell.pieces; base += kBigBatch) {
       const int64_t m = cell.pieces - base < kBigBatch ? cell.pieces - base : kBigBatch;
-      // load pieces [base, base+m): piece 0 is the tail of the cell's first chunk
-      for (int t = tid; t < kBigBatch * 8; t += blockDim.x) {
-        const int q = t >> 3, c = t & 7;
-        float v = 0.0f;
-        if (q < m) {
-          const int64_t g = base + q;
-          v = g == 0 ? tail[cell.chunk * 8 + c] : head[(cell.chunk + g) * 8 + c];
+      // lane holds pieces base + 256*wave + 4*lane + {0..3}
+      const int64_t q0 = 256 * wave + 4 * lane;  // relative to base
+      float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (q0 < m) {
+        float w[8];
+        load_piece(head, tail, cell.chunk, base + q0, v);
+        if (q0 + 1 < m) {
+          load_piece(head, tail, cell.chunk, base + q0 + 1, w);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
         }
-        buf[q][c] = v;
+        if (q0 + 2 < m) {
+          float u[8];
+          load_piece(head, tail, cell.chunk, base + q0 + 2, w);
+          if (q0 + 3 < m) {
+            load_piece(head, tail, cell.chunk, base + q0 + 3, u);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) w[c] = w[c] + u[c];
+          }
+#pragma unroll
+          for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
+        }
+      }
+#pragma unroll
+      for (int sft = 1; sft < 64; sft <<= 1) {
+        const bool take = (lane % (2 * sft)) == 0 && (q0 + 4 * sft) < m;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const float o = __shfl_down(v[c], sft, 64);
+          if (take) v[c] = v[c] + o;
+        }
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s_wave[wave][c] = v[c];
       }
       __syncthreads();
-      for (int s = 1; s < kBigBatch; s <<= 1) {
-        for (int t = tid; t < kBigBatch * 8; t += blockDim.x) {
-          const int l = t >> 3, c = t & 7;
-          if ((l % (2 * s)) == 0 && l + s < m) buf[l][c] = buf[l][c] + buf[l + s][c];
-        }
-        __syncthreads();
-      }
       int d = depth;
       int64_t sz = m;
-      float val = tid < 8 ? buf[0][tid] : 0.0f;
+      float val = 0.0f;
+      if (tid < 8) {
+        const int c = tid;
+        float w0 = s_wave[0][c], w2 = s_wave[2][c];
+        if (256 < m) w0 = w0 + s_wave[1][c];
+        if (768 < m) w2 = w2 + s_wave[3][c];
+        val = 512 < m ? w0 + w2 : w0;
+      }
       while (d > 0 && stack_size[d - 1] == sz) {
         if (tid < 8) val = stack[d - 1][tid] + val;
         sz *= 2;
         --d;
       }
       if (tid < 8) stack[d][tid] = val;
-      __syncthreads();  // every thread has read stack_size
+      __syncthreads();  // every thread has read stack_size and s_wave
       if (tid == 0) stack_size[d] = sz;
       depth = d + 1;
       __syncthreads();
@@ -558,24 +648,33 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   const bool timing = plan->timing;
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[0], stream));
 
-  fill_i32_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(plan->map, cells, -1);
-  PM_CHECK_LAUNCH("fill_i32_kernel");
+  {
+    PM_TIMED("deposit.fill_map", 4.0 * cells, stream);
+    fill_i32_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(plan->map, cells, -1);
+    PM_CHECK_LAUNCH("fill_i32_kernel");
+  }
   if (np == 0) {
     if (timing) {
       for (int q = 1; q < 5; ++q) PM_TRY_HIP(hipEventRecord(plan->ev[q], stream));
     }
-    assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
-                                                                       plan->cap, n, rho);
-    PM_CHECK_LAUNCH("assemble_kernel");
+    {
+      PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
+      assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
+                                                                         plan->cap, n, rho);
+      PM_CHECK_LAUNCH("assemble_kernel");
+    }
     return PM_OK;
   }
 
   // 1. keys
   const float inv_vol = 1.0f / ((dx * dx) * dx);
-  key_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
-      px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel, plan->keys[0], plan->vals[0],
-      plan->rec, keys_out);
-  PM_CHECK_LAUNCH("key_kernel");
+  {
+    PM_TIMED("deposit.key", 40.0 * np, stream);
+    key_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+        px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel, plan->keys[0], plan->vals[0],
+        plan->rec, keys_out);
+    PM_CHECK_LAUNCH("key_kernel");
+  }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[1], stream));
 
   // 2. sort
@@ -584,18 +683,30 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   for (int s = 0; s < plan->passes; ++s) {
     const int bits = plan->digit_bits[s];
     const int buckets = 1 << bits;
-    radix_hist_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], np, tiles, shift, bits,
-                                                  plan->hist);
-    PM_CHECK_LAUNCH("radix_hist_kernel");
-    radix_scan_kernel<<<buckets, 256, 0, stream>>>(plan->hist, tiles, plan->digit);
-    PM_CHECK_LAUNCH("radix_scan_kernel");
-    radix_digit_start_kernel<<<1, 256, 0, stream>>>(plan->digit, buckets);
-    PM_CHECK_LAUNCH("radix_digit_start_kernel");
-    radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], plan->vals[src],
-                                                     plan->keys[1 - src], plan->vals[1 - src],
-                                                     plan->hist, plan->digit, np, tiles, shift,
-                                                     bits);
-    PM_CHECK_LAUNCH("radix_scatter_kernel");
+    {
+      PM_TIMED("deposit.radix_hist", 4.0 * np, stream);
+      radix_hist_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], np, tiles, shift, bits,
+                                                    plan->hist);
+      PM_CHECK_LAUNCH("radix_hist_kernel");
+    }
+    {
+      PM_TIMED("deposit.radix_scan", 8.0 * buckets * tiles, stream);
+      radix_scan_kernel<<<buckets, 256, 0, stream>>>(plan->hist, tiles, plan->digit);
+      PM_CHECK_LAUNCH("radix_scan_kernel");
+    }
+    {
+      PM_TIMED("deposit.radix_digit", 8.0 * buckets, stream);
+      radix_digit_start_kernel<<<1, 256, 0, stream>>>(plan->digit, buckets);
+      PM_CHECK_LAUNCH("radix_digit_start_kernel");
+    }
+    {
+      PM_TIMED("deposit.radix_scatter", 16.0 * np, stream);
+      radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], plan->vals[src],
+                                                       plan->keys[1 - src], plan->vals[1 - src],
+                                                       plan->hist, plan->digit, np, tiles, shift,
+                                                       bits);
+      PM_CHECK_LAUNCH("radix_scatter_kernel");
+    }
     src = 1 - src;
     shift += bits;
   }
@@ -608,25 +719,40 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
 
   // 3. segmented reduction
   const int64_t chunks = (np + kChunk - 1) / kChunk;
-  piece_kernel<<<pm::blocks_for(chunks), pm::kBlock, 0, stream>>>(
-      skey, sidx, plan->rec, np, plan->sentinel, chunks, plan->sums, plan->cap, plan->map,
-      plan->head, plan->tail, plan->tail_slot);
-  PM_CHECK_LAUNCH("piece_kernel");
-  zero_u32_kernel<<<1, 1, 0, stream>>>(plan->big_count);
-  PM_CHECK_LAUNCH("zero_u32_kernel");
-  join_kernel<<<pm::blocks_for(chunks), pm::kBlock, 0, stream>>>(
-      skey, np, chunks, plan->tail_slot, plan->head, plan->tail, plan->sums, plan->cap,
-      plan->map, plan->big, plan->big_count);
-  PM_CHECK_LAUNCH("join_kernel");
-  big_cell_kernel<<<256, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
-                                           plan->tail, plan->sums, plan->cap, plan->map);
-  PM_CHECK_LAUNCH("big_cell_kernel");
+  {
+    PM_TIMED("deposit.piece", 24.0 * np, stream);
+    piece_kernel<<<pm::blocks_for(chunks, kPieceThreads), kPieceThreads, 0, stream>>>(
+        skey, sidx, plan->rec, np, plan->sentinel, chunks, plan->sums, plan->cap, plan->map,
+        plan->head, plan->tail, plan->tail_slot);
+    PM_CHECK_LAUNCH("piece_kernel");
+  }
+  {
+    PM_TIMED("deposit.zero_count", 4.0, stream);
+    zero_u32_kernel<<<1, 1, 0, stream>>>(plan->big_count);
+    PM_CHECK_LAUNCH("zero_u32_kernel");
+  }
+  {
+    PM_TIMED("deposit.join", 8.0 * chunks, stream);
+    join_kernel<<<pm::blocks_for(chunks), pm::kBlock, 0, stream>>>(
+        skey, np, chunks, plan->tail_slot, plan->head, plan->tail, plan->sums, plan->cap,
+        plan->map, plan->big, plan->big_count);
+    PM_CHECK_LAUNCH("join_kernel");
+  }
+  {
+    PM_TIMED("deposit.big_cell", 0.0, stream);
+    big_cell_kernel<<<256, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
+                                             plan->tail, plan->sums, plan->cap, plan->map);
+    PM_CHECK_LAUNCH("big_cell_kernel");
+  }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[3], stream));
 
   // 4. nodes
-  assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
-                                                                     plan->cap, n, rho);
-  PM_CHECK_LAUNCH("assemble_kernel");
+  {
+    PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
+    assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
+                                                                       plan->cap, n, rho);
+    PM_CHECK_LAUNCH("assemble_kernel");
+  }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[4], stream));
   return PM_OK;
 }

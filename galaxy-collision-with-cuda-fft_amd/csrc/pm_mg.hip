@@ -49,6 +49,116 @@ __global__ void __launch_bounds__(256) rb_pass_kernel(float* __restrict__ phi,
   phi[c] = (s - h2 * rhs[c]) / 6.0f;
 }
 
+// One full red-black sweep (colour 0 then colour 1) in a single pass over the level, out of
+// place: phi_in is only read and every node of phi_out is written, so workgroups never see
+// each other's updates. A workgroup owns a (kSwTJ x kSwTK) tile of a (j,k) plane and marches
+// along i through a segment of kSwSeg planes, keeping four planes of phi (tile + 2-node ring) and two planes
+// of rhs (tile + 1-node ring) in LDS. Plane p+2 is loaded, the red nodes of plane p+1 are
+// updated on the tile and its 1-node ring (from old black values), then the black nodes of
+// plane p are updated on the tile (from new red values) and plane p is written back. Red
+// nodes on a ring or at a segment's first plane are recomputed by the neighbouring
+// workgroup from the same inputs, so each node gets exactly the values of two separate
+// colour passes (rb_pass_kernel), bit for bit.
+constexpr int kSwTJ = 32, kSwTK = 64, kSwSeg = 16;
+constexpr int kSwRJ = kSwTJ + 4, kSwRK = kSwTK + 4;  // ring-2 region
+constexpr int kSwHJ = kSwTJ + 2, kSwHK = kSwTK + 2;  // ring-1 region
+
+__global__ void __launch_bounds__(256) rb_sweep_kernel(const float* __restrict__ phi,
+                                                       float* __restrict__ phi_out,
+                                                       const float* __restrict__ rhs, int n,
+                                                       float h2) {
+  __shared__ float s_phi[4][kSwRJ * kSwRK];
+  __shared__ float s_rhs[2][kSwHJ * kSwHK];
+  const int k0 = blockIdx.x * kSwTK;
+  const int j0 = blockIdx.y * kSwTJ;
+  const int i0 = 1 + static_cast<int>(blockIdx.z) * kSwSeg;
+  if (i0 > n - 2) return;  // uniform across the workgroup
+  const int i1 = i0 + kSwSeg < n - 1 ? i0 + kSwSeg : n - 1;  // exclusive
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+
+  auto load_phi = [&](int plane) {
+    float* dst = s_phi[plane & 3];
+    const float* src = phi + plane * n2;
+    for (int t = threadIdx.x; t < kSwRJ * kSwRK; t += blockDim.x) {
+      const int j = j0 - 2 + t / kSwRK, k = k0 - 2 + t % kSwRK;
+      dst[t] = (j >= 0 && j < n && k >= 0 && k < n) ? src[static_cast<int64_t>(j) * n + k] : 0.0f;
+    }
+  };
+  auto load_rhs = [&](int plane) {
+    float* dst = s_rhs[plane & 1];
+    const float* src = rhs + plane * n2;
+    for (int t = threadIdx.x; t < kSwHJ * kSwHK; t += blockDim.x) {
+      const int j = j0 - 1 + t / kSwHK, k = k0 - 1 + t % kSwHK;
+      dst[t] = (j >= 0 && j < n && k >= 0 && k < n) ? src[static_cast<int64_t>(j) * n + k] : 0.0f;
+    }
+  };
+  // colour 0 nodes of `plane` on the tile and its 1-node ring
+  auto red = [&](int plane) {
+    if (plane < 1 || plane > n - 2) return;
+    float* c = s_phi[plane & 3];
+    const float* dn = s_phi[(plane - 1) & 3];
+    const float* up = s_phi[(plane + 1) & 3];
+    const float* r = s_rhs[plane & 1];
+    for (int t = threadIdx.x; t < kSwHJ * kSwHK; t += blockDim.x) {
+      const int hj = t / kSwHK, hk = t % kSwHK;
+      const int j = j0 - 1 + hj, k = k0 - 1 + hk;
+      if (j < 1 || j > n - 2 || k < 1 || k > n - 2 || ((plane + j + k) & 1) != 0) continue;
+      const int q = (hj + 1) * kSwRK + (hk + 1);
+      const float sum = up[q] + dn[q] + c[q + kSwRK] + c[q - kSwRK] + c[q + 1] + c[q - 1];
+      c[q] = (sum - h2 * r[t]) / 6.0f;
+    }
+  };
+  // colour 1 nodes of `plane` on the tile
+  auto black = [&](int plane) {
+    float* c = s_phi[plane & 3];
+    const float* dn = s_phi[(plane - 1) & 3];
+    const float* up = s_phi[(plane + 1) & 3];
+    const float* r = s_rhs[plane & 1];
+    for (int t = threadIdx.x; t < kSwTJ * kSwTK; t += blockDim.x) {
+      const int tj = t / kSwTK, tk = t % kSwTK;
+      const int j = j0 + tj, k = k0 + tk;
+      if (j < 1 || j > n - 2 || k < 1 || k > n - 2 || ((plane + j + k) & 1) != 1) continue;
+      const int q = (tj + 2) * kSwRK + (tk + 2);
+      const float sum = up[q] + dn[q] + c[q + kSwRK] + c[q - kSwRK] + c[q + 1] + c[q - 1];
+      c[q] = (sum - h2 * r[(tj + 1) * kSwHK + (tk + 1)]) / 6.0f;
+    }
+  };
+  // every node of the tile (boundary nodes pass through unchanged)
+  auto store = [&](int plane) {
+    const float* c = s_phi[plane & 3];
+    float* dst = phi_out + plane * n2;
+    for (int t = threadIdx.x; t < kSwTJ * kSwTK; t += blockDim.x) {
+      const int tj = t / kSwTK, tk = t % kSwTK;
+      const int j = j0 + tj, k = k0 + tk;
+      if (j >= n || k >= n) continue;
+      dst[static_cast<int64_t>(j) * n + k] = c[(tj + 2) * kSwRK + (tk + 2)];
+    }
+  };
+
+  // prologue: planes i0-2 .. i0+1 and the red nodes of planes i0-1 and i0
+  for (int p = i0 - 2; p <= i0 + 1; ++p)
+    if (p >= 0 && p <= n - 1) load_phi(p);
+  for (int p = i0 - 1; p <= i0; ++p)
+    if (p >= 0 && p <= n - 1) load_rhs(p);
+  __syncthreads();
+  red(i0 - 1);
+  __syncthreads();
+  red(i0);
+  __syncthreads();
+  if (i0 == 1) store(0);  // the first segment also copies the i = 0 face
+  for (int p = i0; p < i1; ++p) {
+    if (p + 2 <= n - 1) load_phi(p + 2);
+    if (p + 1 <= n - 1) load_rhs(p + 1);
+    __syncthreads();
+    red(p + 1);
+    __syncthreads();
+    black(p);
+    __syncthreads();
+    store(p);
+  }
+  if (i1 == n - 1) store(n - 1);  // the last segment also copies the i = n-1 face
+}
+
 // res = rhs - lap(phi) on the interior, 0 on the boundary (multigrid.py:62-79).
 __device__ __forceinline__ float residual_at(const float* __restrict__ phi,
                                              const float* __restrict__ rhs, int64_t c,
@@ -102,7 +212,9 @@ __global__ void __launch_bounds__(256) residual_sumsq_kernel(const float* __rest
 // Full weighting (1,2,1)^3/64 at fine 2J (multigrid.py:82-97); 0 on the coarse boundary.
 // Coarse interior J in [1, nc-2] reads fine 2J-1..2J+1 <= 2nc-3 <= nf-1 for both hierarchies.
 __global__ void __launch_bounds__(256) restrict_kernel(const float* __restrict__ res_f,
-                                                       float* __restrict__ rhs_c, int nf, int nc) {
+                                                       float* __restrict__ rhs_c,
+                                                       float* __restrict__ phi_c, int nf,
+                                                       int nc) {
   const int64_t nc2 = static_cast<int64_t>(nc) * nc;
   const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c >= nc2 * nc) return;
@@ -128,6 +240,7 @@ __global__ void __launch_bounds__(256) restrict_kernel(const float* __restrict__
     out = acc / 64.0f;
   }
   rhs_c[c] = out;
+  if (phi_c) phi_c[c] = 0.0f;  // the coarse correction starts from zero (multigrid.py:437)
 }
 
 // Trilinear prolongation of the coarse correction added into the fine interior
@@ -255,6 +368,178 @@ __global__ void __launch_bounds__(256) coarse_solve_kernel(float* __restrict__ p
 }
 
 // ---------------------------------------------------------------------------------------
+// Bottom solver: the V-cycle of every level of size <= kBottomMax in one workgroup, with
+// all of those levels resident in LDS. Each phase repeats the per-node arithmetic of the
+// global kernels above (RB pass, residual + restriction, prolongation) in the same order,
+// so the result is bitwise that of the multi-kernel V-cycle; the residual is recomputed
+// inside the restriction instead of being stored.
+// ---------------------------------------------------------------------------------------
+constexpr int kBottomMax = 17;
+constexpr int kBottomLevels = 8;
+constexpr int kBottomThreads = 1024;
+constexpr int kCoarse64 = 5;  // coarsest levels up to this size are solved in float64 (MIXED)
+
+struct BottomDesc {
+  int levels;
+  int n[kBottomLevels];
+  int off[kBottomLevels];  // LDS offset (floats) of phi; rhs follows at off + n^3
+  float h2[kBottomLevels];
+  float inv_h2[kBottomLevels];
+  double h2_coarsest;
+  int pre, post, coarse_sweeps, coarse64_sweeps;
+};
+
+__device__ __forceinline__ void lds_rb_pass(float* phi, const float* rhs, int n, float h2,
+                                            int color) {
+  const int half = (n + 1) >> 1;
+  const int pairs = n * n * half;
+  const int n2 = n * n;
+  for (int t = threadIdx.x; t < pairs; t += blockDim.x) {
+    const int kk = t % half;
+    const int j = (t / half) % n;
+    const int i = t / (half * n);
+    const int k = 2 * kk + ((i + j + color) & 1);
+    if (i < 1 || i > n - 2 || j < 1 || j > n - 2 || k < 1 || k > n - 2) continue;
+    const int c = i * n2 + j * n + k;
+    const float sum = phi[c + n2] + phi[c - n2] + phi[c + n] + phi[c - n] + phi[c + 1] + phi[c - 1];
+    phi[c] = (sum - h2 * rhs[c]) / 6.0f;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void lds_smooth(float* phi, const float* rhs, int n, float h2,
+                                           int sweeps) {
+  for (int q = 0; q < sweeps; ++q) {
+    lds_rb_pass(phi, rhs, n, h2, 0);
+    lds_rb_pass(phi, rhs, n, h2, 1);
+  }
+}
+
+__device__ __forceinline__ float lds_residual(const float* phi, const float* rhs, int n, int i,
+                                              int j, int k, float inv_h2) {
+  if (i < 1 || i > n - 2 || j < 1 || j > n - 2 || k < 1 || k > n - 2) return 0.0f;
+  const int n2 = n * n;
+  const int c = i * n2 + j * n + k;
+  const float lap = (phi[c + n2] + phi[c - n2] + phi[c + n] + phi[c - n] + phi[c + 1] +
+                     phi[c - 1] - 6.0f * phi[c]) *
+                    inv_h2;
+  return rhs[c] - lap;
+}
+
+template <bool kMixed>
+__global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __restrict__ phi_top,
+                                                                       const float* __restrict__ rhs_top,
+                                                                       BottomDesc d) {
+  extern __shared__ float lds[];
+  __shared__ double s64_phi[kCoarse64 * kCoarse64 * kCoarse64];
+  __shared__ double s64_rhs[kCoarse64 * kCoarse64 * kCoarse64];
+  const int L = d.levels;
+  {
+    const int nn = d.n[0] * d.n[0] * d.n[0];
+    float* phi = lds + d.off[0];
+    float* rhs = phi + nn;
+    for (int c = threadIdx.x; c < nn; c += blockDim.x) {
+      phi[c] = phi_top[c];
+      rhs[c] = rhs_top[c];
+    }
+    __syncthreads();
+  }
+  // down
+  for (int l = 0; l + 1 < L; ++l) {
+    const int nf = d.n[l], nc = d.n[l + 1];
+    float* phi = lds + d.off[l];
+    float* rhs = phi + nf * nf * nf;
+    lds_smooth(phi, rhs, nf, d.h2[l], d.pre);
+    float* cphi = lds + d.off[l + 1];
+    float* crhs = cphi + nc * nc * nc;
+    const int ncc = nc * nc * nc;
+    for (int c = threadIdx.x; c < ncc; c += blockDim.x) {
+      const int kc = c % nc, jc = (c / nc) % nc, ic = c / (nc * nc);
+      float out = 0.0f;
+      if (ic >= 1 && ic <= nc - 2 && jc >= 1 && jc <= nc - 2 && kc >= 1 && kc <= nc - 2) {
+        float acc = 0.0f;
+        for (int a = -1; a <= 1; ++a)
+          for (int b = -1; b <= 1; ++b)
+            for (int e = -1; e <= 1; ++e) {
+              const float w = (a == 0 ? 2.0f : 1.0f) * (b == 0 ? 2.0f : 1.0f) * (e == 0 ? 2.0f : 1.0f);
+              acc += w * lds_residual(phi, rhs, nf, 2 * ic + a, 2 * jc + b, 2 * kc + e, d.inv_h2[l]);
+            }
+        out = acc / 64.0f;
+      }
+      crhs[c] = out;
+      cphi[c] = 0.0f;
+    }
+    __syncthreads();
+  }
+  // coarsest
+  {
+    const int nc = d.n[L - 1];
+    float* phi = lds + d.off[L - 1];
+    float* rhs = phi + nc * nc * nc;
+    if (kMixed && nc <= kCoarse64) {
+      const int nn = nc * nc * nc, n2 = nc * nc, half = (nc + 1) >> 1;
+      for (int c = threadIdx.x; c < nn; c += blockDim.x) {
+        s64_phi[c] = phi[c];
+        s64_rhs[c] = rhs[c];
+      }
+      __syncthreads();
+      for (int q = 0; q < d.coarse64_sweeps; ++q) {
+        for (int color = 0; color < 2; ++color) {
+          for (int t = threadIdx.x; t < nc * nc * half; t += blockDim.x) {
+            const int kk = t % half, j = (t / half) % nc, i = t / (half * nc);
+            const int k = 2 * kk + ((i + j + color) & 1);
+            if (i < 1 || i > nc - 2 || j < 1 || j > nc - 2 || k < 1 || k > nc - 2) continue;
+            const int c = i * n2 + j * nc + k;
+            const double sum = s64_phi[c + n2] + s64_phi[c - n2] + s64_phi[c + nc] +
+                               s64_phi[c - nc] + s64_phi[c + 1] + s64_phi[c - 1];
+            s64_phi[c] = (sum - d.h2_coarsest * s64_rhs[c]) / 6.0;
+          }
+          __syncthreads();
+        }
+      }
+      for (int c = threadIdx.x; c < nn; c += blockDim.x) phi[c] = static_cast<float>(s64_phi[c]);
+      __syncthreads();
+    } else {
+      lds_smooth(phi, rhs, nc, d.h2[L - 1], d.coarse_sweeps);
+    }
+  }
+  // up
+  for (int l = L - 2; l >= 0; --l) {
+    const int nf = d.n[l], nc = d.n[l + 1];
+    float* phi = lds + d.off[l];
+    float* rhs = phi + nf * nf * nf;
+    const float* cphi = lds + d.off[l + 1];
+    const int nf2 = nf * nf, nc2 = nc * nc;
+    for (int c = threadIdx.x; c < nf2 * nf; c += blockDim.x) {
+      const int k = c % nf, j = (c / nf) % nf, i = c / nf2;
+      if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) continue;
+      const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
+      const int oi = i & 1, oj = j & 1, ok = k & 1;
+      float val = 0.0f;
+      for (int di = 0; di < 2; ++di) {
+        const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
+        for (int dj = 0; dj < 2; ++dj) {
+          const float wj = oj ? 0.5f : (dj == 0 ? 1.0f : 0.0f);
+          for (int dk = 0; dk < 2; ++dk) {
+            const float wk = ok ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
+            const float w = wi * wj * wk;
+            if (w > 0.0f) val += w * cphi[(ci + di) * nc2 + (cj + dj) * nc + (ck + dk)];
+          }
+        }
+      }
+      phi[c] = phi[c] + val;
+    }
+    __syncthreads();
+    lds_smooth(phi, rhs, nf, d.h2[l], d.post);
+  }
+  {
+    const int nn = d.n[0] * d.n[0] * d.n[0];
+    const float* phi = lds + d.off[0];
+    for (int c = threadIdx.x; c < nn; c += blockDim.x) phi_top[c] = phi[c];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // float64 defect-correction kernels (MIXED mode)
 // ---------------------------------------------------------------------------------------
 
@@ -340,6 +625,7 @@ struct Level {
   float* phi = nullptr;  // level 0: the solve's phi (caller's in PLAIN, e0 in MIXED)
   float* rhs = nullptr;  // level 0: 4*pi*G*rho (PLAIN) or the float32 defect (MIXED)
   float* res = nullptr;
+  float* tmp = nullptr;  // ping-pong partner of phi for the fused sweep (large levels only)
   int64_t nodes() const { return static_cast<int64_t>(n) * n * n; }
 };
 
@@ -363,7 +649,7 @@ std::vector<int> level_sizes(int n, int mode) {
 // Algorithmic bytes per launch (compulsory traffic per node, float32 fields).
 constexpr double kRbBytes = 12.0;        // read phi + rhs, write phi (per colour pass)
 constexpr double kResidualBytes = 12.0;  // read phi + rhs, write res
-constexpr double kRestrictBytes = 4.5;   // per fine node: read res, write 1/8 coarse rhs
+constexpr double kRestrictBytes = 5.0;   // per fine node: read res, write 1/8 coarse rhs + phi
 constexpr double kProlongBytes = 8.5;    // per fine node: read+write phi, read 1/8 coarse
 constexpr double kFillBytes = 4.0;
 constexpr double kRes64Bytes = 16.0;     // read phi64 + f, write r32
@@ -384,6 +670,7 @@ struct pm_mg {
   double* moments = nullptr;  // 10 raw moments
   float* face_params = nullptr;
   float* last_phi = nullptr;  // caller phi of the last solve
+  int bottom_from = -1;       // first level handled by the one-workgroup bottom solver
   bool solved = false;
   std::vector<void*> owned;
 };
@@ -420,9 +707,32 @@ pm_status rb_pass(float* phi, const float* rhs, int n, float h2, int color, hipS
   return PM_OK;
 }
 
+// Levels at least this large would use the fused sweep in the solve. Measured on MI355X at
+// 256^3 the streaming sweep (221 us) is slower than two colour passes (2 x 54 us): its
+// three barriers per plane leave it latency-bound, so the solve keeps the colour passes.
+constexpr int kSweepMin = 1 << 30;
+
+pm_status rb_sweep(const float* phi_in, float* phi_out, const float* rhs, int n, float h2,
+                   hipStream_t s) {
+  const dim3 grid((n + kSwTK - 1) / kSwTK, (n + kSwTJ - 1) / kSwTJ, (n - 2 + kSwSeg - 1) / kSwSeg);
+  PM_TIMED(level_name("mg.rb_sweep", n), 12.0 * n * n * n, s);
+  rb_sweep_kernel<<<grid, 256, 0, s>>>(phi_in, phi_out, rhs, n, h2);
+  PM_CHECK_LAUNCH("rb_sweep_kernel");
+  return PM_OK;
+}
+
+// `sweeps` red-black sweeps in place on L.phi. Large levels ping-pong pairs of fused sweeps
+// through L.tmp (ending back in L.phi); a leftover single sweep uses two colour passes.
 pm_status smooth(const Level& L, int sweeps, hipStream_t s) {
-  for (int q = 0; q < sweeps; ++q) {
-    pm_status st;
+  pm_status st;
+  int q = 0;
+  if (L.tmp) {
+    for (; q + 2 <= sweeps; q += 2) {
+      if ((st = rb_sweep(L.phi, L.tmp, L.rhs, L.n, L.h2, s))) return st;
+      if ((st = rb_sweep(L.tmp, L.phi, L.rhs, L.n, L.h2, s))) return st;
+    }
+  }
+  for (; q < sweeps; ++q) {
     if ((st = rb_pass(L.phi, L.rhs, L.n, L.h2, 0, s))) return st;
     if ((st = rb_pass(L.phi, L.rhs, L.n, L.h2, 1, s))) return st;
   }
@@ -454,8 +764,40 @@ pm_status coarsest(pm_mg* mg, const Level& L, hipStream_t s) {
   return smooth(L, mg->d.coarse_sweeps, s);
 }
 
+// Levels l..end in one workgroup (all of them are <= kBottomMax).
+pm_status bottom(pm_mg* mg, size_t l, hipStream_t s) {
+  BottomDesc d{};
+  d.levels = static_cast<int>(mg->lv.size() - l);
+  int off = 0;
+  double bytes = 0.0;
+  for (int q = 0; q < d.levels; ++q) {
+    const Level& L = mg->lv[l + q];
+    d.n[q] = L.n;
+    d.h2[q] = L.h2;
+    d.inv_h2[q] = L.inv_h2;
+    d.off[q] = off;
+    off += 2 * L.n * L.n * L.n;
+  }
+  const Level& top = mg->lv[l];
+  d.h2_coarsest = mg->lv.back().h * mg->lv.back().h;
+  d.pre = mg->d.pre_sweeps;
+  d.post = mg->d.post_sweeps;
+  d.coarse_sweeps = mg->d.coarse_sweeps;
+  d.coarse64_sweeps = 60;
+  bytes = 12.0 * top.nodes();  // read phi + rhs, write phi of the top level
+  const size_t lds = sizeof(float) * static_cast<size_t>(off);
+  PM_TIMED(level_name("mg.bottom", top.n), bytes, s);
+  if (mg->d.mode == PM_MG_MIXED)
+    bottom_vcycle_kernel<true><<<1, kBottomThreads, lds, s>>>(top.phi, top.rhs, d);
+  else
+    bottom_vcycle_kernel<false><<<1, kBottomThreads, lds, s>>>(top.phi, top.rhs, d);
+  PM_CHECK_LAUNCH("bottom_vcycle_kernel");
+  return PM_OK;
+}
+
 pm_status vcycle(pm_mg* mg, size_t l, hipStream_t s) {
   const Level& L = mg->lv[l];
+  if (mg->bottom_from >= 0 && static_cast<int>(l) >= mg->bottom_from) return bottom(mg, l, s);
   if (l + 1 == mg->lv.size()) return coarsest(mg, L, s);
   pm_status st;
   if ((st = smooth(L, mg->d.pre_sweeps, s))) return st;
@@ -468,10 +810,10 @@ pm_status vcycle(pm_mg* mg, size_t l, hipStream_t s) {
   const Level& C = mg->lv[l + 1];
   {
     PM_TIMED(level_name("mg.restrict", L.n), kRestrictBytes * L.nodes(), s);
-    restrict_kernel<<<pm::blocks_for(C.nodes()), pm::kBlock, 0, s>>>(L.res, C.rhs, L.n, C.n);
+    restrict_kernel<<<pm::blocks_for(C.nodes()), pm::kBlock, 0, s>>>(L.res, C.rhs, C.phi, L.n,
+                                                                      C.n);
     PM_CHECK_LAUNCH("restrict_kernel");
   }
-  if ((st = fill(C.phi, C.nodes(), 0.0f, s))) return st;
   if ((st = vcycle(mg, l + 1, s))) return st;
   {
     PM_TIMED(level_name("mg.prolong", L.n), kProlongBytes * L.nodes(), s);
@@ -593,7 +935,21 @@ pm_status pm_mg_create(const pm_mg_desc* desc, pm_mg** out) {
         return st;
       }
     }
+    if (L.n >= kSweepMin) {
+      if ((st = dalloc(mg, reinterpret_cast<void**>(&L.tmp), bytes))) {
+        destroy(mg);
+        return st;
+      }
+    }
     mg->lv.push_back(L);
+  }
+  // the bottom solver takes over at the first level <= kBottomMax when every level below it
+  // is small too and the hierarchy fits its descriptor
+  for (size_t l = 0; l < mg->lv.size(); ++l) {
+    if (mg->lv[l].n <= kBottomMax && mg->lv.size() - l <= static_cast<size_t>(kBottomLevels)) {
+      mg->bottom_from = static_cast<int>(l);
+      break;
+    }
   }
   const int64_t n0 = mg->lv[0].nodes();
   if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->f0), sizeof(float) * n0)) ||
@@ -774,6 +1130,14 @@ pm_status pm_rb_pass(float* phi, const float* rhs, int32_t n, float h2, int32_t 
   return rb_pass(phi, rhs, n, h2, color, s);
 }
 
+pm_status pm_rb_sweep(const float* phi_in, float* phi_out, const float* rhs, int32_t n,
+                      float h2, hipStream_t s) {
+  PM_REQUIRE(phi_in && phi_out && rhs, "pm_rb_sweep: null pointer");
+  PM_REQUIRE(phi_in != phi_out, "pm_rb_sweep: phi_in and phi_out must differ");
+  PM_REQUIRE(n >= 3 && n <= 2048, "pm_rb_sweep: n must be in [3, 2048]");
+  return rb_sweep(phi_in, phi_out, rhs, n, h2, s);
+}
+
 pm_status pm_residual(const float* phi, const float* rhs, float* res, int32_t n, float inv_h2,
                       hipStream_t s) {
   PM_REQUIRE(phi && rhs && res, "pm_residual: null pointer");
@@ -789,7 +1153,7 @@ pm_status pm_restrict(const float* res_f, float* rhs_c, int32_t nf, int32_t nc, 
   PM_REQUIRE(nc >= 1 && nc <= 1025 && nf >= 2 * nc - 2 && nf <= 2 * nc,
              "pm_restrict: need 2*nc-2 <= nf <= 2*nc (coarse J at fine 2J)");
   const int64_t nodes = static_cast<int64_t>(nc) * nc * nc;
-  restrict_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(res_f, rhs_c, nf, nc);
+  restrict_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(res_f, rhs_c, nullptr, nf, nc);
   PM_CHECK_LAUNCH("restrict_kernel");
   return PM_OK;
 }

@@ -6,6 +6,7 @@
 // numpy float32 restatement.
 #include "pm_cic.h"
 #include "pm_common.h"
+#include "pm_timing.h"
 
 namespace {
 
@@ -233,9 +234,12 @@ pm_status pm_grad(const float* phi, float* gx, float* gy, float* gz, int32_t n, 
   const float inv_dx = 1.0f / dx;
   const float inv_2dx = 1.0f / (2.0f * dx);
   const int64_t total = static_cast<int64_t>(n) * n * n;
-  grad_kernel<<<pm::blocks_for(total), pm::kBlock, 0, stream>>>(phi, gx, gy, gz, n, inv_dx,
-                                                                 inv_2dx);
-  PM_CHECK_LAUNCH("grad_kernel");
+  {
+    PM_TIMED("grad", 16.0 * total, stream);
+    grad_kernel<<<pm::blocks_for(total), pm::kBlock, 0, stream>>>(phi, gx, gy, gz, n, inv_dx,
+                                                                   inv_2dx);
+    PM_CHECK_LAUNCH("grad_kernel");
+  }
   return PM_OK;
 }
 
@@ -247,9 +251,12 @@ pm_status pm_gather(const float* px, const float* py, const float* pz, const flo
   PM_REQUIRE(px && py && pz && gx && gy && gz && ax && ay && az, "pm_gather: null pointer");
   PM_REQUIRE(n >= 1 && n <= 2048, "pm_gather: grid_size must be in [1, 2048]");
   PM_REQUIRE(dx > 0.0f, "pm_gather: dx must be positive");
-  gather_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, gx, gy, gz, ax, ay,
-                                                                az, np, n, dx);
-  PM_CHECK_LAUNCH("gather_kernel");
+  {
+    PM_TIMED("gather", 24.0 * np + 12.0 * n * n * n, stream);
+    gather_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, gx, gy, gz, ax, ay,
+                                                                  az, np, n, dx);
+    PM_CHECK_LAUNCH("gather_kernel");
+  }
   return PM_OK;
 }
 
@@ -263,9 +270,12 @@ pm_status pm_gather_kick(const float* px, const float* py, const float* pz, cons
              "pm_gather_kick: null pointer");
   PM_REQUIRE(n >= 1 && n <= 2048, "pm_gather_kick: grid_size must be in [1, 2048]");
   PM_REQUIRE(dx > 0.0f, "pm_gather_kick: dx must be positive");
-  gather_kick_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
-      px, py, pz, gx, gy, gz, ax, ay, az, vx, vy, vz, np, n, dx, h);
-  PM_CHECK_LAUNCH("gather_kick_kernel");
+  {
+    PM_TIMED("gather_kick", 48.0 * np + 12.0 * n * n * n, stream);
+    gather_kick_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+        px, py, pz, gx, gy, gz, ax, ay, az, vx, vy, vz, np, n, dx, h);
+    PM_CHECK_LAUNCH("gather_kick_kernel");
+  }
   return PM_OK;
 }
 
@@ -274,8 +284,11 @@ pm_status pm_kick(float* vx, float* vy, float* vz, const float* ax, const float*
   PM_REQUIRE(np >= 0, "pm_kick: negative particle count");
   if (np == 0) return PM_OK;
   PM_REQUIRE(vx && vy && vz && ax && ay && az, "pm_kick: null pointer");
-  kick_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(vx, vy, vz, ax, ay, az, np, dt);
-  PM_CHECK_LAUNCH("kick_kernel");
+  {
+    PM_TIMED("kick", 36.0 * np, stream);
+    kick_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(vx, vy, vz, ax, ay, az, np, dt);
+    PM_CHECK_LAUNCH("kick_kernel");
+  }
   return PM_OK;
 }
 
@@ -284,8 +297,11 @@ pm_status pm_drift(float* px, float* py, float* pz, const float* vx, const float
   PM_REQUIRE(np >= 0, "pm_drift: negative particle count");
   if (np == 0) return PM_OK;
   PM_REQUIRE(px && py && pz && vx && vy && vz, "pm_drift: null pointer");
-  drift_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, vx, vy, vz, np, dt);
-  PM_CHECK_LAUNCH("drift_kernel");
+  {
+    PM_TIMED("drift", 36.0 * np, stream);
+    drift_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, vx, vy, vz, np, dt);
+    PM_CHECK_LAUNCH("drift_kernel");
+  }
   return PM_OK;
 }
 
@@ -295,9 +311,12 @@ pm_status pm_kick_drift(float* px, float* py, float* pz, float* vx, float* vy, f
   PM_REQUIRE(np >= 0, "pm_kick_drift: negative particle count");
   if (np == 0) return PM_OK;
   PM_REQUIRE(px && py && pz && vx && vy && vz && ax && ay && az, "pm_kick_drift: null pointer");
-  kick_drift_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, vx, vy, vz, ax,
-                                                                    ay, az, np, h, dt);
-  PM_CHECK_LAUNCH("kick_drift_kernel");
+  {
+    PM_TIMED("kick_drift", 60.0 * np, stream);
+    kick_drift_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, vx, vy, vz, ax,
+                                                                      ay, az, np, h, dt);
+    PM_CHECK_LAUNCH("kick_drift_kernel");
+  }
   return PM_OK;
 }
 
@@ -308,9 +327,12 @@ pm_status pm_direct_accel(const float* px, const float* py, const float* pz, con
   if (np == 0) return PM_OK;
   PM_REQUIRE(np <= (int64_t(1) << 24), "pm_direct_accel: O(N^2) path limited to 2^24 particles");
   PM_REQUIRE(px && py && pz && m && ax && ay && az, "pm_direct_accel: null pointer");
-  direct_accel_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, m, ax, ay, az,
-                                                                      np, grav, eps2);
-  PM_CHECK_LAUNCH("direct_accel_kernel");
+  {
+    PM_TIMED("direct_accel", 16.0 * np, stream);
+    direct_accel_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(px, py, pz, m, ax, ay, az,
+                                                                        np, grav, eps2);
+    PM_CHECK_LAUNCH("direct_accel_kernel");
+  }
   return PM_OK;
 }
 

@@ -11,7 +11,7 @@
 
 namespace pm {
 
-constexpr int kRedBlocks = 512;
+constexpr int kRedBlocks = 2048;
 constexpr int kRedMaxQ = 16;  // quantities per reduction (partials stride)
 
 inline unsigned red_blocks(int64_t count) {

@@ -109,6 +109,7 @@ _SIGNATURES = {
     "pm_mg_residual_norm": [_vp, ctypes.POINTER(_d), _vp],
     "pm_mg_moments": [_vp, _vp, ctypes.POINTER(_d), _vp],
     "pm_rb_pass": [_vp, _vp, _i32, _f, _i32, _vp],
+    "pm_rb_sweep": [_vp, _vp, _vp, _i32, _f, _vp],
     "pm_residual": [_vp, _vp, _vp, _i32, _f, _vp],
     "pm_restrict": [_vp, _vp, _i32, _i32, _vp],
     "pm_prolong_add": [_vp, _vp, _i32, _i32, _vp],

@@ -174,6 +174,11 @@ pm_status pm_mg_moments(pm_mg* mg, const float* rho, double out[10], hipStream_t
 /* one red-black colour pass (multigrid.py:46-59) */
 pm_status pm_rb_pass(float* phi, const float* rhs, int32_t n, float h2, int32_t color,
                      hipStream_t stream);
+/* one full sweep, colour 0 then colour 1, in a single streaming pass from phi_in into
+ * phi_out (every node written; distinct buffers); phi_out equals phi_in after two
+ * pm_rb_pass calls, bit for bit (multigrid.py:422-427 `_smooth` with sweeps=1) */
+pm_status pm_rb_sweep(const float* phi_in, float* phi_out, const float* rhs, int32_t n,
+                      float h2, hipStream_t stream);
 /* res = rhs - lap(phi) inside, 0 on the boundary (multigrid.py:62-79) */
 pm_status pm_residual(const float* phi, const float* rhs, float* res, int32_t n,
                       float inv_h2, hipStream_t stream);

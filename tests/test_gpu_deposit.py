@@ -93,6 +93,22 @@ def test_random_bitwise(device, npart, n, dx, dense):
     _check(device, pos, mass, n, dx)
 
 
+@pytest.mark.parametrize("dense", [129, 1000, 16_383, 16_384 + 7, 32_768 + 1000])
+def test_populous_cell_trees(device, dense):
+    """One cell with `dense` particles after a few sparse ones: its pieces span 9..2111
+    chunks with arbitrary alignment, covering the in-thread, one-batch and multi-batch
+    tree paths of the segmented reduction."""
+    announce(f"deposit populous cell dense={dense}")
+    rng = np.random.default_rng(dense)
+    n = 16
+    sparse = rng.uniform(0.0, 5.0, size=(37, 3)).astype(F32)
+    cell = rng.uniform(9.0, 9.999, size=(dense, 3)).astype(F32)
+    pos = np.concatenate([sparse, cell, rng.uniform(10.0, 15.0, size=(11, 3)).astype(F32)])
+    pos = pos[rng.permutation(len(pos))]
+    mass = rng.uniform(0.5, 2.0, len(pos)).astype(F32)
+    _check(device, pos, mass, n, 1.0)
+
+
 def test_deterministic_repeat(device):
     announce("deposit determinism")
     rng = np.random.default_rng(11)

@@ -59,6 +59,30 @@ def test_rb_pass_and_residual_bitwise(device, n):
     assert_bitwise(_np(res), O.residual(phi, rhs, 4.0))
 
 
+@pytest.mark.parametrize("n", [3, 5, 37, 70, 129, 256])
+def test_fused_sweep_equals_two_passes(device, n):
+    """pm_rb_sweep (streaming red+black in one pass) == pm_rb_pass(0) then pm_rb_pass(1)."""
+    announce(f"fused sweep n={n}")
+    from galaxy_collision_amd import _native as nat
+
+    rng = np.random.default_rng(100 + n)
+    phi = rng.normal(size=(n, n, n)).astype(F32)
+    rhs = rng.normal(size=(n, n, n)).astype(F32)
+    a, b, tr = _t(phi, device), _t(phi, device), _t(rhs, device)
+    tmp = torch.full_like(a, 9.0)
+    for _ in range(2):
+        nat.call("pm_rb_sweep", nat.ptr(a), nat.ptr(tmp), nat.ptr(tr), n, 0.3, nat.stream())
+        a, tmp = tmp, a
+        nat.call("pm_rb_pass", nat.ptr(b), nat.ptr(tr), n, 0.3, 0, nat.stream())
+        nat.call("pm_rb_pass", nat.ptr(b), nat.ptr(tr), n, 0.3, 1, nat.stream())
+    assert_bitwise(_np(a), _np(b), "sweep vs passes")
+    if n <= 70:
+        for _ in range(2):
+            O.rb_pass(phi, rhs, 0.3, 0)
+            O.rb_pass(phi, rhs, 0.3, 1)
+        assert_bitwise(_np(a), phi, "sweep vs oracle")
+
+
 @pytest.mark.parametrize("nf", [6, 16, 64])
 def test_restrict_prolong_bitwise(device, nf):
     announce(f"restrict/prolong nf={nf}")
@@ -261,3 +285,28 @@ def test_solver_factory():
     assert isinstance(make_solver("multigrid", 16), MultigridPoissonSolver)
     with pytest.raises(ValueError, match="unknown solver"):
         make_solver("nope", 16)
+
+
+@pytest.mark.parametrize("n,warm", [(128, False), (256, True)])
+def test_plain_solve_large_matches_c_port(device, n, warm):
+    """Large grids use the fused sweep and the bottom solver: the full PLAIN solve still
+    reproduces the oracle's C port of the reference solve (same cycles, phi to 1e-6)."""
+    announce(f"plain solve vs C port n={n}")
+    from galaxy_collision_amd.solver.multigrid import MultigridPoissonSolver
+
+    G, dx = 1.0, 40.0 / n
+    rho = _gaussian_rho(n, dx, 6.0 * dx, 1.0, centre=(0.45 * n * dx, 0.52 * n * dx, 0.5 * n * dx))
+    L = O.lib()
+    h = L.cpu_mg_create(n, dx, G, 20, 2, 2, 40, 1, 1e-3)
+    phi_c = np.zeros((n, n, n), F32)
+    if warm:
+        L.cpu_mg_solve(h, O._fp(np.ascontiguousarray(rho)), O._fp(phi_c), 0)
+    phi0 = phi_c.copy()
+    cycles = L.cpu_mg_solve(h, O._fp(np.ascontiguousarray(rho)), O._fp(phi_c), 1 if warm else 0)
+    L.cpu_mg_destroy(h)
+    solver = MultigridPoissonSolver(n, dx=dx, grav_constant=G)
+    tphi = _t(phi0, device)
+    solver.solve(_t(rho, device), tphi, warm_start=warm)
+    assert solver.last_cycles == cycles
+    got = _np(tphi)
+    assert np.abs(got - phi_c).max() <= 1e-6 * np.abs(phi_c).max()
