@@ -133,6 +133,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the phi accuracy check")
     ap.add_argument("--no-kernel-pass", action="store_true", help="skip the per-kernel pass")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy a 256 MiB buffer 3 times (a known-size kernel for scaling "
+                         "profiler byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -154,6 +157,13 @@ def main() -> int:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            nat.call("pm_copy_f32", nat.ptr(dst), nat.ptr(src), src.numel(), nat.stream())
+        torch.cuda.synchronize()
+        del src, dst
     w = dict(WORKLOADS[args.workload])
     solver_name = args.solver or w["solver"]
     grav = 1.0

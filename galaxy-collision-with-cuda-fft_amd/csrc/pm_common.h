@@ -41,6 +41,26 @@ inline unsigned stream_blocks(int64_t work, int per_block = kBlock) {
   return static_cast<unsigned>(b);
 }
 
+// Logical workgroup index for 1-D launches, arranged so that the workgroups one XCD runs
+// cover one contiguous range of the index space. The hardware hands workgroup w to XCD
+// w % 8; with the identity mapping a stencil's neighbouring rows would sit in other XCDs'
+// L2 caches and be fetched again from HBM. The mapping is a bijection on [0, gridDim.x),
+// so it changes only which XCD does what, never the result.
+__device__ __forceinline__ int64_t xcd_block() {
+  const unsigned nb = gridDim.x, w = blockIdx.x;
+  const unsigned per = nb >> 3, rem = nb & 7u, x = w & 7u;
+  return static_cast<int64_t>(x * per + (x < rem ? x : rem) + (w >> 3));
+}
+
+// Contiguous element range [begin, end) of this workgroup when `count` elements are split
+// into gridDim.x chunks (multiples of 256), in XCD-aware order.
+__device__ __forceinline__ void block_range(int64_t count, int64_t& begin, int64_t& end) {
+  const int64_t nb = gridDim.x;
+  const int64_t chunk = ((count + nb - 1) / nb + 255) / 256 * 256;
+  begin = xcd_block() * chunk;
+  end = begin + chunk < count ? begin + chunk : count;
+}
+
 // Correctly rounded float32 square root. The hardware estimate (v_sqrt_f32) is within one
 // ulp; each candidate is then checked against the exact midpoints to its neighbours: a
 // midpoint of two floats has at most 25 significant bits and its square at most 50, so both

@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(256) key_kernel(
     const float* __restrict__ mass, int64_t np, int n, float dx, float inv_vol,
     uint32_t sentinel, uint32_t* __restrict__ keys, int32_t* __restrict__ vals,
     float4* __restrict__ rec, uint32_t* __restrict__ keys_out) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
   int ix = 0, iy = 0, iz = 0;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
                                                        int64_t cap, int n,
                                                        float* __restrict__ rho) {
   const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= n2 * n) return;
   const int k = static_cast<int>(idx % n);
   const int j = static_cast<int>((idx / n) % n);
@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
 
 __global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* __restrict__ p, int64_t count,
                                                        int32_t value) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (i < count) p[i] = value;
 }
 

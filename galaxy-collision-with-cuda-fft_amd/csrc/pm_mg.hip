@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(256) rb_pass_kernel(float* __restrict__ phi,
                                                       float h2, int color) {
   const int half = (n + 1) >> 1;
   const int64_t total = static_cast<int64_t>(n) * n * half;
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t t = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int kk = static_cast<int>(t % half);
   const int j = static_cast<int>((t / half) % n);
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) residual_kernel(const float* __restrict__
                                                        float* __restrict__ res, int n,
                                                        float inv_h2) {
   const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c >= n2 * n) return;
   const int k = static_cast<int>(c % n);
   const int j = static_cast<int>((c / n) % n);
@@ -194,9 +194,9 @@ __global__ void __launch_bounds__(256) residual_sumsq_kernel(const float* __rest
   double v[1] = {0.0};
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const int64_t total = n2 * n;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < total;
-       c += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(total, r_begin, r_end);
+  for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
     const int k = static_cast<int>(c % n);
     const int j = static_cast<int>((c / n) % n);
     const int i = static_cast<int>(c / n2);
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(256) restrict_kernel(const float* __restrict__
                                                        float* __restrict__ phi_c, int nf,
                                                        int nc) {
   const int64_t nc2 = static_cast<int64_t>(nc) * nc;
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc2 * nc) return;
   const int kc = static_cast<int>(c % nc);
   const int jc = static_cast<int>((c / nc) % nc);
@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restric
                                                           float* __restrict__ phi_f, int nf,
                                                           int nc) {
   const int64_t nf2 = static_cast<int64_t>(nf) * nf;
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nf2 * nf) return;
   const int k = static_cast<int>(c % nf);
   const int j = static_cast<int>((c / nf) % nf);
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restric
 
 __global__ void __launch_bounds__(256) fill_f32_kernel(float* __restrict__ p, int64_t count,
                                                        float v) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (i < count) p[i] = v;
 }
 
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(256) fill_f32_kernel(float* __restrict__ p, in
 __global__ void __launch_bounds__(256) scale_kernel(const float* __restrict__ src,
                                                     float* __restrict__ dst, int64_t count,
                                                     float factor) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (i < count) dst[i] = factor * src[i];
 }
 
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(256) init64_kernel(const float* __restrict__ p
                                                      double* __restrict__ phi64, int n,
                                                      int warm) {
   const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c >= n2 * n) return;
   const int k = static_cast<int>(c % n);
   const int j = static_cast<int>((c / n) % n);
@@ -567,9 +567,9 @@ __global__ void __launch_bounds__(256) residual64_kernel(const double* __restric
   double v[1] = {0.0};
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const int64_t total = n2 * n;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < total;
-       c += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(total, r_begin, r_end);
+  for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
     const int k = static_cast<int>(c % n);
     const int j = static_cast<int>((c / n) % n);
     const int i = static_cast<int>(c / n2);
@@ -594,9 +594,9 @@ __global__ void __launch_bounds__(256) update64_kernel(double* __restrict__ phi6
                                                        double* __restrict__ partials) {
   __shared__ double s_red[4 * 2];
   double v[2] = {0.0, 0.0};
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < count;
-       c += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(count, r_begin, r_end);
+  for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
     const double d = static_cast<double>(e[c]);
     const double p = phi64[c] + d;
     phi64[c] = p;
@@ -613,7 +613,7 @@ __global__ void __launch_bounds__(256) update64_kernel(double* __restrict__ phi6
 __global__ void __launch_bounds__(256) round64_kernel(const double* __restrict__ phi64,
                                                       float* __restrict__ phi32,
                                                       int64_t count) {
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c < count) phi32[c] = static_cast<float>(phi64[c]);
 }
 

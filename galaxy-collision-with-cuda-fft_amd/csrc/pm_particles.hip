@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi
                                                    float inv_dx, float inv_2dx) {
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const int64_t total = n2 * n;
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= total) return;
   const int k = static_cast<int>(idx % n);
   const int j = static_cast<int>((idx / n) % n);
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(256) gather_kernel(
     const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gz,
     float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az, int64_t np, int n,
     float dx) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
   float a0, a1, a2;
   cic_gather(px[p], py[p], pz[p], gx, gy, gz, n, dx, a0, a1, a2);
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) gather_kick_kernel(
     float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az,
     float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz, int64_t np, int n,
     float dx, float h) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
   float a0, a1, a2;
   cic_gather(px[p], py[p], pz[p], gx, gy, gz, n, dx, a0, a1, a2);
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) kick_kernel(float* __restrict__ vx,
                                                    const float* __restrict__ ay,
                                                    const float* __restrict__ az, int64_t np,
                                                    float dt) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
   vx[p] = vx[p] + ax[p] * dt;
   vy[p] = vy[p] + ay[p] * dt;
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) drift_kernel(float* __restrict__ px,
                                                     const float* __restrict__ vy,
                                                     const float* __restrict__ vz, int64_t np,
                                                     float dt) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
   px[p] = px[p] + vx[p] * dt;
   py[p] = py[p] + vy[p] * dt;
@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(256) kick_drift_kernel(
     float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
     const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
     int64_t np, float h, float dt) {
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const float v0 = vx[p] + ax[p] * h;
   const float v1 = vy[p] + ay[p] * h;
@@ -222,9 +222,27 @@ __global__ void __launch_bounds__(256) direct_accel_kernel(
   }
 }
 
+// dst = src, one float per thread: the known-size streaming kernel used to calibrate
+// profiler byte counters.
+__global__ void __launch_bounds__(256) copy_f32_kernel(float* __restrict__ dst,
+                                                       const float* __restrict__ src,
+                                                       int64_t count) {
+  const int64_t i = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+pm_status pm_copy_f32(float* dst, const float* src, int64_t count, hipStream_t stream) {
+  PM_REQUIRE(count >= 0, "pm_copy_f32: negative count");
+  if (count == 0) return PM_OK;
+  PM_REQUIRE(dst && src, "pm_copy_f32: null pointer");
+  copy_f32_kernel<<<pm::blocks_for(count), pm::kBlock, 0, stream>>>(dst, src, count);
+  PM_CHECK_LAUNCH("copy_f32_kernel");
+  return PM_OK;
+}
 
 pm_status pm_grad(const float* phi, float* gx, float* gy, float* gz, int32_t n, float dx,
                   hipStream_t stream) {

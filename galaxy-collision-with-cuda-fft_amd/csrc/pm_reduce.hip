@@ -12,9 +12,9 @@ __global__ void __launch_bounds__(256) sumsq_f32_kernel(const float* __restrict_
                                                         double* __restrict__ partials) {
   __shared__ double s_red[4];
   double v[1] = {0.0};
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
-       i += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(count, r_begin, r_end);
+  for (int64_t i = r_begin + threadIdx.x; i < r_end; i += blockDim.x) {
     const double t = static_cast<double>(x[i]);
     v[0] = v[0] + t * t;
   }
@@ -27,9 +27,9 @@ __global__ void __launch_bounds__(256) sumsq_f64_kernel(const double* __restrict
                                                         double* __restrict__ partials) {
   __shared__ double s_red[4];
   double v[1] = {0.0};
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
-       i += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(count, r_begin, r_end);
+  for (int64_t i = r_begin + threadIdx.x; i < r_end; i += blockDim.x) {
     const double t = x[i];
     v[0] = v[0] + t * t;
   }
@@ -47,9 +47,9 @@ __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ 
   for (int q = 0; q < 10; ++q) v[q] = 0.0;
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const int64_t total = n2 * n;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
-       idx += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(total, r_begin, r_end);
+  for (int64_t idx = r_begin + threadIdx.x; idx < r_end; idx += blockDim.x) {
     const int k = static_cast<int>(idx % n);
     const int j = static_cast<int>((idx / n) % n);
     const int i = static_cast<int>(idx / n2);
@@ -116,9 +116,9 @@ __global__ void __launch_bounds__(256) particle_scalars_kernel(
   double v[11];
 #pragma unroll
   for (int q = 0; q < 11; ++q) v[q] = 0.0;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < np;
-       i += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(np, r_begin, r_end);
+  for (int64_t i = r_begin + threadIdx.x; i < r_end; i += blockDim.x) {
     const double m = mass[i];
     const double x = px[i], y = py[i], z = pz[i];
     const double ux = vx[i], uy = vy[i], uz = vz[i];
@@ -148,9 +148,9 @@ __global__ void __launch_bounds__(256) grid_pe_kernel(const float* __restrict__ 
                                                       double* __restrict__ partials) {
   __shared__ double s_red[4];
   double v[1] = {0.0};
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
-       i += stride)
+  int64_t r_begin, r_end;
+  pm::block_range(count, r_begin, r_end);
+  for (int64_t i = r_begin + threadIdx.x; i < r_end; i += blockDim.x)
     v[0] = v[0] + static_cast<double>(rho[i]) * static_cast<double>(phi[i]);
   pm::block_sum<1>(v, s_red);
   if (threadIdx.x == 0) partials[blockIdx.x * pm::kRedMaxQ] = v[0];
@@ -170,9 +170,9 @@ __global__ void __launch_bounds__(256) radial_hist_kernel(
   extern __shared__ double s_hist[];
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0.0;
   __syncthreads();
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < np;
-       i += stride) {
+  int64_t r_begin, r_end;
+  pm::block_range(np, r_begin, r_end);
+  for (int64_t i = r_begin + threadIdx.x; i < r_end; i += blockDim.x) {
     const double rx = static_cast<double>(px[i]) - cx;
     const double ry = static_cast<double>(py[i]) - cy;
     const double rz = static_cast<double>(pz[i]) - cz;

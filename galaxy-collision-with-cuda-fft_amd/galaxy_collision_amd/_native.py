@@ -117,6 +117,7 @@ _SIGNATURES = {
     "pm_reduce_particles_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_d), _vp],
     "pm_reduce_grid_pe_f64": [_vp, _vp, _i64, ctypes.POINTER(_d), _vp],
     "pm_radial_histogram_f64": [_vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _i32, _vp, _vp],
+    "pm_copy_f32": [_vp, _vp, _i64, _vp],
     "pm_timing_enable": [_i32],
     "pm_timing_reset": [],
     "pm_timing_report": [_i32, _vp, ctypes.POINTER(_i32)],

@@ -214,6 +214,10 @@ pm_status pm_radial_histogram_f64(const float* pos_x, const float* pos_y, const 
                                   float cz, float inv_binw, int32_t nbins, double* hist_dev,
                                   hipStream_t stream);
 
+/* dst = src (count floats): a plain streaming copy of known size, used to calibrate
+ * profiler byte counters (no reference counterpart). */
+pm_status pm_copy_f32(float* dst, const float* src, int64_t count, hipStream_t stream);
+
 /* ------------------------------------------------------------------------------------
  * Kernel timing (no reference counterpart; the reference times stages with wall clocks
  * around ti.sync(), bench.py:66-72). When enabled, each library kernel launch records a
