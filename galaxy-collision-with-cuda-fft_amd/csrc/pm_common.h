@@ -61,6 +61,21 @@ __device__ __forceinline__ void block_range(int64_t count, int64_t& begin, int64
   end = begin + chunk < count ? begin + chunk : count;
 }
 
+// (i, j, k) of flat node index c on an n^3 grid (k fastest) with 32-bit arithmetic; every
+// grid entry point limits n to 1024, so n^3 < 2^32.
+struct Idx3 {
+  int i, j, k;
+};
+__device__ __forceinline__ Idx3 idx3(int64_t c, int n) {
+  const uint32_t u = static_cast<uint32_t>(c), un = static_cast<uint32_t>(n);
+  const uint32_t q = u / un;
+  Idx3 r;
+  r.k = static_cast<int>(u - q * un);
+  r.j = static_cast<int>(q % un);
+  r.i = static_cast<int>(q / un);
+  return r;
+}
+
 // Correctly rounded float32 square root. The hardware estimate (v_sqrt_f32) is within one
 // ulp; each candidate is then checked against the exact midpoints to its neighbours: a
 // midpoint of two floats has at most 25 significant bits and its square at most 50, so both

@@ -486,9 +486,8 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= n2 * n) return;
-  const int k = static_cast<int>(idx % n);
-  const int j = static_cast<int>((idx / n) % n);
-  const int i = static_cast<int>(idx / n2);
+  const pm::Idx3 q3 = pm::idx3(idx, n);
+  const int i = q3.i, j = q3.j, k = q3.k;
   const int64_t m = n + 1;
   float acc = 0.0f;
 #pragma unroll

@@ -8,10 +8,13 @@
 // per-node float32 operation order, so single passes are bitwise equal to a numpy
 // restatement.
 //
-// MIXED mode is defect correction: the iterate phi64 and the residual r = f - A phi64 are
-// float64; each outer iteration solves A e = r with one float32 V-cycle (float64 coarsest
-// solve) and adds e into phi64. f is the same float32 4*pi*G*rho as PLAIN mode and the
-// faces are the same float32 values, so both modes solve the identical discrete problem.
+// MIXED mode is defect correction (mixed-precision iterative refinement): the residual
+// r = f - A phi is evaluated in float64 arithmetic from the float32 iterate; each outer
+// iteration solves A e = r with one float32 V-cycle (float64 coarsest solve) on an aligned
+// hierarchy and adds e into phi. The float32 iterate bounds the attainable error near the
+// float32 resolution of phi (~1e-7 relative), far below the 1e-5 target. f is the same
+// float32 4*pi*G*rho as PLAIN mode and the faces are the same float32 values, so both
+// modes solve the identical discrete problem.
 #include <cmath>
 #include <vector>
 
@@ -38,9 +41,10 @@ __global__ void __launch_bounds__(256) rb_pass_kernel(float* __restrict__ phi,
   const int64_t total = static_cast<int64_t>(n) * n * half;
   const int64_t t = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int kk = static_cast<int>(t % half);
-  const int j = static_cast<int>((t / half) % n);
-  const int i = static_cast<int>(t / (static_cast<int64_t>(half) * n));
+  const uint32_t u = static_cast<uint32_t>(t), q = u / static_cast<uint32_t>(half);
+  const int kk = static_cast<int>(u - q * static_cast<uint32_t>(half));
+  const int j = static_cast<int>(q % static_cast<uint32_t>(n));
+  const int i = static_cast<int>(q / static_cast<uint32_t>(n));
   const int k = 2 * kk + ((i + j + color) & 1);
   if (i < 1 || i > n - 2 || j < 1 || j > n - 2 || k < 1 || k > n - 2) return;
   const int64_t n2 = static_cast<int64_t>(n) * n;
@@ -176,9 +180,8 @@ __global__ void __launch_bounds__(256) residual_kernel(const float* __restrict__
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c >= n2 * n) return;
-  const int k = static_cast<int>(c % n);
-  const int j = static_cast<int>((c / n) % n);
-  const int i = static_cast<int>(c / n2);
+  const pm::Idx3 q3 = pm::idx3(c, n);
+  const int i = q3.i, j = q3.j, k = q3.k;
   float r = 0.0f;
   if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2)
     r = residual_at(phi, rhs, c, n, n2, inv_h2);
@@ -197,9 +200,8 @@ __global__ void __launch_bounds__(256) residual_sumsq_kernel(const float* __rest
   int64_t r_begin, r_end;
   pm::block_range(total, r_begin, r_end);
   for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
-    const int k = static_cast<int>(c % n);
-    const int j = static_cast<int>((c / n) % n);
-    const int i = static_cast<int>(c / n2);
+    const pm::Idx3 q3 = pm::idx3(c, n);
+    const int i = q3.i, j = q3.j, k = q3.k;
     if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
       const double r = static_cast<double>(residual_at(phi, rhs, c, n, n2, inv_h2));
       v[0] = v[0] + r * r;
@@ -218,9 +220,8 @@ __global__ void __launch_bounds__(256) restrict_kernel(const float* __restrict__
   const int64_t nc2 = static_cast<int64_t>(nc) * nc;
   const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc2 * nc) return;
-  const int kc = static_cast<int>(c % nc);
-  const int jc = static_cast<int>((c / nc) % nc);
-  const int ic = static_cast<int>(c / nc2);
+  const pm::Idx3 q3 = pm::idx3(c, nc);
+  const int ic = q3.i, jc = q3.j, kc = q3.k;
   float out = 0.0f;
   if (ic >= 1 && ic <= nc - 2 && jc >= 1 && jc <= nc - 2 && kc >= 1 && kc <= nc - 2) {
     const int64_t nf2 = static_cast<int64_t>(nf) * nf;
@@ -252,9 +253,8 @@ __global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restric
   const int64_t nf2 = static_cast<int64_t>(nf) * nf;
   const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nf2 * nf) return;
-  const int k = static_cast<int>(c % nf);
-  const int j = static_cast<int>((c / nf) % nf);
-  const int i = static_cast<int>(c / nf2);
+  const pm::Idx3 q3 = pm::idx3(c, nf);
+  const int i = q3.i, j = q3.j, k = q3.k;
   if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) return;
   const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
   const int oi = i & 1, oj = j & 1, ok = k & 1;
@@ -543,22 +543,9 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
 // float64 defect-correction kernels (MIXED mode)
 // ---------------------------------------------------------------------------------------
 
-// phi64 = phi32 on the faces; interior = warm ? phi32 : 0.
-__global__ void __launch_bounds__(256) init64_kernel(const float* __restrict__ phi32,
-                                                     double* __restrict__ phi64, int n,
-                                                     int warm) {
-  const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= n2 * n) return;
-  const int k = static_cast<int>(c % n);
-  const int j = static_cast<int>((c / n) % n);
-  const int i = static_cast<int>(c / n2);
-  const bool interior = i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2;
-  phi64[c] = (interior && !warm) ? 0.0 : static_cast<double>(phi32[c]);
-}
-
-// r = f - lap64(phi64) (interior, float64); r32 = (float)r; partial sums of r^2.
-__global__ void __launch_bounds__(256) residual64_kernel(const double* __restrict__ phi,
+// r = f - lap(phi) evaluated in float64 from the float32 iterate (interior; 0 on the
+// boundary); r32 = (float)r; partial sums of r^2 (float64).
+__global__ void __launch_bounds__(256) residual64_kernel(const float* __restrict__ phi,
                                                          const float* __restrict__ f, int n,
                                                          double inv_h2,
                                                          float* __restrict__ r32,
@@ -570,14 +557,14 @@ __global__ void __launch_bounds__(256) residual64_kernel(const double* __restric
   int64_t r_begin, r_end;
   pm::block_range(total, r_begin, r_end);
   for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
-    const int k = static_cast<int>(c % n);
-    const int j = static_cast<int>((c / n) % n);
-    const int i = static_cast<int>(c / n2);
+    const pm::Idx3 q3 = pm::idx3(c, n);
+    const int i = q3.i, j = q3.j, k = q3.k;
     double r = 0.0;
     if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
-      const double lap = (phi[c + n2] + phi[c - n2] + phi[c + n] + phi[c - n] + phi[c + 1] +
-                          phi[c - 1] - 6.0 * phi[c]) *
-                         inv_h2;
+      const double sum = static_cast<double>(phi[c + n2]) + static_cast<double>(phi[c - n2]) +
+                         static_cast<double>(phi[c + n]) + static_cast<double>(phi[c - n]) +
+                         static_cast<double>(phi[c + 1]) + static_cast<double>(phi[c - 1]);
+      const double lap = (sum - 6.0 * static_cast<double>(phi[c])) * inv_h2;
       r = static_cast<double>(f[c]) - lap;
     }
     r32[c] = static_cast<float>(r);
@@ -587,34 +574,27 @@ __global__ void __launch_bounds__(256) residual64_kernel(const double* __restric
   if (threadIdx.x == 0) partials[blockIdx.x * pm::kRedMaxQ] = v[0];
 }
 
-// phi64 += e (interior; e is 0 on the faces); partial sums of e^2 and phi64^2 (float64).
-__global__ void __launch_bounds__(256) update64_kernel(double* __restrict__ phi64,
-                                                       const float* __restrict__ e,
-                                                       int64_t count,
-                                                       double* __restrict__ partials) {
+// phi += e (e is 0 on the faces); partial sums of e^2 and phi^2 (float64).
+__global__ void __launch_bounds__(256) update_kernel(float* __restrict__ phi,
+                                                     const float* __restrict__ e,
+                                                     int64_t count,
+                                                     double* __restrict__ partials) {
   __shared__ double s_red[4 * 2];
   double v[2] = {0.0, 0.0};
   int64_t r_begin, r_end;
   pm::block_range(count, r_begin, r_end);
   for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
-    const double d = static_cast<double>(e[c]);
-    const double p = phi64[c] + d;
-    phi64[c] = p;
-    v[0] = v[0] + d * d;
-    v[1] = v[1] + p * p;
+    const float d = e[c];
+    const float p = phi[c] + d;
+    phi[c] = p;
+    v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
+    v[1] = v[1] + static_cast<double>(p) * static_cast<double>(p);
   }
   pm::block_sum<2>(v, s_red);
   if (threadIdx.x == 0) {
     partials[blockIdx.x * pm::kRedMaxQ] = v[0];
     partials[blockIdx.x * pm::kRedMaxQ + 1] = v[1];
   }
-}
-
-__global__ void __launch_bounds__(256) round64_kernel(const double* __restrict__ phi64,
-                                                      float* __restrict__ phi32,
-                                                      int64_t count) {
-  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (c < count) phi32[c] = static_cast<float>(phi64[c]);
 }
 
 struct Level {
@@ -652,8 +632,8 @@ constexpr double kResidualBytes = 12.0;  // read phi + rhs, write res
 constexpr double kRestrictBytes = 5.0;   // per fine node: read res, write 1/8 coarse rhs + phi
 constexpr double kProlongBytes = 8.5;    // per fine node: read+write phi, read 1/8 coarse
 constexpr double kFillBytes = 4.0;
-constexpr double kRes64Bytes = 16.0;     // read phi64 + f, write r32
-constexpr double kUpd64Bytes = 20.0;     // read phi64 + e, write phi64
+constexpr double kRes64Bytes = 12.0;     // read phi + f, write r32
+constexpr double kUpdBytes = 12.0;       // read phi + e, write phi
 
 }  // namespace
 
@@ -663,7 +643,6 @@ struct pm_mg {
   float* f0 = nullptr;        // 4*pi*G*rho, float32 (both modes)
   float* e0 = nullptr;        // MIXED: finest correction
   float* r32 = nullptr;       // MIXED: finest float32 defect
-  double* phi64 = nullptr;    // MIXED: float64 iterate
   double* partials = nullptr;
   double* red = nullptr;      // device reduction results [kRedMaxQ]
   double* host = nullptr;     // pinned mirror
@@ -850,13 +829,13 @@ pm_status plain_residual_norm(pm_mg* mg, const float* phi, double* out, hipStrea
   return PM_OK;
 }
 
-// r = f - A phi64 -> r32; its float64 sum of squares lands in red[2] (no sync).
-pm_status mixed_residual(pm_mg* mg, hipStream_t s) {
+// r = f - A phi (float64 arithmetic) -> r32; its sum of squares lands in red[2] (no sync).
+pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s) {
   const Level& L = mg->lv[0];
   const unsigned b = pm::red_blocks(L.nodes());
   {
     PM_TIMED("mg.residual64", kRes64Bytes * L.nodes(), s);
-    residual64_kernel<<<b, pm::kBlock, 0, s>>>(mg->phi64, mg->f0, L.n, 1.0 / (L.h * L.h),
+    residual64_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, 1.0 / (L.h * L.h),
                                                 mg->r32, mg->partials);
     PM_CHECK_LAUNCH("residual64_kernel");
   }
@@ -866,18 +845,19 @@ pm_status mixed_residual(pm_mg* mg, hipStream_t s) {
   return PM_OK;
 }
 
-// One defect-correction iteration: e = V(r32); phi64 += e. Returns ||e||, ||phi64||
+// One defect-correction iteration: e = V(r32); phi += e. Returns ||e||, ||phi||
 // (synchronizing).
-pm_status mixed_correct(pm_mg* mg, double* e_norm, double* phi_norm, hipStream_t s) {
+pm_status mixed_correct(pm_mg* mg, float* phi, double* e_norm, double* phi_norm,
+                        hipStream_t s) {
   Level& L0 = mg->lv[0];
   pm_status st;
   if ((st = fill(mg->e0, L0.nodes(), 0.0f, s))) return st;
   if ((st = vcycle(mg, 0, s))) return st;
   const unsigned b = pm::red_blocks(L0.nodes());
   {
-    PM_TIMED("mg.update64", kUpd64Bytes * L0.nodes(), s);
-    update64_kernel<<<b, pm::kBlock, 0, s>>>(mg->phi64, mg->e0, L0.nodes(), mg->partials);
-    PM_CHECK_LAUNCH("update64_kernel");
+    PM_TIMED("mg.update", kUpdBytes * L0.nodes(), s);
+    update_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->e0, L0.nodes(), mg->partials);
+    PM_CHECK_LAUNCH("update_kernel");
   }
   pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(mg->partials, static_cast<int>(b),
                                                         mg->red);
@@ -886,14 +866,6 @@ pm_status mixed_correct(pm_mg* mg, double* e_norm, double* phi_norm, hipStream_t
   if ((st = read_scalars(mg, mg->red, 2, v, s))) return st;
   *e_norm = std::sqrt(v[0]);
   *phi_norm = std::sqrt(v[1]);
-  return PM_OK;
-}
-
-pm_status round64(pm_mg* mg, float* phi, hipStream_t s) {
-  const int64_t nodes = mg->lv[0].nodes();
-  PM_TIMED("mg.round64", 12.0 * nodes, s);
-  round64_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(mg->phi64, phi, nodes);
-  PM_CHECK_LAUNCH("round64_kernel");
   return PM_OK;
 }
 
@@ -963,8 +935,7 @@ pm_status pm_mg_create(const pm_mg_desc* desc, pm_mg** out) {
   }
   if (d.mode == PM_MG_MIXED) {
     if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->e0), sizeof(float) * n0)) ||
-        (st = dalloc(mg, reinterpret_cast<void**>(&mg->r32), sizeof(float) * n0)) ||
-        (st = dalloc(mg, reinterpret_cast<void**>(&mg->phi64), sizeof(double) * n0))) {
+        (st = dalloc(mg, reinterpret_cast<void**>(&mg->r32), sizeof(float) * n0))) {
       destroy(mg);
       return st;
     }
@@ -1045,19 +1016,14 @@ pm_status pm_mg_solve(pm_mg* mg, const float* rho, float* phi, int32_t warm,
       }
     }
   } else {
-    {
-      PM_TIMED("mg.init64", 12.0 * nodes, s);
-      init64_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(phi, mg->phi64, n, warm ? 1 : 0);
-      PM_CHECK_LAUNCH("init64_kernel");
-    }
     L0.phi = mg->e0;
     L0.rhs = mg->r32;
     mg->last_phi = phi;
-    if ((st = mixed_residual(mg, s))) return st;
+    if ((st = mixed_residual(mg, phi, s))) return st;
     double prev_e = -1.0;
     for (int c = 0; c < mg->d.n_cycles; ++c) {
       double e_norm, phi_norm;
-      if ((st = mixed_correct(mg, &e_norm, &phi_norm, s))) return st;
+      if ((st = mixed_correct(mg, phi, &e_norm, &phi_norm, s))) return st;
       // contraction estimate from successive corrections; 0.5 before two are known
       double rate = prev_e > 0.0 ? e_norm / prev_e : 0.5;
       if (rate > 0.95) rate = 0.95;
@@ -1067,9 +1033,8 @@ pm_status pm_mg_solve(pm_mg* mg, const float* rho, float* phi, int32_t warm,
         cycles = c + 1;
         break;
       }
-      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, s))) return st;
+      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s))) return st;
     }
-    if ((st = round64(mg, phi, s))) return st;
     double v[2];
     if ((st = read_scalars(mg, mg->red + 2, 2, v, s))) return st;
     resid = std::sqrt(v[0]);
@@ -1090,11 +1055,10 @@ pm_status pm_mg_vcycle(pm_mg* mg, hipStream_t s) {
   PM_REQUIRE(mg, "pm_mg_vcycle: null handle");
   PM_REQUIRE(mg->solved, "pm_mg_vcycle: no solve has run on this handle");
   if (mg->d.mode == PM_MG_MIXED) {
-    pm_status st = mixed_residual(mg, s);
+    pm_status st = mixed_residual(mg, mg->last_phi, s);
     if (st) return st;
     double e, p;
-    if ((st = mixed_correct(mg, &e, &p, s))) return st;
-    return round64(mg, mg->last_phi, s);
+    return mixed_correct(mg, mg->last_phi, &e, &p, s);
   }
   return vcycle(mg, 0, s);
 }
@@ -1103,7 +1067,7 @@ pm_status pm_mg_residual_norm(pm_mg* mg, double* out, hipStream_t s) {
   PM_REQUIRE(mg && out, "pm_mg_residual_norm: null pointer");
   PM_REQUIRE(mg->solved, "pm_mg_residual_norm: no solve has run on this handle");
   if (mg->d.mode == PM_MG_MIXED) {
-    pm_status st = mixed_residual(mg, s);
+    pm_status st = mixed_residual(mg, mg->last_phi, s);
     if (st) return st;
     double ss;
     if ((st = read_scalars(mg, mg->red + 2, 1, &ss, s))) return st;
@@ -1125,7 +1089,7 @@ pm_status pm_mg_moments(pm_mg* mg, const float* rho, double out[10], hipStream_t
 pm_status pm_rb_pass(float* phi, const float* rhs, int32_t n, float h2, int32_t color,
                      hipStream_t s) {
   PM_REQUIRE(phi && rhs, "pm_rb_pass: null pointer");
-  PM_REQUIRE(n >= 1 && n <= 2048, "pm_rb_pass: n must be in [1, 2048]");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_rb_pass: n must be in [1, 1024]");
   PM_REQUIRE(color == 0 || color == 1, "pm_rb_pass: color must be 0 or 1");
   return rb_pass(phi, rhs, n, h2, color, s);
 }
@@ -1134,14 +1098,14 @@ pm_status pm_rb_sweep(const float* phi_in, float* phi_out, const float* rhs, int
                       float h2, hipStream_t s) {
   PM_REQUIRE(phi_in && phi_out && rhs, "pm_rb_sweep: null pointer");
   PM_REQUIRE(phi_in != phi_out, "pm_rb_sweep: phi_in and phi_out must differ");
-  PM_REQUIRE(n >= 3 && n <= 2048, "pm_rb_sweep: n must be in [3, 2048]");
+  PM_REQUIRE(n >= 3 && n <= 1024, "pm_rb_sweep: n must be in [3, 1024]");
   return rb_sweep(phi_in, phi_out, rhs, n, h2, s);
 }
 
 pm_status pm_residual(const float* phi, const float* rhs, float* res, int32_t n, float inv_h2,
                       hipStream_t s) {
   PM_REQUIRE(phi && rhs && res, "pm_residual: null pointer");
-  PM_REQUIRE(n >= 1 && n <= 2048, "pm_residual: n must be in [1, 2048]");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_residual: n must be in [1, 1024]");
   const int64_t nodes = static_cast<int64_t>(n) * n * n;
   residual_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(phi, rhs, res, n, inv_h2);
   PM_CHECK_LAUNCH("residual_kernel");
@@ -1160,7 +1124,7 @@ pm_status pm_restrict(const float* res_f, float* rhs_c, int32_t nf, int32_t nc, 
 
 pm_status pm_prolong_add(const float* phi_c, float* phi_f, int32_t nf, int32_t nc, hipStream_t s) {
   PM_REQUIRE(phi_c && phi_f, "pm_prolong_add: null pointer");
-  PM_REQUIRE(nf >= 3 && nf <= 2048 && nc >= 2 && 2 * (nc - 1) >= nf - 2 && nf >= 2 * nc - 2,
+  PM_REQUIRE(nf >= 3 && nf <= 1024 && nc >= 2 && 2 * (nc - 1) >= nf - 2 && nf >= 2 * nc - 2,
              "pm_prolong_add: need 2*nc-2 <= nf <= 2*nc (coarse J at fine 2J)");
   const int64_t nodes = static_cast<int64_t>(nf) * nf * nf;
   prolong_add_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(phi_c, phi_f, nf, nc);
@@ -1170,7 +1134,7 @@ pm_status pm_prolong_add(const float* phi_c, float* phi_f, int32_t nf, int32_t n
 
 pm_status pm_set_faces(float* phi, int32_t n, float dx, const float params[11], hipStream_t s) {
   PM_REQUIRE(phi && params, "pm_set_faces: null pointer");
-  PM_REQUIRE(n >= 1 && n <= 2048, "pm_set_faces: n must be in [1, 2048]");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_set_faces: n must be in [1, 1024]");
   pm::RedScratch* rs = nullptr;
   pm_status st = pm::red_scratch(&rs);
   if (st) return st;

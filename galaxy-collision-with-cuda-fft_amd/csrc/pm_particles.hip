@@ -23,9 +23,8 @@ __global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi
   const int64_t total = n2 * n;
   const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= total) return;
-  const int k = static_cast<int>(idx % n);
-  const int j = static_cast<int>((idx / n) % n);
-  const int i = static_cast<int>(idx / n2);
+  const pm::Idx3 q3 = pm::idx3(idx, n);
+  const int i = q3.i, j = q3.j, k = q3.k;
   float a, b, s;
   // x
   if (i == 0) {
@@ -247,7 +246,7 @@ pm_status pm_copy_f32(float* dst, const float* src, int64_t count, hipStream_t s
 pm_status pm_grad(const float* phi, float* gx, float* gy, float* gz, int32_t n, float dx,
                   hipStream_t stream) {
   PM_REQUIRE(phi && gx && gy && gz, "pm_grad: null pointer");
-  PM_REQUIRE(n >= 2 && n <= 2048, "pm_grad: grid_size must be in [2, 2048]");
+  PM_REQUIRE(n >= 2 && n <= 1024, "pm_grad: grid_size must be in [2, 1024]");
   PM_REQUIRE(dx > 0.0f, "pm_grad: dx must be positive");
   const float inv_dx = 1.0f / dx;
   const float inv_2dx = 1.0f / (2.0f * dx);
@@ -267,7 +266,7 @@ pm_status pm_gather(const float* px, const float* py, const float* pz, const flo
   PM_REQUIRE(np >= 0, "pm_gather: negative particle count");
   if (np == 0) return PM_OK;
   PM_REQUIRE(px && py && pz && gx && gy && gz && ax && ay && az, "pm_gather: null pointer");
-  PM_REQUIRE(n >= 1 && n <= 2048, "pm_gather: grid_size must be in [1, 2048]");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_gather: grid_size must be in [1, 1024]");
   PM_REQUIRE(dx > 0.0f, "pm_gather: dx must be positive");
   {
     PM_TIMED("gather", 24.0 * np + 12.0 * n * n * n, stream);
@@ -286,7 +285,7 @@ pm_status pm_gather_kick(const float* px, const float* py, const float* pz, cons
   if (np == 0) return PM_OK;
   PM_REQUIRE(px && py && pz && gx && gy && gz && ax && ay && az && vx && vy && vz,
              "pm_gather_kick: null pointer");
-  PM_REQUIRE(n >= 1 && n <= 2048, "pm_gather_kick: grid_size must be in [1, 2048]");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_gather_kick: grid_size must be in [1, 1024]");
   PM_REQUIRE(dx > 0.0f, "pm_gather_kick: dx must be positive");
   {
     PM_TIMED("gather_kick", 48.0 * np + 12.0 * n * n * n, stream);

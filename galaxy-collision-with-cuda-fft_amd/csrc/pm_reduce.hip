@@ -50,9 +50,8 @@ __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ 
   int64_t r_begin, r_end;
   pm::block_range(total, r_begin, r_end);
   for (int64_t idx = r_begin + threadIdx.x; idx < r_end; idx += blockDim.x) {
-    const int k = static_cast<int>(idx % n);
-    const int j = static_cast<int>((idx / n) % n);
-    const int i = static_cast<int>(idx / n2);
+    const pm::Idx3 q3 = pm::idx3(idx, n);
+    const int i = q3.i, j = q3.j, k = q3.k;
     const double m = static_cast<double>(rho[idx]) * vol;
     const double x = static_cast<double>(i) * dxd;
     const double y = static_cast<double>(j) * dxd;
