@@ -12,4 +12,4 @@ void clear_error() { g_last_error.clear(); }
 
 extern "C" const char* pm_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" int32_t pm_abi_version(void) { return 1; }
+extern "C" int32_t pm_abi_version(void) { return 2; }

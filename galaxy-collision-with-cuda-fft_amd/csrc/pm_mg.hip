@@ -22,6 +22,7 @@
 
 #include "pm_reduce.h"
 #include "pm_timing.h"
+#include "pm_warm.h"
 
 namespace {
 
@@ -651,6 +652,15 @@ struct pm_mg {
   float* last_phi = nullptr;  // caller phi of the last solve
   int bottom_from = -1;       // first level handled by the one-workgroup bottom solver
   bool solved = false;
+  // warm-start state (allocated on the first pm_mg_solve_warm)
+  float* hist = nullptr;      // phi of the previous warm solve (the time history)
+  float* cand = nullptr;      // extrapolated guess of the current solve
+  bool hist_valid = false;
+  int warm_pool = 0;          // network pool factor the buffers below are sized for
+  float* pooled = nullptr;    // [R^3][3] pooled input channels
+  uint16_t* net_in = nullptr; // [R^3][8] bf16 network input
+  float* net_out = nullptr;   // [R^3] network output
+  float* scales = nullptr;    // [8] input/output scales
   std::vector<void*> owned;
 };
 
@@ -869,6 +879,104 @@ pm_status mixed_correct(pm_mg* mg, float* phi, double* e_norm, double* phi_norm,
   return PM_OK;
 }
 
+// Moments, f = 4*pi*G*rho, the initial guess (zero unless warm) with multipole faces, and
+// ||f|| (into red[3]). Enqueued only.
+pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStream_t s) {
+  Level& L0 = mg->lv[0];
+  const int n = L0.n;
+  const int64_t nodes = L0.nodes();
+  pm_status st;
+  // boundary moments (multigrid.py:458) -> float32 face parameters, on the device
+  {
+    PM_TIMED("mg.moments", 4.0 * nodes, s);
+    if ((st = pm::launch_moments(rho, n, mg->d.dx, mg->d.grav, mg->partials, mg->moments,
+                                 mg->face_params, s)))
+      return st;
+  }
+  // f = 4*pi*G*rho (multigrid.py:463)
+  {
+    const float factor = static_cast<float>(4.0 * kPi * mg->d.grav);
+    PM_TIMED("mg.scale", 8.0 * nodes, s);
+    scale_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(rho, mg->f0, nodes, factor);
+    PM_CHECK_LAUNCH("scale_kernel");
+  }
+  if (!warm && (st = fill(phi, nodes, 0.0f, s))) return st;
+  {
+    PM_TIMED("mg.faces", 4.0 * 6 * n * n, s);
+    faces_kernel<<<pm::blocks_for(6 * static_cast<int64_t>(n) * n), pm::kBlock, 0, s>>>(
+        phi, n, mg->d.dx, mg->face_params);
+    PM_CHECK_LAUNCH("faces_kernel");
+  }
+  // ||rhs|| (multigrid.py:477)
+  {
+    PM_TIMED("mg.rhs_norm", 4.0 * nodes, s);
+    if ((st = pm::launch_sumsq_f32(mg->f0, nodes, mg->partials, mg->red + 3, s))) return st;
+  }
+  return PM_OK;
+}
+
+// The V-cycle loop of a solve whose setup has run (PLAIN: reference stop rule; MIXED:
+// defect correction with the error-estimate stop). Fills stats.
+pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s) {
+  Level& L0 = mg->lv[0];
+  const bool mixed = mg->d.mode == PM_MG_MIXED;
+  pm_status st;
+  int cycles = mg->d.n_cycles;
+  double resid = -1.0, err_est = -1.0, rhs_norm = 0.0;
+
+  if (!mixed) {
+    double rhs_ss;
+    if ((st = read_scalars(mg, mg->red + 3, 1, &rhs_ss, s))) return st;
+    rhs_norm = std::sqrt(rhs_ss);
+    L0.phi = phi;
+    L0.rhs = mg->f0;
+    mg->last_phi = phi;
+    for (int c = 0; c < mg->d.n_cycles; ++c) {
+      if ((st = vcycle(mg, 0, s))) return st;
+      if (c + 1 >= mg->d.min_cycles && rhs_norm > 0.0) {
+        if ((st = plain_residual_norm(mg, phi, &resid, s))) return st;
+        if (resid < mg->d.tol * rhs_norm) {
+          cycles = c + 1;
+          break;
+        }
+      }
+    }
+  } else {
+    L0.phi = mg->e0;
+    L0.rhs = mg->r32;
+    mg->last_phi = phi;
+    if ((st = mixed_residual(mg, phi, s))) return st;
+    double prev_e = -1.0;
+    for (int c = 0; c < mg->d.n_cycles; ++c) {
+      double e_norm, phi_norm;
+      if ((st = mixed_correct(mg, phi, &e_norm, &phi_norm, s))) return st;
+      // contraction estimate from successive corrections; 0.5 before two are known
+      double rate = prev_e > 0.0 ? e_norm / prev_e : 0.5;
+      if (rate > 0.95) rate = 0.95;
+      prev_e = e_norm;
+      err_est = phi_norm > 0.0 ? e_norm * rate / (1.0 - rate) / phi_norm : 0.0;
+      if (c + 1 >= mg->d.min_cycles && err_est < mg->d.tol) {
+        cycles = c + 1;
+        break;
+      }
+      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s))) return st;
+    }
+    double v[2];
+    if ((st = read_scalars(mg, mg->red + 2, 2, v, s))) return st;
+    resid = std::sqrt(v[0]);
+    rhs_norm = std::sqrt(v[1]);
+  }
+  mg->solved = true;
+  if (stats) {
+    stats->cycles = cycles;
+    stats->levels = static_cast<int32_t>(mg->lv.size());
+    stats->resid_norm = resid;
+    stats->rhs_norm = rhs_norm;
+    stats->err_est = mixed ? err_est : -1.0;
+  }
+  return PM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -964,90 +1072,115 @@ pm_status pm_mg_solve(pm_mg* mg, const float* rho, float* phi, int32_t warm,
                       pm_mg_stats* stats, hipStream_t s) {
   PM_REQUIRE(mg && rho && phi, "pm_mg_solve: null pointer");
   PM_REQUIRE(rho != phi, "pm_mg_solve: rho and phi must not alias");
-  Level& L0 = mg->lv[0];
+  mg->hist_valid = false;  // a solve outside the warm-start sequence ends its history
+  pm_status st;
+  if ((st = solve_setup(mg, rho, phi, warm != 0, s))) return st;
+  return solve_iterate(mg, phi, stats, s);
+}
+
+pm_status pm_mg_warm_reset(pm_mg* mg) {
+  PM_REQUIRE(mg, "pm_mg_warm_reset: null handle");
+  mg->hist_valid = false;
+  return PM_OK;
+}
+
+pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* w,
+                           pm_mg_stats* stats, hipStream_t s) {
+  pm::clear_error();
+  PM_REQUIRE(mg && rho && phi && w, "pm_mg_solve_warm: null pointer");
+  PM_REQUIRE(rho != phi, "pm_mg_solve_warm: rho and phi must not alias");
+  PM_REQUIRE(w->mode == PM_WARM_PREVIOUS || w->mode == PM_WARM_EXTRAPOLATE ||
+                 w->mode == PM_WARM_LEARNED,
+             "pm_mg_solve_warm: unknown mode");
+  const Level& L0 = mg->lv[0];
   const int n = L0.n;
   const int64_t nodes = L0.nodes();
-  const bool mixed = mg->d.mode == PM_MG_MIXED;
+  const bool learned = w->mode == PM_WARM_LEARNED;
+  const bool pairs = w->pair_input || w->pair_target;
+  const bool want_net_grid = learned || pairs;
+  w->extrapolated = 0;
+  w->used_net = 0;
+  w->resid_before = -1.0;
+  w->resid_after = -1.0;
   pm_status st;
-  // boundary moments (multigrid.py:458) -> float32 face parameters, on the device
-  {
-    PM_TIMED("mg.moments", 4.0 * nodes, s);
-    if ((st = pm::launch_moments(rho, n, mg->d.dx, mg->d.grav, mg->partials, mg->moments,
-                                 mg->face_params, s)))
+  if (want_net_grid) {
+    PM_REQUIRE(w->pool >= 1 && n % w->pool == 0 && (n / w->pool) % 8 == 0,
+               "pm_mg_solve_warm: grid_size / pool must be a multiple of 8");
+  }
+  if (learned) {
+    PM_REQUIRE(w->net, "pm_mg_solve_warm: PM_WARM_LEARNED needs a network");
+    pm_wsnet_layer info;
+    if ((st = pm_wsnet_layer_info(w->net, 0, &info))) return st;
+    PM_REQUIRE(info.res == n / w->pool,
+               "pm_mg_solve_warm: network resolution must equal grid_size / pool");
+    PM_REQUIRE(w->fallback_ratio > 0.0, "pm_mg_solve_warm: fallback_ratio must be positive");
+  }
+  // buffers: time history always; the network grid only when used
+  if (!mg->hist) {
+    if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->hist), sizeof(float) * nodes)) ||
+        (st = dalloc(mg, reinterpret_cast<void**>(&mg->cand), sizeof(float) * nodes)))
       return st;
   }
-  // f = 4*pi*G*rho (multigrid.py:463)
-  {
-    const float factor = static_cast<float>(4.0 * kPi * mg->d.grav);
-    PM_TIMED("mg.scale", 8.0 * nodes, s);
-    scale_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(rho, mg->f0, nodes, factor);
-    PM_CHECK_LAUNCH("scale_kernel");
+  if (want_net_grid && !mg->r32) {
+    if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->r32), sizeof(float) * nodes)) ||
+        (st = dalloc(mg, reinterpret_cast<void**>(&mg->e0), sizeof(float) * nodes)))
+      return st;
   }
-  if (!warm && (st = fill(phi, nodes, 0.0f, s))) return st;
-  {
-    PM_TIMED("mg.faces", 4.0 * 6 * n * n, s);
-    faces_kernel<<<pm::blocks_for(6 * static_cast<int64_t>(n) * n), pm::kBlock, 0, s>>>(
-        phi, n, mg->d.dx, mg->face_params);
-    PM_CHECK_LAUNCH("faces_kernel");
+  if (want_net_grid && mg->warm_pool != w->pool) {
+    const int R = n / w->pool;
+    const int64_t nv = static_cast<int64_t>(R) * R * R;
+    if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->pooled), sizeof(float) * 3 * nv)) ||
+        (st = dalloc(mg, reinterpret_cast<void**>(&mg->net_in), sizeof(uint16_t) * 8 * nv)) ||
+        (st = dalloc(mg, reinterpret_cast<void**>(&mg->net_out), sizeof(float) * nv)) ||
+        (st = dalloc(mg, reinterpret_cast<void**>(&mg->scales), sizeof(float) * 8)))
+      return st;
+    mg->warm_pool = w->pool;
   }
-  // ||rhs|| (multigrid.py:477)
-  {
-    PM_TIMED("mg.rhs_norm", 4.0 * nodes, s);
-    if ((st = pm::launch_sumsq_f32(mg->f0, nodes, mg->partials, mg->red + 3, s))) return st;
-  }
-  int cycles = mg->d.n_cycles;
-  double resid = -1.0, err_est = -1.0, rhs_norm = 0.0;
 
-  if (!mixed) {
-    double rhs_ss;
-    if ((st = read_scalars(mg, mg->red + 3, 1, &rhs_ss, s))) return st;
-    rhs_norm = std::sqrt(rhs_ss);
-    L0.phi = phi;
-    L0.rhs = mg->f0;
-    mg->last_phi = phi;
-    for (int c = 0; c < mg->d.n_cycles; ++c) {
-      if ((st = vcycle(mg, 0, s))) return st;
-      if (c + 1 >= mg->d.min_cycles && rhs_norm > 0.0) {
-        if ((st = plain_residual_norm(mg, phi, &resid, s))) return st;
-        if (resid < mg->d.tol * rhs_norm) {
-          cycles = c + 1;
-          break;
-        }
-      }
-    }
-  } else {
-    L0.phi = mg->e0;
-    L0.rhs = mg->r32;
-    mg->last_phi = phi;
-    if ((st = mixed_residual(mg, phi, s))) return st;
-    double prev_e = -1.0;
-    for (int c = 0; c < mg->d.n_cycles; ++c) {
-      double e_norm, phi_norm;
-      if ((st = mixed_correct(mg, phi, &e_norm, &phi_norm, s))) return st;
-      // contraction estimate from successive corrections; 0.5 before two are known
-      double rate = prev_e > 0.0 ? e_norm / prev_e : 0.5;
-      if (rate > 0.95) rate = 0.95;
-      prev_e = e_norm;
-      err_est = phi_norm > 0.0 ? e_norm * rate / (1.0 - rate) / phi_norm : 0.0;
-      if (c + 1 >= mg->d.min_cycles && err_est < mg->d.tol) {
-        cycles = c + 1;
-        break;
-      }
-      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s))) return st;
-    }
-    double v[2];
-    if ((st = read_scalars(mg, mg->red + 2, 2, v, s))) return st;
-    resid = std::sqrt(v[0]);
-    rhs_norm = std::sqrt(v[1]);
+  if ((st = solve_setup(mg, rho, phi, true, s))) return st;
+  const bool extrap = w->mode != PM_WARM_PREVIOUS && mg->hist_valid;
+  if (!extrap) {
+    // first solve of a sequence (or PREVIOUS mode): phi itself is the guess; remember it
+    PM_TRY_HIP(hipMemcpyAsync(mg->hist, phi, sizeof(float) * nodes, hipMemcpyDeviceToDevice, s));
+    mg->hist_valid = true;
+    return solve_iterate(mg, phi, stats, s);
   }
-  mg->solved = true;
-  if (stats) {
-    stats->cycles = cycles;
-    stats->levels = static_cast<int32_t>(mg->lv.size());
-    stats->resid_norm = resid;
-    stats->rhs_norm = rhs_norm;
-    stats->err_est = mixed ? err_est : -1.0;
+  w->extrapolated = 1;
+  if ((st = pm::warm::extrapolate(phi, mg->hist, (learned || w->pair_target) ? mg->cand : nullptr,
+                                  n, s)))
+    return st;
+  if (want_net_grid) {
+    if ((st = mixed_residual(mg, phi, s))) return st;  // r32, ||r||^2 -> red[2]
+    if ((st = pm::warm::make_input(mg->r32, phi, mg->hist, mg->f0, n, w->pool, L0.h, mg->pooled,
+                                   mg->net_in, mg->partials, mg->red + 4, mg->scales, s)))
+      return st;
+    if (w->pair_input) {
+      const int R = n / w->pool;
+      PM_TRY_HIP(hipMemcpyAsync(w->pair_input, mg->net_in,
+                                sizeof(uint16_t) * 8 * static_cast<size_t>(R) * R * R,
+                                hipMemcpyDeviceToDevice, s));
+    }
   }
+  if (learned) {
+    if ((st = pm_wsnet_forward(w->net, mg->net_in, mg->net_out, s))) return st;
+    if ((st = pm::warm::apply(mg->net_out, n, w->pool, mg->scales, mg->e0, s))) return st;
+    if ((st = pm::warm::guard(mg->r32, mg->e0, n, w->pool, L0.h, mg->partials, mg->red + 8, s)))
+      return st;
+    double v[10];  // red[8], red[9]: pooled residual before and after the correction
+    if ((st = read_scalars(mg, mg->red, 10, v, s))) return st;
+    w->resid_before = std::sqrt(v[8]);
+    w->resid_after = std::sqrt(v[9]);
+    if (std::isfinite(w->resid_after) && w->resid_after < w->fallback_ratio * w->resid_before) {
+      const unsigned b = pm::red_blocks(nodes);
+      PM_TIMED("mg.update", kUpdBytes * nodes, s);
+      update_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->e0, nodes, mg->partials);
+      PM_CHECK_LAUNCH("update_kernel");
+      w->used_net = 1;
+    }
+  }
+  if ((st = solve_iterate(mg, phi, stats, s))) return st;
+  if (w->pair_target)
+    return pm::warm::target(phi, mg->cand, n, w->pool, mg->scales, w->pair_target, s);
   return PM_OK;
 }
 

@@ -18,6 +18,9 @@ PM_OK = 0
 PM_ERR_CAPACITY = 4
 PM_MG_PLAIN = 0
 PM_MG_MIXED = 1
+PM_WARM_PREVIOUS = 0
+PM_WARM_EXTRAPOLATE = 1
+PM_WARM_LEARNED = 2
 
 _vp = ctypes.c_void_p
 _f = ctypes.c_float
@@ -56,6 +59,44 @@ class MGStats(ctypes.Structure):
         ("resid_norm", _d),
         ("rhs_norm", _d),
         ("err_est", _d),
+    ]
+
+
+class MGWarm(ctypes.Structure):
+    _fields_ = [
+        ("mode", _i32),
+        ("pool", _i32),
+        ("net", _vp),
+        ("fallback_ratio", _d),
+        ("pair_input", _vp),
+        ("pair_target", _vp),
+        ("extrapolated", _i32),
+        ("used_net", _i32),
+        ("resid_before", _d),
+        ("resid_after", _d),
+    ]
+
+
+class WsnetDesc(ctypes.Structure):
+    _fields_ = [
+        ("res", _i32),
+        ("in_channels", _i32),
+        ("width", _i32 * 4),
+        ("slope", _f),
+    ]
+
+
+class WsnetLayer(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 8),
+        ("cin_a", _i32),
+        ("cin_b", _i32),
+        ("cout", _i32),
+        ("res", _i32),
+        ("upsample_a", _i32),
+        ("act", _i32),
+        ("w_offset", _i64),
+        ("b_offset", _i64),
     ]
 
 
@@ -105,6 +146,8 @@ _SIGNATURES = {
     "pm_mg_destroy": [_vp],
     "pm_mg_num_levels": [_vp, ctypes.POINTER(_i32)],
     "pm_mg_solve": [_vp, _vp, _vp, _i32, ctypes.POINTER(MGStats), _vp],
+    "pm_mg_solve_warm": [_vp, _vp, _vp, ctypes.POINTER(MGWarm), ctypes.POINTER(MGStats), _vp],
+    "pm_mg_warm_reset": [_vp],
     "pm_mg_vcycle": [_vp, _vp],
     "pm_mg_residual_norm": [_vp, ctypes.POINTER(_d), _vp],
     "pm_mg_moments": [_vp, _vp, ctypes.POINTER(_d), _vp],
@@ -118,6 +161,25 @@ _SIGNATURES = {
     "pm_reduce_grid_pe_f64": [_vp, _vp, _i64, ctypes.POINTER(_d), _vp],
     "pm_radial_histogram_f64": [_vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _i32, _vp, _vp],
     "pm_copy_f32": [_vp, _vp, _i64, _vp],
+    "pm_conv3d_packed_size": [_i32, _i32, _i32, ctypes.POINTER(_i64)],
+    "pm_conv3d_pack_weights": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "pm_conv3d_fwd_bf16": [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _vp, _i32, _i32, _f, _vp, _vp,
+                           _vp],
+    "pm_conv3d_bwd_data_bf16": [_vp, _i32, _i32, _vp, _i32, _vp, _vp],
+    "pm_conv3d_wgrad_partial_size": [_i32, _i32, _i32, ctypes.POINTER(_i64)],
+    "pm_conv3d_bwd_weight_bf16": [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _i32,
+                                  _vp, _i64, _vp],
+    "pm_pool2_bf16": [_vp, _i32, _i32, _vp, _vp],
+    "pm_act_bwd_bf16": [_vp, _i32, _vp, _vp, _f, _i32, _i32, _i32, _vp, _vp],
+    "pm_upsum_f32": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "pm_f32_to_bf16": [_vp, _i64, _vp, _vp],
+    "pm_wsnet_create": [ctypes.POINTER(WsnetDesc), ctypes.POINTER(_vp)],
+    "pm_wsnet_destroy": [_vp],
+    "pm_wsnet_num_params": [_vp, ctypes.POINTER(_i64)],
+    "pm_wsnet_layer_info": [_vp, _i32, ctypes.POINTER(WsnetLayer)],
+    "pm_wsnet_set_params": [_vp, _vp, _vp],
+    "pm_wsnet_forward": [_vp, _vp, _vp, _vp],
+    "pm_wsnet_backward": [_vp, _vp, _vp, _i32, _vp],
     "pm_timing_enable": [_i32],
     "pm_timing_reset": [],
     "pm_timing_report": [_i32, _vp, ctypes.POINTER(_i32)],

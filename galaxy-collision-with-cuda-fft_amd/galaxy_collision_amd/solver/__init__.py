@@ -5,12 +5,15 @@ Names:
 - ``multigrid_mp``: the same discrete problem solved by float64 defect correction around
   float32 V-cycles (mixed precision);
 - ``fft``: zero-padded float64 FFT convolution with the continuum Green's function on the
-  device (the reference's validation solver).
+  device (the reference's validation solver);
+- ``multigrid_learned``: ``multigrid_mp`` warm-started by time extrapolation plus the
+  learned correction (``net=WarmStartNet`` or ``weights=path``; by default the packaged
+  weights for grid_size/pool, if present).
 """
 
 from galaxy_collision_amd.solver.base import PoissonSolver
 
-SOLVER_NAMES = ("multigrid", "multigrid_mp", "fft")
+SOLVER_NAMES = ("multigrid", "multigrid_mp", "fft", "multigrid_learned")
 
 
 def make_solver(name: str, grid_size: int, **kwargs) -> PoissonSolver:
@@ -23,6 +26,18 @@ def make_solver(name: str, grid_size: int, **kwargs) -> PoissonSolver:
         from galaxy_collision_amd.solver.multigrid import MixedPrecisionMultigridSolver
 
         return MixedPrecisionMultigridSolver(grid_size, **kwargs)
+    if name == "multigrid_learned":
+        from galaxy_collision_amd.solver.multigrid import MixedPrecisionMultigridSolver
+        from galaxy_collision_amd.warmstart import WEIGHTS_DIR, WarmStartNet
+
+        net = kwargs.pop("net", None)
+        weights = kwargs.pop("weights", None)
+        pool = int(kwargs.pop("pool", 4))
+        if net is None:
+            path = weights or WEIGHTS_DIR / f"wsnet_r{grid_size // pool}.npz"
+            net = WarmStartNet.load(path)
+        return MixedPrecisionMultigridSolver(grid_size, warm="learned", net=net, pool=pool,
+                                             **kwargs)
     if name == "fft":
         from galaxy_collision_amd.solver.fft import FFTPoissonSolver
 

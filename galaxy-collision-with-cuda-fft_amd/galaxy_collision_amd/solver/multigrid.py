@@ -6,9 +6,14 @@ Dirichlet faces from float64 moments, red-black Gauss-Seidel V(2,2) cycles, 40 s
 coarsest level, and the adaptive stop ||r|| < tol*||4*pi*G*rho||.
 
 ``MixedPrecisionMultigridSolver`` solves the identical discrete problem (same float32
-right-hand side and faces) by float64 defect correction: the iterate and residual are
-float64, each correction is one float32 V-cycle with a float64 coarsest solve. Its default
-tolerance is far below what a float32 residual can resolve.
+right-hand side and faces) by defect correction: the residual of the float32 iterate is
+evaluated in float64 arithmetic, each correction is one float32 V-cycle with a float64
+coarsest solve. Its default tolerance is far below what a float32 residual can resolve.
+
+Both take ``warm``: "previous" (the reference's warm start from the last phi),
+"extrapolate" (2*phi(t-1) - phi(t-2) across consecutive warm solves) or "learned"
+(extrapolation plus the warm-start network's correction, ``net`` required); every mode
+solves to the same tolerance.
 """
 
 from __future__ import annotations
@@ -23,13 +28,28 @@ from galaxy_collision_amd.data import as_tensor
 from galaxy_collision_amd.solver.base import PoissonSolver
 
 
+_WARM_MODES = {"previous": nat.PM_WARM_PREVIOUS, "extrapolate": nat.PM_WARM_EXTRAPOLATE,
+               "learned": nat.PM_WARM_LEARNED}
+
+
 class _NativeMultigrid(PoissonSolver):
     _mode = nat.PM_MG_PLAIN
 
     def __init__(self, grid_size: int, dx: float = 1.0, grav_constant: float | None = None,
                  n_cycles: int = 20, pre_sweeps: int = 2, post_sweeps: int = 2,
-                 coarse_sweeps: int = 40, tol: float = 1e-3, min_cycles: int = 1):
+                 coarse_sweeps: int = 40, tol: float = 1e-3, min_cycles: int = 1,
+                 warm: str = "previous", net=None, pool: int = 4,
+                 fallback_ratio: float = 1.0):
         nat.require_device()
+        if warm not in _WARM_MODES:
+            raise ValueError(f"warm must be one of {sorted(_WARM_MODES)}, got {warm!r}")
+        if warm == "learned" and net is None:
+            raise ValueError("warm='learned' needs a WarmStartNet (net=...)")
+        self.warm = warm
+        self.net = net
+        self.pool = int(pool)
+        self.fallback_ratio = float(fallback_ratio)
+        self.last_warm: dict | None = None
         self.grid_size = int(grid_size)
         self.dx = float(dx)
         self.G = units.G if grav_constant is None else float(grav_constant)
@@ -61,12 +81,45 @@ class _NativeMultigrid(PoissonSolver):
                              f"{(self.grid_size,) * 3}")
 
     def solve(self, rho, phi, warm_start: bool = False) -> None:
+        """phi <- solution; with warm_start the configured warm mode builds the guess."""
+        if warm_start and self.warm != "previous":
+            self.solve_warm(rho, phi, _WARM_MODES[self.warm])
+            return
         r, p = as_tensor(rho), as_tensor(phi)
         self._check_grid(r, "rho")
         self._check_grid(p, "phi")
         st = nat.MGStats()
         nat.call("pm_mg_solve", self._h, nat.ptr(r), nat.ptr(p), 1 if warm_start else 0,
                  ctypes.byref(st), nat.stream())
+        self._record(st, p)
+        self.last_warm = None
+
+    def solve_warm(self, rho, phi, mode: int, pair_input=None, pair_target=None) -> dict:
+        """One step of a warm-started sequence (pm_mg_solve_warm); returns its warm record
+        (extrapolated, used_net, resid_before, resid_after)."""
+        r, p = as_tensor(rho), as_tensor(phi)
+        self._check_grid(r, "rho")
+        self._check_grid(p, "phi")
+        w = nat.MGWarm()
+        w.mode = int(mode)
+        w.pool = self.pool
+        w.net = self.net._h if (self.net is not None and mode == nat.PM_WARM_LEARNED) else None
+        w.fallback_ratio = self.fallback_ratio
+        w.pair_input = nat.ptr(pair_input, torch.bfloat16) if pair_input is not None else None
+        w.pair_target = nat.ptr(pair_target) if pair_target is not None else None
+        st = nat.MGStats()
+        nat.call("pm_mg_solve_warm", self._h, nat.ptr(r), nat.ptr(p), ctypes.byref(w),
+                 ctypes.byref(st), nat.stream())
+        self._record(st, p)
+        self.last_warm = {"extrapolated": bool(w.extrapolated), "used_net": bool(w.used_net),
+                          "resid_before": float(w.resid_before),
+                          "resid_after": float(w.resid_after)}
+        return self.last_warm
+
+    def warm_reset(self) -> None:
+        nat.call("pm_mg_warm_reset", self._h)
+
+    def _record(self, st, p) -> None:
         self.last_cycles = int(st.cycles)
         self.last_stats = {"cycles": int(st.cycles), "levels": int(st.levels),
                            "resid_norm": float(st.resid_norm), "rhs_norm": float(st.rhs_norm),
@@ -132,9 +185,11 @@ class MixedPrecisionMultigridSolver(_NativeMultigrid):
 
     def __init__(self, grid_size: int, dx: float = 1.0, grav_constant: float | None = None,
                  n_cycles: int = 40, pre_sweeps: int = 2, post_sweeps: int = 2,
-                 coarse_sweeps: int = 40, tol: float = 1e-6, min_cycles: int = 1):
+                 coarse_sweeps: int = 40, tol: float = 1e-6, min_cycles: int = 1,
+                 warm: str = "previous", net=None, pool: int = 4,
+                 fallback_ratio: float = 1.0):
         super().__init__(grid_size, dx, grav_constant, n_cycles, pre_sweeps, post_sweeps,
-                         coarse_sweeps, tol, min_cycles)
+                         coarse_sweeps, tol, min_cycles, warm, net, pool, fallback_ratio)
 
 
 __all__ = ["MultigridPoissonSolver", "MixedPrecisionMultigridSolver"]

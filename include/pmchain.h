@@ -158,6 +158,52 @@ pm_status pm_mg_num_levels(pm_mg* mg, int32_t* out);
 pm_status pm_mg_solve(pm_mg* mg, const float* rho, float* phi, int32_t warm_start,
                       pm_mg_stats* stats, hipStream_t stream);
 
+/* ------------------------------------------------------------------------------------
+ * Warm starts across time steps (no reference counterpart: the reference warm-starts from
+ * the previous phi, multigrid.py:465-466, which is PM_WARM_PREVIOUS here).
+ *
+ * A sequence of pm_mg_solve_warm calls on one handle, one per time step with phi holding
+ * the previous step's solution, keeps the previous phi in the handle. PM_WARM_EXTRAPOLATE
+ * then starts from 2*phi(t-1) - phi(t-2) on the interior; PM_WARM_LEARNED adds the
+ * warm-start network's prediction of that guess's error (network input: the residual of
+ * the extrapolated guess, phi(t-1) - phi(t-2) and 4*pi*G*rho, each pooled by pool^3 and
+ * divided by its RMS; output: float32 at grid_size/pool, scaled by RMS(pooled residual) *
+ * (pool*dx)^2 and interpolated trilinearly onto the fine interior). The multigrid judges
+ * every inference: the correction is kept only if the norm of the pooled residual (the mean
+ * residual over each network voxel, float64; the smooth part the correction addresses) after
+ * it is below fallback_ratio times the norm before it, otherwise the solve proceeds from the
+ * extrapolated guess alone. Either way the solve runs to the handle's tolerance, so the
+ * result meets the same accuracy. The first call of a sequence (or after pm_mg_solve or
+ * pm_mg_warm_reset) has no history and starts from phi like PM_WARM_PREVIOUS.
+ * pair_input / pair_target, when set, receive the network input and the training target
+ * (the pooled error of the extrapolated guess against the converged phi, divided by the
+ * output scale) of this solve; they are written only when `extrapolated` comes back 1.
+ * Buffers for the history (2 grids) and the network grid are allocated on first use.
+ * ---------------------------------------------------------------------------------- */
+typedef struct pm_wsnet pm_wsnet;
+
+enum { PM_WARM_PREVIOUS = 0, PM_WARM_EXTRAPOLATE = 1, PM_WARM_LEARNED = 2 };
+
+typedef struct {
+  int32_t mode;            /* PM_WARM_* */
+  int32_t pool;            /* fine nodes per network voxel per axis; grid_size/pool % 8 == 0 */
+  pm_wsnet* net;           /* PM_WARM_LEARNED: network of resolution grid_size/pool */
+  double fallback_ratio;   /* keep the correction iff |pooled r| after < ratio * before (1.0) */
+  uint16_t* pair_input;    /* optional out: bf16 [R^3][8] network input (R = grid_size/pool) */
+  float* pair_target;      /* optional out: float32 [R^3] training target */
+  int32_t extrapolated;    /* out: 1 if the guess was extrapolated in time */
+  int32_t used_net;        /* out: 1 if the learned correction was kept */
+  double resid_before;     /* out (LEARNED): pooled residual norm of the extrapolated guess */
+  double resid_after;      /* out (LEARNED): the same after the learned correction */
+} pm_mg_warm;
+
+/* Warm-started solve (see above). Synchronizes like pm_mg_solve, plus once before the
+ * V-cycles when the network grid is used. */
+pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* warm,
+                           pm_mg_stats* stats, hipStream_t stream);
+/* Forget the time history (the next warm solve starts from phi alone). */
+pm_status pm_mg_warm_reset(pm_mg* mg);
+
 /* One more V-cycle on the state of the last solve (reference multigrid.py:429-440
  * `_vcycle(0)` as called by tests). */
 pm_status pm_mg_vcycle(pm_mg* mg, hipStream_t stream);
@@ -219,6 +265,95 @@ pm_status pm_radial_histogram_f64(const float* pos_x, const float* pos_y, const 
 pm_status pm_copy_f32(float* dst, const float* src, int64_t count, hipStream_t stream);
 
 /* ------------------------------------------------------------------------------------
+ * BF16 3x3x3 convolutions on the matrix cores (no reference counterpart: they serve the
+ * learned warm start).
+ *
+ * Tensors are channels-last bf16 bit patterns (uint16_t): element (voxel v, channel c) of
+ * an r^3 tensor with C channels sits at v*C + c, v = (z*r + y)*r + x. Stored channel counts
+ * are multiples of 8 and pointers 16-byte aligned. A convolution is a "same" (padding 1)
+ * cross-correlation with torch.nn.functional.conv3d's weight layout w[cout][cin][3][3][3]
+ * (float32 master copy), accumulated in float32 (v_mfma_f32_16x16x32_bf16). Its input is
+ * xa (ca channels) followed by xb (cb channels, may be absent); with up_a != 0, xa has
+ * resolution r/2 and is read nearest-neighbour upsampled.
+ * ---------------------------------------------------------------------------------- */
+/* Elements of the packed bf16 operand of w: transpose 0 for the forward pass, 1 for the
+ * backward-data pass (weights flipped in space, channels transposed). */
+pm_status pm_conv3d_packed_size(int32_t cout, int32_t cin, int32_t transpose, int64_t* elems);
+pm_status pm_conv3d_pack_weights(const float* w, int32_t cout, int32_t cin, int32_t transpose,
+                                 uint16_t* packed, hipStream_t stream);
+/* y = act(conv(x) + bias), act 1 = leaky ReLU with negative slope `slope`, 0 = identity.
+ * Outputs (either may be null): y bf16 [r^3][roundup8(cout)] (pad channels 0), y32 float32
+ * [r^3][cout]. bias may be null. */
+pm_status pm_conv3d_fwd_bf16(const uint16_t* xa, int32_t ca, const uint16_t* xb, int32_t cb,
+                             int32_t r, int32_t up_a, const uint16_t* packed, const float* bias,
+                             int32_t cout, int32_t act, float slope, uint16_t* y, float* y32,
+                             hipStream_t stream);
+/* dx float32 [r^3][cin] = gradient of the (un-upsampled, concatenated) convolution input
+ * for the output gradient dz (bf16 [r^3][roundup8(cout)]); packed_t from transpose = 1. */
+pm_status pm_conv3d_bwd_data_bf16(const uint16_t* dz, int32_t cout, int32_t r,
+                                  const uint16_t* packed_t, int32_t cin, float* dx,
+                                  hipStream_t stream);
+/* Floats of scratch pm_conv3d_bwd_weight_bf16 needs for these shapes. */
+pm_status pm_conv3d_wgrad_partial_size(int32_t cin, int32_t cout, int32_t r, int64_t* floats);
+/* dw float32 [cout][cin][27] and db float32 [cout] (may be null) of the convolution for the
+ * input (xa, xb, up_a) and output gradient dz; accumulate != 0 adds to dw and db. The sum
+ * over voxels is split in a fixed order (deterministic). roundup8(cout) must be a power of
+ * two. */
+pm_status pm_conv3d_bwd_weight_bf16(const uint16_t* xa, int32_t ca, const uint16_t* xb,
+                                    int32_t cb, int32_t r, int32_t up_a, const uint16_t* dz,
+                                    int32_t cout, float* dw, float* db, int32_t accumulate,
+                                    float* partial, int64_t partial_floats, hipStream_t stream);
+/* 2x2x2 average pooling, bf16 [r^3][c] -> [(r/2)^3][c] (float32 sum, fixed order). */
+pm_status pm_pool2_bf16(const uint16_t* x, int32_t r, int32_t c, uint16_t* y, hipStream_t stream);
+/* dz[v][c] = bf16((g1[v*g1_stride + c] + g2[parent(v)*gc + c]/8) * act'(y[v][c])) for
+ * c < gc and 0 for gc <= c < dz_c; g1, g2 (at r/2: the gradient through a 2x2x2 average
+ * pool) and y (null: no activation) may be null. act' = 1 where y > 0, else slope. */
+pm_status pm_act_bwd_bf16(const float* g1, int32_t g1_stride, const float* g2, const uint16_t* y,
+                          float slope, int32_t r, int32_t gc, int32_t dz_c, uint16_t* dz,
+                          hipStream_t stream);
+/* ga[vc][c] = sum over the 8 children of dx[child*cstride + c], c < ca (gradient of a
+ * nearest-upsampled tensor), at resolution r/2. */
+pm_status pm_upsum_f32(const float* dx, int32_t r, int32_t cstride, int32_t ca, float* ga,
+                       hipStream_t stream);
+/* y = bf16(x), round to nearest even. */
+pm_status pm_f32_to_bf16(const float* x, int64_t count, uint16_t* y, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------
+ * The warm-start network (no reference counterpart): a 4-level 3D convolutional
+ * encoder-decoder, 15 conv3x3x3 layers e0a e0b e1a e1b e2a e2b e3a e3b (encoder, 2x2x2
+ * average pooling between levels) d2a d2b d1a d1b d0a d0b (decoder, nearest upsampling
+ * concatenated with the encoder tensor of the same level) and a linear 1-channel output
+ * layer; leaky ReLU elsewhere; bf16 activations, float32 accumulation and output.
+ * Parameters: one float32 vector, per layer w [cout][cin][27] then b [cout].
+ * ---------------------------------------------------------------------------------- */
+typedef struct {
+  int32_t res;           /* finest resolution (multiple of 8, <= 256) */
+  int32_t in_channels;   /* channels the caller fills (<= 8; the stored input has 8) */
+  int32_t width[4];      /* channels per level (16, 32, 64 or 128 each) */
+  float slope;           /* leaky ReLU negative slope */
+} pm_wsnet_desc;
+
+typedef struct {
+  char name[8];
+  int32_t cin_a, cin_b, cout, res, upsample_a, act;
+  int64_t w_offset, b_offset;  /* into the parameter vector */
+} pm_wsnet_layer;
+
+pm_status pm_wsnet_create(const pm_wsnet_desc* desc, pm_wsnet** out);
+pm_status pm_wsnet_destroy(pm_wsnet* net);
+pm_status pm_wsnet_num_params(const pm_wsnet* net, int64_t* out);
+/* index 0..14 */
+pm_status pm_wsnet_layer_info(const pm_wsnet* net, int32_t index, pm_wsnet_layer* out);
+/* Copies the float32 parameter vector (device) and packs the bf16 operands. */
+pm_status pm_wsnet_set_params(pm_wsnet* net, const float* params, hipStream_t stream);
+/* out float32 [res^3] from input bf16 [res^3][8]; keeps the activations for backward. */
+pm_status pm_wsnet_forward(pm_wsnet* net, const uint16_t* input, float* out, hipStream_t stream);
+/* grads (float32, parameter layout) of sum(dout * out) for the last forward pass;
+ * accumulate != 0 adds to grads. */
+pm_status pm_wsnet_backward(pm_wsnet* net, const float* dout, float* grads, int32_t accumulate,
+                            hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------
  * Kernel timing (no reference counterpart; the reference times stages with wall clocks
  * around ti.sync(), bench.py:66-72). When enabled, each library kernel launch records a
  * HIP event pair on its own stream, tagged with the kernel's algorithmic bytes.
@@ -227,7 +362,8 @@ typedef struct {
   char name[48];
   int64_t launches;
   double total_ms;   /* summed event time */
-  double bytes;      /* summed algorithmic (compulsory) bytes of those launches */
+  double bytes;      /* summed algorithmic (compulsory) bytes of those launches; for the
+                        net.*conv kernels the summed floating-point operations */
 } pm_kernel_time;
 
 pm_status pm_timing_enable(int32_t on);

@@ -40,7 +40,7 @@ def test_python_binding_covers_header():
     from galaxy_collision_amd import _native as nat
 
     assert set(declared_functions()) <= set(nat.exported_names())
-    assert nat.lib().pm_abi_version() == 1
+    assert nat.lib().pm_abi_version() == 2
 
 
 @pytest.mark.skipif(not LIB.exists(), reason="libpmchain.so not built")
