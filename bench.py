@@ -2,7 +2,8 @@
 
 Workloads (seeded synthetic inputs, built on the host, resident in HBM before timing):
   collision10m  two-galaxy collision, 10M particles, 256^3 grid, box 40, dt 0.01,
-                mixed-precision multigrid to an estimated phi error of 1e-6 (default)
+                mixed-precision multigrid to an estimated phi error of 1e-6, warm-started by
+                linear extrapolation of phi in time (default)
   collision2m   the same recipe with 2 x 1,048,576 particles on a 128^3 grid, box 40
   plummer       131,072-particle Plummer sphere, 64^3 grid over a box of 20a, dt 0.01
 
@@ -19,7 +20,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -33,7 +33,7 @@ REFERENCE_STEPS_PER_S = 24.8  # reference docs/performance.md:17 (RTX 3070, 256^
 
 WORKLOADS = {
     "collision10m": dict(kind="two_galaxy", n_per=5_000_000, grid=256, box=40.0, dt=0.01,
-                         solver="multigrid_mp"),
+                         solver="multigrid_mp", warm="extrapolate"),
     "collision2m": dict(kind="two_galaxy", n_per=1_048_576, grid=128, box=40.0, dt=0.01,
                         solver="multigrid"),
     "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
@@ -54,47 +54,11 @@ def build_ic(w: dict, seed: int):
     return ic.sort_by_cell(icr, n, dx), dx
 
 
-def _dst_exact(f, boundary, dx):
-    """Exact float64 solution of the 7-point Dirichlet problem (scipy DST-I)."""
-    import numpy as np
-    from scipy.fft import dstn, idstn
-
-    n = f.shape[0]
-    b = f[1:-1, 1:-1, 1:-1].astype(np.float64) * dx * dx
-    bd = boundary.astype(np.float64).copy()
-    bd[1:-1, 1:-1, 1:-1] = 0.0
-    b -= (bd[2:, 1:-1, 1:-1] + bd[:-2, 1:-1, 1:-1] + bd[1:-1, 2:, 1:-1] + bd[1:-1, :-2, 1:-1]
-          + bd[1:-1, 1:-1, 2:] + bd[1:-1, 1:-1, :-2])
-    m = n - 2
-    lam1 = 2.0 * (np.cos(np.pi * np.arange(1, m + 1) / (m + 1)) - 1.0)
-    lam = lam1[:, None, None] + lam1[None, :, None] + lam1[None, None, :]
-    u = idstn(dstn(b, type=1, workers=-1) / lam, type=1, workers=-1)
-    out = boundary.astype(np.float64).copy()
-    out[1:-1, 1:-1, 1:-1] = u
-    return out
-
-
 def phi_accuracy(chain, grav: float) -> dict:
-    """phi of the last step against (a) the exact solution of the same discrete problem and
-    (b) the zero-padded continuum-kernel FFT solve (float64, on the device)."""
-    import numpy as np
-    import torch
+    """phi of the last step against the exact discrete solution and the continuum FFT."""
+    from galaxy_collision_amd.accuracy import phi_accuracy as acc
 
-    from galaxy_collision_amd.solver.fft import FFTPoissonSolver
-
-    rho = chain.grid.rho.to_numpy()
-    phi = chain.grid.phi.to_numpy().astype(np.float64)
-    f = (np.float32(4.0 * math.pi * grav) * rho).astype(np.float32)
-    exact = _dst_exact(f, phi, chain.dx)
-    alg = float(np.linalg.norm(phi - exact) / np.linalg.norm(exact))
-    fft = FFTPoissonSolver(chain.n, dx=chain.dx, grav_constant=grav)
-    out = torch.empty_like(chain.grid.phi.tensor)
-    fft.solve(chain.grid.rho, out)
-    ref = out.cpu().numpy().astype(np.float64)
-    del fft, out
-    torch.cuda.empty_cache()
-    tot = float(np.linalg.norm(phi - ref) / np.linalg.norm(ref))
-    return {"phi_rel_l2_vs_exact_discrete": alg, "phi_rel_l2_vs_fft_continuum": tot}
+    return acc(chain.grid.rho, chain.grid.phi, chain.n, chain.dx, grav)
 
 
 def cpu_baseline(icr, w: dict, dx: float, budget_s: float, threads: int | None) -> dict:
@@ -127,6 +91,8 @@ def main() -> int:
     ap.add_argument("--workload", default="collision10m", choices=sorted(WORKLOADS))
     ap.add_argument("--solver", default=None, help="override the workload's solver")
     ap.add_argument("--tol", type=float, default=None, help="solver tolerance override")
+    ap.add_argument("--warm", default=None, choices=["previous", "extrapolate"],
+                    help="multigrid warm start across steps (default: the workload's)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of timed CPU steps")
@@ -179,6 +145,9 @@ def main() -> int:
     kw = {"dx": dx, "grav_constant": grav}
     if args.tol is not None:
         kw["tol"] = args.tol
+    warm = args.warm or (w.get("warm", "previous") if solver_name == w["solver"] else "previous")
+    if solver_name in ("multigrid", "multigrid_mp") and warm != "previous":
+        kw["warm"] = warm
     solver = make_solver(solver_name, w["grid"], **kw)
     chain = PMChain(parts, w["grid"], dx, solver)
     chain.prime()
@@ -267,6 +236,7 @@ def main() -> int:
             "dt": w["dt"],
             "solver": solver_name,
             "solver_tol": getattr(solver, "tol", None),
+            "warm_start": getattr(solver, "warm", None),
             "precision": "f32 particles/V-cycles; f64 residual, coarsest solve, reductions"
             if solver_name == "multigrid_mp" else "f32",
             "parallelism": "single GPU",

@@ -31,9 +31,13 @@ int64_t wgrad_partial_floats(int cin, int cout, int r);
 void wgrad_plan(int cin, int cout, int r, int* splits, int* chunks_per_split);
 
 pm_status pack(const float* w, int cout, int cin, int transpose, uint16_t* wp, hipStream_t s);
+// scratch (may be null): float32 space for split-K partial sums; with less than
+// fwd_scratch_floats(cin, cout, r) the reduction is not split.
+int64_t fwd_scratch_floats(int cin, int cout, int r);
 pm_status fwd(const uint16_t* xa, int ca, const uint16_t* xb, int cb, int r, int up,
               const uint16_t* wp, const float* bias, int cout, int act, float slope, uint16_t* y,
-              float* y32, const char* timer, hipStream_t s);
+              float* y32, const char* timer, hipStream_t s, float* scratch,
+              int64_t scratch_floats);
 pm_status wgrad(const uint16_t* xa, int ca, const uint16_t* xb, int cb, int r, int up,
                 const uint16_t* dz, int cout, float* dw, float* db, int accumulate,
                 float* partial, int64_t partial_floats, const char* timer, hipStream_t s);

@@ -51,7 +51,8 @@ struct ConvFwdArgs {
   int ca, cb, cin, r, ra, up;
   const uint16_t* w;  // packed [rows][kpad]
   int kpad, kreal, rows;
-  int co0;            // first output channel of this launch (chunks of <= 128 channels)
+  int kc_per;         // 32-wide reduction chunks per split (blockIdx.z)
+  float* partial;     // split-K: raw sums [split][r^3][rows] (epilogue in a second kernel)
   const float* bias;
   int cout, cout_st, act;
   float slope;
@@ -75,10 +76,14 @@ __device__ __forceinline__ const uint16_t* conv_src(const uint16_t* xa, const ui
   return xb + static_cast<int64_t>((z * r + y) * r + x) * cb + (ci - ca);
 }
 
-// One wave: 64 output voxels (4 MFMA row blocks) x output channels co0 .. co0 + 16*NB - 1
-// (column blocks at or beyond the packed weight rows are skipped).
+// One wave: 64 output voxels (4 MFMA row blocks) x output channels co0 .. co0 + 16*NB - 1,
+// co0 = blockIdx.y * 16 * NB (column blocks at or beyond the packed weight rows are
+// skipped), over the reduction chunks of split blockIdx.z.
 template <int NB>
 __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvFwdArgs a) {
+  const int co0 = static_cast<int>(blockIdx.y) * 16 * NB;
+  const int kbeg = static_cast<int>(blockIdx.z) * a.kc_per * 32;
+  const int kend = min(a.kpad, kbeg + a.kc_per * 32);
   const int lane = threadIdx.x & 63;
   const int row = lane & 15, grp = lane >> 4;
   const int r = a.r;
@@ -106,17 +111,17 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvFwdArgs a) {
     for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 
   // this lane's k = kc + 8*grp as (tap, ci); advanced by 32 per chunk
-  int tap = (8 * grp) / a.cin;
-  int ci = 8 * grp - tap * a.cin;
+  int tap = (kbeg + 8 * grp) / a.cin;
+  int ci = kbeg + 8 * grp - tap * a.cin;
   const uint16_t* wrow[NB];
   bool wvalid[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
-    wvalid[nb] = a.co0 + 16 * nb < a.rows;
-    wrow[nb] = a.w + static_cast<int64_t>(wvalid[nb] ? a.co0 + 16 * nb + row : 0) * a.kpad + 8 * grp;
+    wvalid[nb] = co0 + 16 * nb < a.rows;
+    wrow[nb] = a.w + static_cast<int64_t>(wvalid[nb] ? co0 + 16 * nb + row : 0) * a.kpad + 8 * grp;
   }
 
-  for (int kc = 0; kc < a.kpad; kc += 32) {
+  for (int kc = kbeg; kc < kend; kc += 32) {
     u32x4_t av[4];
     const bool kin = kc + 8 * grp < a.kreal;
     const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
@@ -143,9 +148,25 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvFwdArgs a) {
   }
 
   // D layout: lane holds rows 4*grp + i (voxels), column `row` (output channel)
+  if (a.partial) {
+    float* out = a.partial + static_cast<int64_t>(blockIdx.z) * nvox * a.rows;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = co0 + 16 * nb + row;
+      if (co >= a.rows) continue;
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int v = vbase + 16 * mb + 4 * grp + i;
+          if (v < nvox) out[static_cast<int64_t>(v) * a.rows + co] = acc[mb][nb][i];
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
-    const int co = a.co0 + 16 * nb + row;
+    const int co = co0 + 16 * nb + row;
     if (co >= a.cout_st) continue;
     const bool real = co < a.cout;
     const float b = (real && a.bias) ? a.bias[co] : 0.0f;
@@ -165,6 +186,28 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvFwdArgs a) {
       }
     }
   }
+}
+
+// Split-K epilogue: y = act(bias + sum over splits in order), one thread per output value.
+__global__ void __launch_bounds__(256) conv_epilogue_kernel(const float* __restrict__ partial,
+                                                            int splits, int nvox, int rows,
+                                                            const float* __restrict__ bias,
+                                                            int cout, int cout_st, int act,
+                                                            float slope, uint16_t* y, float* y32) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<int64_t>(nvox) * cout_st) return;
+  const int co = static_cast<int>(idx % cout_st);
+  const int v = static_cast<int>(idx / cout_st);
+  float val = 0.0f;
+  if (co < cout) {
+    float sum = 0.0f;
+    for (int z = 0; z < splits; ++z)
+      sum += partial[(static_cast<int64_t>(z) * nvox + v) * rows + co];
+    val = sum + (bias ? bias[co] : 0.0f);
+    if (act && !(val > 0.0f)) val = val * slope;
+  }
+  if (y) y[idx] = f2bf(val);
+  if (y32 && co < cout) y32[static_cast<int64_t>(v) * cout + co] = val;
 }
 
 struct ConvWgradArgs {
@@ -434,9 +477,44 @@ pm_status check_input(const uint16_t* xa, int ca, const uint16_t* xb, int cb, in
   return PM_OK;
 }
 
+// Launch shape of a forward convolution: column groups of 16*NB output channels and
+// reduction splits so that small grids (the network's coarse levels) still spread over the
+// chip. Split-K needs float32 scratch for the raw sums (fwd_scratch_floats).
+struct FwdPlan {
+  int nb = 8, groups = 1, ksplit = 1, kc_per = 1;
+};
+
+FwdPlan fwd_plan(int nvox, int rows, int kpad_, bool allow_split) {
+  constexpr int64_t kTargetWaves = 2048;
+  FwdPlan p;
+  const int64_t vox_waves = (nvox + 63) / 64;
+  p.nb = std::min(8, rows / 16);
+  while (p.nb > 1 && vox_waves * ((rows + 16 * p.nb - 1) / (16 * p.nb)) < kTargetWaves) p.nb /= 2;
+  if (p.nb == 3) p.nb = 2;
+  p.groups = (rows + 16 * p.nb - 1) / (16 * p.nb);
+  const int kchunks = kpad_ / 32;
+  const int64_t waves = vox_waves * p.groups;
+  if (allow_split && waves < kTargetWaves) {
+    int64_t want = kTargetWaves / std::max<int64_t>(1, waves);
+    want = std::min<int64_t>(want, std::max(1, kchunks / 4));
+    want = std::min<int64_t>(want, 32);
+    p.ksplit = static_cast<int>(std::max<int64_t>(1, want));
+  }
+  p.kc_per = (kchunks + p.ksplit - 1) / p.ksplit;
+  p.ksplit = (kchunks + p.kc_per - 1) / p.kc_per;
+  return p;
+}
+
+int64_t fwd_scratch_floats(int cin, int cout, int r) {
+  const int64_t nvox = static_cast<int64_t>(r) * r * r;
+  const FwdPlan p = fwd_plan(static_cast<int>(nvox), rows_pad(cout), kpad(cin), true);
+  return p.ksplit > 1 ? p.ksplit * nvox * rows_pad(cout) : 0;
+}
+
 pm_status fwd(const uint16_t* xa, int ca, const uint16_t* xb, int cb, int r, int up,
               const uint16_t* wp, const float* bias, int cout, int act, float slope, uint16_t* y,
-              float* y32, const char* timer, hipStream_t s) {
+              float* y32, const char* timer, hipStream_t s, float* scratch,
+              int64_t scratch_floats) {
   pm_status st;
   if ((st = check_input(xa, ca, xb, cb, r, up, "pm_conv3d_fwd_bf16"))) return st;
   PM_REQUIRE(wp && aligned16(wp), "pm_conv3d_fwd_bf16: weights null or not 16-byte aligned");
@@ -462,21 +540,28 @@ pm_status fwd(const uint16_t* xa, int ca, const uint16_t* xb, int cb, int r, int
   a.slope = slope;
   a.y = y;
   a.y32 = y32;
-  const int64_t nvox = static_cast<int64_t>(r) * r * r;
-  const unsigned blocks = static_cast<unsigned>((nvox + 255) / 256);
+  const int nvox = r * r * r;
+  FwdPlan p = fwd_plan(nvox, a.rows, a.kpad, scratch != nullptr);
+  if (p.ksplit > 1 && static_cast<int64_t>(p.ksplit) * nvox * a.rows > scratch_floats)
+    p = fwd_plan(nvox, a.rows, a.kpad, false);
+  a.kc_per = p.kc_per;
+  a.partial = p.ksplit > 1 ? scratch : nullptr;
+  const dim3 grid(static_cast<unsigned>((nvox + 255) / 256), static_cast<unsigned>(p.groups),
+                  static_cast<unsigned>(p.ksplit));
   const double flops = 2.0 * 27.0 * a.cin * cout * static_cast<double>(nvox);
   PM_TIMED(timer, flops, s);
-  for (int co0 = 0; co0 < a.rows; co0 += 128) {
-    a.co0 = co0;
-    const int nb = std::min(8, (a.rows - co0) / 16);
-    switch (nb) {
-      case 1: conv_fwd_kernel<1><<<blocks, 256, 0, s>>>(a); break;
-      case 2: conv_fwd_kernel<2><<<blocks, 256, 0, s>>>(a); break;
-      case 3:
-      case 4: conv_fwd_kernel<4><<<blocks, 256, 0, s>>>(a); break;
-      default: conv_fwd_kernel<8><<<blocks, 256, 0, s>>>(a); break;
-    }
-    PM_CHECK_LAUNCH("conv_fwd_kernel");
+  switch (p.nb) {
+    case 1: conv_fwd_kernel<1><<<grid, 256, 0, s>>>(a); break;
+    case 2: conv_fwd_kernel<2><<<grid, 256, 0, s>>>(a); break;
+    case 4: conv_fwd_kernel<4><<<grid, 256, 0, s>>>(a); break;
+    default: conv_fwd_kernel<8><<<grid, 256, 0, s>>>(a); break;
+  }
+  PM_CHECK_LAUNCH("conv_fwd_kernel");
+  if (p.ksplit > 1) {
+    const int64_t total = static_cast<int64_t>(nvox) * a.cout_st;
+    conv_epilogue_kernel<<<pm::blocks_for(total), 256, 0, s>>>(
+        scratch, p.ksplit, nvox, a.rows, bias, cout, a.cout_st, a.act, slope, y, y32);
+    PM_CHECK_LAUNCH("conv_epilogue_kernel");
   }
   return PM_OK;
 }
@@ -651,22 +736,30 @@ pm_status pm_conv3d_pack_weights(const float* w, int32_t cout, int32_t cin, int3
   return pm::conv::pack(w, cout, cin, transpose, packed, s);
 }
 
+pm_status pm_conv3d_scratch_size(int32_t cin, int32_t cout, int32_t r, int64_t* floats) {
+  PM_REQUIRE(floats && cin >= 1 && cout >= 1 && r >= 1, "pm_conv3d_scratch_size: bad argument");
+  *floats = std::max(pm::conv::fwd_scratch_floats(cin, cout, r),
+                     pm::conv::fwd_scratch_floats(pm::conv::store_channels(cout), cin, r));
+  return PM_OK;
+}
+
 pm_status pm_conv3d_fwd_bf16(const uint16_t* xa, int32_t ca, const uint16_t* xb, int32_t cb,
                              int32_t r, int32_t up_a, const uint16_t* packed, const float* bias,
                              int32_t cout, int32_t act, float slope, uint16_t* y, float* y32,
-                             hipStream_t s) {
+                             float* scratch, int64_t scratch_floats, hipStream_t s) {
   pm::clear_error();
   return pm::conv::fwd(xa, ca, xb, cb, r, up_a, packed, bias, cout, act, slope, y, y32,
-                       "net.conv_fwd", s);
+                       "net.conv_fwd", s, scratch, scratch ? scratch_floats : 0);
 }
 
 pm_status pm_conv3d_bwd_data_bf16(const uint16_t* dz, int32_t cout, int32_t r,
                                   const uint16_t* packed_t, int32_t cin, float* dx,
-                                  hipStream_t s) {
+                                  float* scratch, int64_t scratch_floats, hipStream_t s) {
   pm::clear_error();
   PM_REQUIRE(dz && packed_t && dx, "pm_conv3d_bwd_data_bf16: null pointer");
   return pm::conv::fwd(dz, pm::conv::store_channels(cout), nullptr, 0, r, 0, packed_t, nullptr,
-                       cin, 0, 0.0f, nullptr, dx, "net.conv_bwd_data", s);
+                       cin, 0, 0.0f, nullptr, dx, "net.conv_bwd_data", s, scratch,
+                       scratch ? scratch_floats : 0);
 }
 
 pm_status pm_conv3d_wgrad_partial_size(int32_t cin, int32_t cout, int32_t r, int64_t* floats) {

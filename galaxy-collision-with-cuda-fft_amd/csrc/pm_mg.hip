@@ -16,6 +16,7 @@
 // float32 4*pi*G*rho as PLAIN mode and the faces are the same float32 values, so both
 // modes solve the identical discrete problem.
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include <cstdio>
@@ -916,8 +917,10 @@ pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStr
 }
 
 // The V-cycle loop of a solve whose setup has run (PLAIN: reference stop rule; MIXED:
-// defect correction with the error-estimate stop). Fills stats.
-pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s) {
+// defect correction with the error-estimate stop). Fills stats. residual_ready: MIXED only,
+// r32 already holds the defect of phi and red[2] its squared norm.
+pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s,
+                        bool residual_ready = false) {
   Level& L0 = mg->lv[0];
   const bool mixed = mg->d.mode == PM_MG_MIXED;
   pm_status st;
@@ -945,7 +948,7 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
     L0.phi = mg->e0;
     L0.rhs = mg->r32;
     mg->last_phi = phi;
-    if ((st = mixed_residual(mg, phi, s))) return st;
+    if (!residual_ready && (st = mixed_residual(mg, phi, s))) return st;
     double prev_e = -1.0;
     for (int c = 0; c < mg->d.n_cycles; ++c) {
       double e_norm, phi_norm;
@@ -1164,21 +1167,31 @@ pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* 
   if (learned) {
     if ((st = pm_wsnet_forward(w->net, mg->net_in, mg->net_out, s))) return st;
     if ((st = pm::warm::apply(mg->net_out, n, w->pool, mg->scales, mg->e0, s))) return st;
-    if ((st = pm::warm::guard(mg->r32, mg->e0, n, w->pool, L0.h, mg->partials, mg->red + 8, s)))
+    // the judge also writes the corrected guess's residual into the level-0 scratch
+    Level& lv0 = mg->lv[0];
+    if ((st = pm::warm::guard(mg->r32, mg->e0, n, w->pool, L0.h, lv0.res, mg->partials,
+                              mg->red + 8, s)))
       return st;
-    double v[10];  // red[8], red[9]: pooled residual before and after the correction
-    if ((st = read_scalars(mg, mg->red, 10, v, s))) return st;
+    double v[11];  // red[8], red[9]: pooled residual before / after; red[10]: ||r after||^2
+    if ((st = read_scalars(mg, mg->red, 11, v, s))) return st;
     w->resid_before = std::sqrt(v[8]);
     w->resid_after = std::sqrt(v[9]);
     if (std::isfinite(w->resid_after) && w->resid_after < w->fallback_ratio * w->resid_before) {
       const unsigned b = pm::red_blocks(nodes);
-      PM_TIMED("mg.update", kUpdBytes * nodes, s);
-      update_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->e0, nodes, mg->partials);
-      PM_CHECK_LAUNCH("update_kernel");
+      {
+        PM_TIMED("mg.update", kUpdBytes * nodes, s);
+        update_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->e0, nodes, mg->partials);
+        PM_CHECK_LAUNCH("update_kernel");
+      }
+      std::swap(mg->r32, lv0.res);  // the defect of the corrected phi
+      PM_TRY_HIP(hipMemcpyAsync(mg->red + 2, mg->red + 10, sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
       w->used_net = 1;
     }
   }
-  if ((st = solve_iterate(mg, phi, stats, s))) return st;
+  // the MIXED loop can start from the defect computed above (of phi, corrected or not)
+  const bool ready = want_net_grid && mg->d.mode == PM_MG_MIXED;
+  if ((st = solve_iterate(mg, phi, stats, s, ready))) return st;
   if (w->pair_target)
     return pm::warm::target(phi, mg->cand, n, w->pool, mg->scales, w->pair_target, s);
   return PM_OK;

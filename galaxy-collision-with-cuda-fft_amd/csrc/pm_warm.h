@@ -16,10 +16,11 @@ pm_status make_input(const float* r32, const float* phi, const float* hist, cons
                      double* red, float* scales, hipStream_t s);
 // e = scales[3] * trilinear(out) on the interior, 0 on the faces.
 pm_status apply(const float* out, int n, int p, const float* scales, float* e, hipStream_t s);
-// red[0], red[1] = sums over the network voxels of the squared mean residual before (r32)
-// and after (r32 - lap(e)) the correction, float64, interior nodes only.
-pm_status guard(const float* r32, const float* e, int n, int p, double h, double* partials,
-                double* red, hipStream_t s);
+// rnew = r32 - lap(e) (float64 arithmetic, 0 on the faces); red[0], red[1] = sums over the
+// network voxels of the squared mean residual before (r32) and after (rnew); red[2] =
+// sum of rnew^2. The pool factor p must divide 64.
+pm_status guard(const float* r32, const float* e, int n, int p, double h, float* rnew,
+                double* partials, double* red, hipStream_t s);
 // tgt[V] = scales[4] * mean over V's fine nodes of (phi - cand).
 pm_status target(const float* phi, const float* cand, int n, int p, const float* scales, float* tgt,
                  hipStream_t s);

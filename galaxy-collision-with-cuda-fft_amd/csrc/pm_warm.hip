@@ -9,6 +9,7 @@
 // rule with edge clamping.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "pm_common.h"
@@ -39,9 +40,46 @@ __global__ void __launch_bounds__(256) extrap_kernel(float* __restrict__ phi,
   if (cand) cand[c] = v;
 }
 
-// Pooled channels [r, phi - hist, f] at coarse voxel V -> pooled[V][3]; float64 partial sums
-// of squares per channel. Grid-stride over coarse voxels (a fixed order for a given launch
-// size); a wave reads p*64 contiguous floats per fine row.
+// Slab decomposition shared by the kernels below: slab (Z, Y, seg) is the p x p x 64 block
+// of fine nodes i = p*Z + a, j = p*Y + b, k = 64*seg + lane (a, b < p), i.e. the fine nodes
+// of 64/p consecutive network voxels along x. Lane = k offset (coalesced rows), wave w takes
+// the (a, b) pairs w, w+4, ...; p divides 64.
+constexpr int kSeg = 64;
+
+struct Slab {
+  int Z, Y, seg;
+};
+
+__device__ __forceinline__ Slab slab_of(int64_t slab, int R, int nseg) {
+  Slab q;
+  q.seg = static_cast<int>(slab % nseg);
+  q.Y = static_cast<int>((slab / nseg) % R);
+  q.Z = static_cast<int>(slab / (static_cast<int64_t>(nseg) * R));
+  return q;
+}
+
+// Sums Q per-thread values over the block's 4 waves (LDS) and then over aligned groups of p
+// lanes (shuffles): on return lane X*p of wave 0 holds the sum over the p^3 fine nodes of
+// network voxel X of the slab. Contains __syncthreads(): uniform control flow only.
+template <int Q>
+__device__ __forceinline__ void slab_reduce(float (&t)[Q], int p, float (*s)[kSeg][Q]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) s[wave][lane][q] = t[q];
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) t[q] = ((s[0][lane][q] + s[1][lane][q]) + s[2][lane][q]) + s[3][lane][q];
+    for (int off = 1; off < p; off <<= 1) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) t[q] += __shfl_down(t[q], off, 64);
+    }
+  }
+  __syncthreads();
+}
+
+// Pooled channels [r, phi - hist, f] of every network voxel -> pooled[V][3]; float64
+// partial sums of their squares (grid-stride over slabs, fixed order per launch size).
 __global__ void __launch_bounds__(256) pool_input_kernel(const float* __restrict__ r32,
                                                          const float* __restrict__ phi,
                                                          const float* __restrict__ hist,
@@ -49,33 +87,39 @@ __global__ void __launch_bounds__(256) pool_input_kernel(const float* __restrict
                                                          int p, int R,
                                                          float* __restrict__ pooled,
                                                          double* __restrict__ partials) {
+  __shared__ float s[4][kSeg][3];
   __shared__ double s_red[4 * 3];
   double v[3] = {0.0, 0.0, 0.0};
-  const int64_t nv = static_cast<int64_t>(R) * R * R;
-  for (int64_t V = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; V < nv;
-       V += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int X = static_cast<int>(V % R), Y = static_cast<int>((V / R) % R),
-              Z = static_cast<int>(V / (static_cast<int64_t>(R) * R));
-    float sr = 0.0f, sd = 0.0f, sf = 0.0f;
-    for (int a = 0; a < p; ++a)
-      for (int b = 0; b < p; ++b) {
-        const int64_t base = (static_cast<int64_t>(p * Z + a) * n + (p * Y + b)) * n + p * X;
-        for (int c = 0; c < p; ++c) {
-          sr += r32[base + c];
-          sd += phi[base + c] - hist[base + c];
-          sf += f[base + c];
-        }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nseg = (n + kSeg - 1) / kSeg;
+  const int64_t slabs = static_cast<int64_t>(R) * R * nseg;
+  const float inv = 1.0f / static_cast<float>(p * p * p);
+  for (int64_t sl = blockIdx.x; sl < slabs; sl += gridDim.x) {
+    const Slab q = slab_of(sl, R, nseg);
+    const int k = q.seg * kSeg + lane;
+    float t[3] = {0.0f, 0.0f, 0.0f};
+    if (k < n) {
+      for (int ab = wave; ab < p * p; ab += 4) {
+        const int i = p * q.Z + ab / p, j = p * q.Y + ab % p;
+        const int64_t c = (static_cast<int64_t>(i) * n + j) * n + k;
+        t[0] += r32[c];
+        t[1] += phi[c] - hist[c];
+        t[2] += f[c];
       }
-    const float inv = 1.0f / static_cast<float>(p * p * p);
-    const float o[3] = {sr * inv, sd * inv, sf * inv};
-    for (int q = 0; q < 3; ++q) {
-      pooled[V * 3 + q] = o[q];
-      v[q] = v[q] + static_cast<double>(o[q]) * static_cast<double>(o[q]);
+    }
+    slab_reduce<3>(t, p, s);
+    if (wave == 0 && k < n && lane % p == 0) {
+      const int64_t V = (static_cast<int64_t>(q.Z) * R + q.Y) * R + k / p;
+      for (int c = 0; c < 3; ++c) {
+        const float o = t[c] * inv;
+        pooled[V * 3 + c] = o;
+        v[c] = v[c] + static_cast<double>(o) * static_cast<double>(o);
+      }
     }
   }
   pm::block_sum<3>(v, s_red);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 3; ++q) partials[blockIdx.x * pm::kRedMaxQ + q] = v[q];
+    for (int c = 0; c < 3; ++c) partials[blockIdx.x * pm::kRedMaxQ + c] = v[c];
 }
 
 // scales[q] = RMS of pooled channel q (q < 3); scales[3] = RMS(r) * (p*h)^2, the output
@@ -110,7 +154,8 @@ __global__ void __launch_bounds__(256) normalize_kernel(const float* __restrict_
 }
 
 __device__ __forceinline__ void interp_axis(int i, int p, int R, int& i0, int& i1, float& w) {
-  const float u = (static_cast<float>(i) - 0.5f * static_cast<float>(p - 1)) / static_cast<float>(p);
+  // u = (i - (p-1)/2) / p: the node's coordinate in network voxels (cell centres)
+  const float u = static_cast<float>(2 * i - (p - 1)) / static_cast<float>(2 * p);
   const float f = floorf(u);
   w = u - f;
   const int a = static_cast<int>(f);
@@ -118,81 +163,95 @@ __device__ __forceinline__ void interp_axis(int i, int p, int R, int& i0, int& i
   i1 = a + 1 < 0 ? 0 : (a + 1 > R - 1 ? R - 1 : a + 1);
 }
 
-// e[c] = scales[3] * trilinear(out)(node c) on the interior, 0 on the faces.
+// e = scales[3] * trilinear(out) on the interior, 0 on the faces; one slab per block.
 __global__ void __launch_bounds__(256) apply_kernel(const float* __restrict__ out, int R, int p,
                                                     int n, const float* __restrict__ scales,
                                                     float* __restrict__ e) {
-  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
-  const int64_t total = static_cast<int64_t>(n) * n * n;
-  if (c >= total) return;
-  const pm::Idx3 q = pm::idx3(c, n);
-  float val = 0.0f;
-  if (q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2 && q.k >= 1 && q.k <= n - 2) {
-    int z0, z1, y0, y1, x0, x1;
-    float wz, wy, wx;
-    interp_axis(q.i, p, R, z0, z1, wz);
-    interp_axis(q.j, p, R, y0, y1, wy);
-    interp_axis(q.k, p, R, x0, x1, wx);
-    auto at = [&](int z, int y, int x) {
-      return out[(static_cast<int64_t>(z) * R + y) * R + x];
-    };
-    const float c00 = at(z0, y0, x0) * (1.0f - wx) + at(z0, y0, x1) * wx;
-    const float c01 = at(z0, y1, x0) * (1.0f - wx) + at(z0, y1, x1) * wx;
-    const float c10 = at(z1, y0, x0) * (1.0f - wx) + at(z1, y0, x1) * wx;
-    const float c11 = at(z1, y1, x0) * (1.0f - wx) + at(z1, y1, x1) * wx;
-    const float c0 = c00 * (1.0f - wy) + c01 * wy;
-    const float c1 = c10 * (1.0f - wy) + c11 * wy;
-    val = scales[3] * (c0 * (1.0f - wz) + c1 * wz);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nseg = (n + kSeg - 1) / kSeg;
+  const Slab q = slab_of(blockIdx.x, R, nseg);
+  const int k = q.seg * kSeg + lane;
+  if (k >= n) return;  // no barriers below
+  int x0, x1;
+  float wx;
+  interp_axis(k, p, R, x0, x1, wx);
+  const float so = scales[3];
+  for (int ab = wave; ab < p * p; ab += 4) {
+    const int i = p * q.Z + ab / p, j = p * q.Y + ab % p;
+    const int64_t c = (static_cast<int64_t>(i) * n + j) * n + k;
+    float val = 0.0f;
+    if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
+      int z0, z1, y0, y1;
+      float wz, wy;
+      interp_axis(i, p, R, z0, z1, wz);
+      interp_axis(j, p, R, y0, y1, wy);
+      const float* r00 = out + (static_cast<int64_t>(z0) * R + y0) * R;
+      const float* r01 = out + (static_cast<int64_t>(z0) * R + y1) * R;
+      const float* r10 = out + (static_cast<int64_t>(z1) * R + y0) * R;
+      const float* r11 = out + (static_cast<int64_t>(z1) * R + y1) * R;
+      const float c00 = r00[x0] * (1.0f - wx) + r00[x1] * wx;
+      const float c01 = r01[x0] * (1.0f - wx) + r01[x1] * wx;
+      const float c10 = r10[x0] * (1.0f - wx) + r10[x1] * wx;
+      const float c11 = r11[x0] * (1.0f - wx) + r11[x1] * wx;
+      const float c0 = c00 * (1.0f - wy) + c01 * wy;
+      const float c1 = c10 * (1.0f - wy) + c11 * wy;
+      val = so * (c0 * (1.0f - wz) + c1 * wz);
+    }
+    e[c] = val;
   }
-  e[c] = val;
 }
 
-// The judge of a learned correction e: per network voxel V, the mean over its fine nodes
-// of the residual before (r) and after (r - lap(e)) the correction, evaluated in float64 on
-// the interior; float64 partial sums of the squares of both means. The pooled residual
-// measures the smooth error the correction targets (the full-resolution residual norm is
-// dominated by high-frequency components that the smoother removes anyway).
+// The judge of a learned correction e. Per fine node: r' = r - lap(e) in float64 on the
+// interior (0 on the faces), written to rnew (the residual of the corrected guess); per
+// network voxel: the mean residual before (r) and after (r'). Float64 partial sums of
+// [mean before^2, mean after^2, r'^2]. The pooled residual measures the smooth error the
+// correction targets (the full-resolution residual norm is dominated by high-frequency
+// components that the smoother removes anyway).
 __global__ void __launch_bounds__(256) guard_kernel(const float* __restrict__ r32,
                                                     const float* __restrict__ e, int n, int p,
                                                     int R, double inv_h2,
+                                                    float* __restrict__ rnew,
                                                     double* __restrict__ partials) {
-  __shared__ double s_red[4 * 2];
-  double v[2] = {0.0, 0.0};
+  __shared__ float s[4][kSeg][2];
+  __shared__ double s_red[4 * 3];
+  double v[3] = {0.0, 0.0, 0.0};
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nseg = (n + kSeg - 1) / kSeg;
+  const int64_t slabs = static_cast<int64_t>(R) * R * nseg;
   const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int64_t nv = static_cast<int64_t>(R) * R * R;
-  for (int64_t V = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; V < nv;
-       V += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int X = static_cast<int>(V % R), Y = static_cast<int>((V / R) % R),
-              Z = static_cast<int>(V / (static_cast<int64_t>(R) * R));
-    double sb = 0.0, sa = 0.0;
-    for (int a = 0; a < p; ++a) {
-      const int i = p * Z + a;
-      for (int b = 0; b < p; ++b) {
-        const int j = p * Y + b;
-        for (int c = 0; c < p; ++c) {
-          const int k = p * X + c;
-          if (i < 1 || i > n - 2 || j < 1 || j > n - 2 || k < 1 || k > n - 2) continue;
-          const int64_t q = (static_cast<int64_t>(i) * n + j) * n + k;
-          const double sum = static_cast<double>(e[q + n2]) + static_cast<double>(e[q - n2]) +
-                             static_cast<double>(e[q + n]) + static_cast<double>(e[q - n]) +
-                             static_cast<double>(e[q + 1]) + static_cast<double>(e[q - 1]);
-          const double r = static_cast<double>(r32[q]);
-          sb += r;
-          sa += r - (sum - 6.0 * static_cast<double>(e[q])) * inv_h2;
+  const float inv = 1.0f / static_cast<float>(p * p * p);
+  for (int64_t sl = blockIdx.x; sl < slabs; sl += gridDim.x) {
+    const Slab q = slab_of(sl, R, nseg);
+    const int k = q.seg * kSeg + lane;
+    float t[2] = {0.0f, 0.0f};
+    if (k < n) {
+      for (int ab = wave; ab < p * p; ab += 4) {
+        const int i = p * q.Z + ab / p, j = p * q.Y + ab % p;
+        const int64_t c = (static_cast<int64_t>(i) * n + j) * n + k;
+        double ra = 0.0, rb = 0.0;
+        if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
+          const double sum = static_cast<double>(e[c + n2]) + static_cast<double>(e[c - n2]) +
+                             static_cast<double>(e[c + n]) + static_cast<double>(e[c - n]) +
+                             static_cast<double>(e[c + 1]) + static_cast<double>(e[c - 1]);
+          rb = static_cast<double>(r32[c]);
+          ra = rb - (sum - 6.0 * static_cast<double>(e[c])) * inv_h2;
         }
+        rnew[c] = static_cast<float>(ra);
+        v[2] = v[2] + ra * ra;
+        t[0] += static_cast<float>(rb);
+        t[1] += static_cast<float>(ra);
       }
     }
-    const double inv = 1.0 / (static_cast<double>(p) * p * p);
-    sb *= inv;
-    sa *= inv;
-    v[0] = v[0] + sb * sb;
-    v[1] = v[1] + sa * sa;
+    slab_reduce<2>(t, p, s);
+    if (wave == 0 && k < n && lane % p == 0) {
+      const double mb = static_cast<double>(t[0] * inv), ma = static_cast<double>(t[1] * inv);
+      v[0] = v[0] + mb * mb;
+      v[1] = v[1] + ma * ma;
+    }
   }
-  pm::block_sum<2>(v, s_red);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x * pm::kRedMaxQ] = v[0];
-    partials[blockIdx.x * pm::kRedMaxQ + 1] = v[1];
-  }
+  pm::block_sum<3>(v, s_red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) partials[blockIdx.x * pm::kRedMaxQ + c] = v[c];
 }
 
 // tgt[V] = mean over V's fine nodes of (phi - cand), times scales[4].
@@ -221,7 +280,7 @@ namespace warm {
 
 pm_status extrapolate(float* phi, float* hist, float* cand, int n, hipStream_t s) {
   const int64_t total = static_cast<int64_t>(n) * n * n;
-  PM_TIMED("warm.extrapolate", (cand ? 16.0 : 12.0) * static_cast<double>(total), s);
+  PM_TIMED("warm.extrapolate", (cand ? 20.0 : 16.0) * static_cast<double>(total), s);
   extrap_kernel<<<pm::blocks_for(total), pm::kBlock, 0, s>>>(phi, hist, cand, n);
   PM_CHECK_LAUNCH("extrap_kernel");
   return PM_OK;
@@ -230,10 +289,13 @@ pm_status extrapolate(float* phi, float* hist, float* cand, int n, hipStream_t s
 pm_status make_input(const float* r32, const float* phi, const float* hist, const float* f, int n,
                      int p, double h, float* pooled, uint16_t* input, double* partials,
                      double* red, float* scales, hipStream_t s) {
-  PM_REQUIRE(p >= 1 && n % p == 0, "warm start: grid size must be a multiple of the pool factor");
+  PM_REQUIRE(p >= 1 && p <= kSeg && kSeg % p == 0 && n % p == 0,
+             "warm start: the pool factor must divide 64 and the grid size");
   const int R = n / p;
   const int64_t nv = static_cast<int64_t>(R) * R * R;
-  const unsigned blocks = pm::red_blocks(nv);
+  const int nseg = (n + kSeg - 1) / kSeg;
+  const int64_t slabs = static_cast<int64_t>(R) * R * nseg;
+  const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(slabs, pm::kRedBlocks));
   {
     PM_TIMED("warm.input", 16.0 * static_cast<double>(n) * n * n, s);
     pool_input_kernel<<<blocks, pm::kBlock, 0, s>>>(r32, phi, hist, f, n, p, R, pooled, partials);
@@ -249,25 +311,26 @@ pm_status make_input(const float* r32, const float* phi, const float* hist, cons
 }
 
 pm_status apply(const float* out, int n, int p, const float* scales, float* e, hipStream_t s) {
-  const int64_t total = static_cast<int64_t>(n) * n * n;
-  PM_TIMED("warm.apply", 4.0 * static_cast<double>(total), s);
-  apply_kernel<<<pm::blocks_for(total), pm::kBlock, 0, s>>>(out, n / p, p, n, scales, e);
+  const int R = n / p;
+  const int64_t slabs = static_cast<int64_t>(R) * R * ((n + kSeg - 1) / kSeg);
+  PM_TIMED("warm.apply", 4.0 * static_cast<double>(n) * n * n, s);
+  apply_kernel<<<static_cast<unsigned>(slabs), pm::kBlock, 0, s>>>(out, R, p, n, scales, e);
   PM_CHECK_LAUNCH("apply_kernel");
   return PM_OK;
 }
 
-pm_status guard(const float* r32, const float* e, int n, int p, double h, double* partials,
-                double* red, hipStream_t s) {
+pm_status guard(const float* r32, const float* e, int n, int p, double h, float* rnew,
+                double* partials, double* red, hipStream_t s) {
   const int R = n / p;
-  const int64_t nv = static_cast<int64_t>(R) * R * R;
-  const unsigned b = pm::red_blocks(nv);
+  const int64_t slabs = static_cast<int64_t>(R) * R * ((n + kSeg - 1) / kSeg);
+  const unsigned b = static_cast<unsigned>(std::min<int64_t>(slabs, pm::kRedBlocks));
   {
-    PM_TIMED("warm.guard", 8.0 * static_cast<double>(n) * n * n, s);
-    guard_kernel<<<b, pm::kBlock, 0, s>>>(r32, e, n, p, R, 1.0 / (h * h), partials);
+    PM_TIMED("warm.guard", 12.0 * static_cast<double>(n) * n * n, s);
+    guard_kernel<<<b, pm::kBlock, 0, s>>>(r32, e, n, p, R, 1.0 / (h * h), rnew, partials);
     PM_CHECK_LAUNCH("guard_kernel");
   }
-  pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(b), red);
-  PM_CHECK_LAUNCH("fold_partials_kernel<2>");
+  pm::fold_partials_kernel<3><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(b), red);
+  PM_CHECK_LAUNCH("fold_partials_kernel<3>");
   return PM_OK;
 }
 

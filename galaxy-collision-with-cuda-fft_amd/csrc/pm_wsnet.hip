@@ -61,8 +61,10 @@ struct pm_wsnet {
   uint16_t* db[3] = {};
   uint16_t* dz = nullptr;         // output gradient of the layer being differentiated
   float* g = nullptr;             // gradient of an upsampled tensor (coarse)
-  float* partial = nullptr;
+  float* partial = nullptr;        // backward-weight partial sums
   int64_t partial_floats = 0;
+  float* fscratch = nullptr;       // split-K partial sums of forward / backward-data passes
+  int64_t fscratch_floats = 0;
   float* dout_copy = nullptr;
   bool have_params = false;
   bool have_forward = false;
@@ -112,7 +114,8 @@ pm_status layer_bwd(pm_wsnet* net, int i, float* grads, int accumulate, bool nee
   if (!need_dx) return PM_OK;
   const std::string td = timer_name("bwd_data", l);
   return pm::conv::fwd(net->dz, pm::conv::store_channels(l.cout), nullptr, 0, l.r, 0, l.wpt,
-                       nullptr, l.ca + l.cb, 0, 0.0f, nullptr, l.dx, td.c_str(), s);
+                       nullptr, l.ca + l.cb, 0, 0.0f, nullptr, l.dx, td.c_str(), s,
+                       net->fscratch, net->fscratch_floats);
 }
 
 }  // namespace
@@ -199,8 +202,13 @@ pm_status pm_wsnet_create(const pm_wsnet_desc* desc, pm_wsnet** out) {
     dz_elems = std::max<int64_t>(dz_elems, vox(l.r) * pm::conv::store_channels(l.cout));
     if (l.up) g_elems = std::max<int64_t>(g_elems, vox(l.r / 2) * l.ca);
     part = std::max<int64_t>(part, pm::conv::wgrad_partial_floats(l.ca + l.cb, l.cout, l.r));
+    net->fscratch_floats = std::max<int64_t>(
+        net->fscratch_floats,
+        std::max(pm::conv::fwd_scratch_floats(l.ca + l.cb, l.cout, l.r),
+                 pm::conv::fwd_scratch_floats(pm::conv::store_channels(l.cout), l.ca + l.cb, l.r)));
   }
   net->partial_floats = part;
+  if ((st = f32(&net->fscratch, std::max<int64_t>(net->fscratch_floats, 4)))) return fail(st);
   if ((st = bf(&net->dz, dz_elems)) || (st = f32(&net->g, g_elems)) ||
       (st = f32(&net->partial, part)) || (st = f32(&net->dout_copy, vox(R[0]))))
     return fail(st);
@@ -290,7 +298,8 @@ pm_status pm_wsnet_forward(pm_wsnet* net, const uint16_t* input, float* out, hip
     const Layer& l = net->L[i];
     const std::string t = timer_name("fwd", l);
     return pm::conv::fwd(l.xa, l.ca, l.xb, l.cb, l.r, l.up, l.wp, net->params + l.b_off, l.cout,
-                         l.act, slope, l.y, y32, t.c_str(), s);
+                         l.act, slope, l.y, y32, t.c_str(), s, net->fscratch,
+                         net->fscratch_floats);
   };
   for (int lv = 0; lv < 4; ++lv) {
     if ((st = conv(2 * lv, nullptr)) || (st = conv(2 * lv + 1, nullptr))) return st;

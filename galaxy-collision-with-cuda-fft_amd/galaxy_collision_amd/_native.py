@@ -163,9 +163,10 @@ _SIGNATURES = {
     "pm_copy_f32": [_vp, _vp, _i64, _vp],
     "pm_conv3d_packed_size": [_i32, _i32, _i32, ctypes.POINTER(_i64)],
     "pm_conv3d_pack_weights": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "pm_conv3d_scratch_size": [_i32, _i32, _i32, ctypes.POINTER(_i64)],
     "pm_conv3d_fwd_bf16": [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _vp, _i32, _i32, _f, _vp, _vp,
-                           _vp],
-    "pm_conv3d_bwd_data_bf16": [_vp, _i32, _i32, _vp, _i32, _vp, _vp],
+                           _vp, _i64, _vp],
+    "pm_conv3d_bwd_data_bf16": [_vp, _i32, _i32, _vp, _i32, _vp, _vp, _i64, _vp],
     "pm_conv3d_wgrad_partial_size": [_i32, _i32, _i32, ctypes.POINTER(_i64)],
     "pm_conv3d_bwd_weight_bf16": [_vp, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _i32,
                                   _vp, _i64, _vp],

@@ -50,6 +50,7 @@ class _NativeMultigrid(PoissonSolver):
         self.pool = int(pool)
         self.fallback_ratio = float(fallback_ratio)
         self.last_warm: dict | None = None
+        self.pairs = None  # a warmstart.PairSet: warm solves then record training pairs
         self.grid_size = int(grid_size)
         self.dx = float(dx)
         self.G = units.G if grav_constant is None else float(grav_constant)
@@ -83,7 +84,12 @@ class _NativeMultigrid(PoissonSolver):
     def solve(self, rho, phi, warm_start: bool = False) -> None:
         """phi <- solution; with warm_start the configured warm mode builds the guess."""
         if warm_start and self.warm != "previous":
-            self.solve_warm(rho, phi, _WARM_MODES[self.warm])
+            if self.pairs is None:
+                self.solve_warm(rho, phi, _WARM_MODES[self.warm])
+            else:
+                x, t = self.pairs.new_buffers()
+                if self.solve_warm(rho, phi, _WARM_MODES[self.warm], x, t)["extrapolated"]:
+                    self.pairs.add(x, t)
             return
         r, p = as_tensor(rho), as_tensor(phi)
         self._check_grid(r, "rho")

@@ -185,19 +185,23 @@ def relative_error(net: WarmStartNet, pairs: PairSet, idx) -> float:
 
 
 def train(net: WarmStartNet, pairs: PairSet, *, epochs: int = 10, batch: int = 4,
-          lr: float = 1e-3, clip: float = 1.0, seed: int = 0, val_idx=None,
-          log=None) -> list[dict]:
+          lr: float = 1e-3, lr_final: float | None = None, clip: float = 1.0, seed: int = 0,
+          val_idx=None, log=None) -> list[dict]:
     """Fit the network to the pairs (MSE on the normalized target) with Adam.
 
     Gradients come from pm_wsnet_backward, accumulated over each minibatch and clipped to a
     global norm of ``clip`` (the network has no normalization layers; unclipped Adam steps
-    can diverge); returns one record per epoch with the training loss and, if val_idx is
-    given, the validation relative error.
+    can diverge). The learning rate follows a cosine from ``lr`` to ``lr_final`` (default
+    lr/10). Returns one record per epoch with the training loss and, if val_idx is given, the
+    validation relative error.
     """
     rng = np.random.default_rng(seed)
     val = set(val_idx or [])
     train_idx = [i for i in range(len(pairs)) if i not in val]
     opt = Adam(net.params, lr=lr)
+    lr_end = lr / 10.0 if lr_final is None else float(lr_final)
+    total = max(1, epochs * math.ceil(len(train_idx) / batch))
+    step = 0
     dout = torch.empty_like(net.out)
     hist = []
     for ep in range(epochs):
@@ -215,6 +219,8 @@ def train(net: WarmStartNet, pairs: PairSet, *, epochs: int = 10, batch: int = 4
             if clip > 0:
                 gn = net.grads.norm()
                 net.grads.mul_(torch.clamp(clip / (gn + 1e-12), max=1.0))
+            opt.lr = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * step / total))
+            step += 1
             opt.step(net.grads)
             net.sync_params()
         rec = {"epoch": ep, "loss": tot / max(1, len(order))}

@@ -281,18 +281,22 @@ pm_status pm_copy_f32(float* dst, const float* src, int64_t count, hipStream_t s
 pm_status pm_conv3d_packed_size(int32_t cout, int32_t cin, int32_t transpose, int64_t* elems);
 pm_status pm_conv3d_pack_weights(const float* w, int32_t cout, int32_t cin, int32_t transpose,
                                  uint16_t* packed, hipStream_t stream);
+/* Float32 scratch that lets the two calls below split their reduction over workgroups on
+ * small grids (split-K, partial sums folded in a fixed order); without it (scratch null or
+ * smaller) they run unsplit. Same result either way up to float32 summation order. */
+pm_status pm_conv3d_scratch_size(int32_t cin, int32_t cout, int32_t r, int64_t* floats);
 /* y = act(conv(x) + bias), act 1 = leaky ReLU with negative slope `slope`, 0 = identity.
  * Outputs (either may be null): y bf16 [r^3][roundup8(cout)] (pad channels 0), y32 float32
- * [r^3][cout]. bias may be null. */
+ * [r^3][cout]. bias and scratch may be null. */
 pm_status pm_conv3d_fwd_bf16(const uint16_t* xa, int32_t ca, const uint16_t* xb, int32_t cb,
                              int32_t r, int32_t up_a, const uint16_t* packed, const float* bias,
                              int32_t cout, int32_t act, float slope, uint16_t* y, float* y32,
-                             hipStream_t stream);
+                             float* scratch, int64_t scratch_floats, hipStream_t stream);
 /* dx float32 [r^3][cin] = gradient of the (un-upsampled, concatenated) convolution input
  * for the output gradient dz (bf16 [r^3][roundup8(cout)]); packed_t from transpose = 1. */
 pm_status pm_conv3d_bwd_data_bf16(const uint16_t* dz, int32_t cout, int32_t r,
                                   const uint16_t* packed_t, int32_t cin, float* dx,
-                                  hipStream_t stream);
+                                  float* scratch, int64_t scratch_floats, hipStream_t stream);
 /* Floats of scratch pm_conv3d_bwd_weight_bf16 needs for these shapes. */
 pm_status pm_conv3d_wgrad_partial_size(int32_t cin, int32_t cout, int32_t r, int64_t* floats);
 /* dw float32 [cout][cin][27] and db float32 [cout] (may be null) of the convolution for the

@@ -65,6 +65,17 @@ def store(c):
     return (c + 7) // 8 * 8
 
 
+def scratch_for(cin, cout, r, split, dev):
+    """Split-K scratch (or None): small grids then reduce over several workgroups."""
+    if not split:
+        return None, 0
+    nat = _nat()
+    n = ctypes.c_int64()
+    nat.call("pm_conv3d_scratch_size", cin, cout, r, ctypes.byref(n))
+    buf = torch.empty(max(n.value, 1), dtype=torch.float32, device=dev)
+    return buf, n.value
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
@@ -86,9 +97,10 @@ FWD_CASES = [
 ]
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("r,ca,cb,up,cout,act", FWD_CASES)
-def test_conv_forward_matches_float64(device, r, ca, cb, up, cout, act):
-    announce(f"conv fwd r={r} ca={ca} cb={cb} up={up} cout={cout} act={act}")
+def test_conv_forward_matches_float64(device, r, ca, cb, up, cout, act, split):
+    announce(f"conv fwd r={r} ca={ca} cb={cb} up={up} cout={cout} act={act} split={split}")
     nat = _nat()
     g = torch.Generator().manual_seed(100 + r * 7 + ca + cout)
     ra = r // 2 if up else r
@@ -104,10 +116,13 @@ def test_conv_forward_matches_float64(device, r, ca, cb, up, cout, act):
     packed = pack(w_d, cout, cin, 0, device)
     y = torch.empty(r ** 3 * store(cout), dtype=torch.bfloat16, device=device)
     y32 = torch.empty(r ** 3 * cout, dtype=torch.float32, device=device)
+    scr, nscr = scratch_for(cin, cout, r, split, device)
+    if split:
+        assert nscr > 0, "these small grids split their reduction"
     nat.call("pm_conv3d_fwd_bf16", nat.ptr(xa_d, torch.bfloat16), ca,
              nat.ptr(xb_d, torch.bfloat16) if cb else None, cb, r, up,
              nat.ptr(packed, torch.bfloat16), nat.ptr(b_d), cout, act, slope,
-             nat.ptr(y, torch.bfloat16), nat.ptr(y32), nat.stream())
+             nat.ptr(y, torch.bfloat16), nat.ptr(y32), nat.ptr(scr), nscr, nat.stream())
     torch.cuda.synchronize()
     x = concat_input(xa, ca, xb, cb, r, up)
     ref = F.conv3d(x, wb.double(), bias.double(), padding=1)
@@ -138,9 +153,10 @@ BWD_CASES = [
 ]
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("r,ca,cb,up,cout", BWD_CASES)
-def test_conv_backward_matches_float64_autograd(device, r, ca, cb, up, cout):
-    announce(f"conv bwd r={r} ca={ca} cb={cb} up={up} cout={cout}")
+def test_conv_backward_matches_float64_autograd(device, r, ca, cb, up, cout, split):
+    announce(f"conv bwd r={r} ca={ca} cb={cb} up={up} cout={cout} split={split}")
     nat = _nat()
     g = torch.Generator().manual_seed(7 + r + ca * 3 + cout)
     ra = r // 2 if up else r
@@ -168,8 +184,10 @@ def test_conv_backward_matches_float64_autograd(device, r, ca, cb, up, cout):
     w_d = w32.to(device)
     packed_t = pack(w_d, cout, cin, 1, device)
     dx = torch.empty(r ** 3 * cin, dtype=torch.float32, device=device)
+    scr, nscr = scratch_for(cin, cout, r, split, device)
     nat.call("pm_conv3d_bwd_data_bf16", nat.ptr(dz_d, torch.bfloat16), cout, r,
-             nat.ptr(packed_t, torch.bfloat16), cin, nat.ptr(dx), nat.stream())
+             nat.ptr(packed_t, torch.bfloat16), cin, nat.ptr(dx), nat.ptr(scr), nscr,
+             nat.stream())
     nfl = ctypes.c_int64()
     nat.call("pm_conv3d_wgrad_partial_size", cin, cout, r, ctypes.byref(nfl))
     part = torch.empty(nfl.value, dtype=torch.float32, device=device)
@@ -240,4 +258,4 @@ def test_conv_argument_checks(device):
     with pytest.raises(nat.PMChainError, match="multiples of 8"):
         nat.call("pm_conv3d_fwd_bf16", nat.ptr(x, torch.bfloat16), 12, None, 0, 8, 0,
                  nat.ptr(x, torch.bfloat16), None, 16, 0, 0.0, nat.ptr(x, torch.bfloat16), None,
-                 nat.stream())
+                 None, 0, nat.stream())
