@@ -10,7 +10,9 @@ Workloads (seeded synthetic inputs, built on the host, resident in HBM before ti
 A step is one fused KDK step (kick+drift, deposit, Poisson solve, -grad, gather+kick).
 Pass 1 times --steps steps with no instrumentation (the reported value). Pass 2 repeats
 them with per-kernel HIP events on the launch stream, giving per-kernel times and the
-roofline figures of the dominant kernel. The CPU baseline runs the reference algorithm
+roofline figures of the dominant kernel. Pass 3 (collisions) restarts from the initial
+state once per multigrid warm start (previous phi, extrapolation, extrapolation plus the
+trained network) and for the float32 reference multigrid: the learned-vs-plain comparison. The CPU baseline runs the reference algorithm
 (OpenMP C port: atomic CIC scatter, plain float32 multigrid tol 1e-3) on the same input.
 
 Prints one JSON line. ``--gpus`` above 1 is refused: this build runs on one GPU.
@@ -54,11 +56,88 @@ def build_ic(w: dict, seed: int):
     return ic.sort_by_cell(icr, n, dx), dx
 
 
+# rocprofv3 kernel symbol of a timed kernel name (the finest-grid dispatch is the timed one)
+_PROFILE_KERNELS = {"mg.rb_pass": "rb_pass_kernel", "gather_kick": "gather_kick_kernel",
+                    "mg.residual": "residual_kernel", "mg.prolong": "prolong_add_kernel",
+                    "deposit.radix_scatter": "radix_scatter_kernel"}
+
+
+def measured_traffic(workload: str, grid: int, timer_name: str):
+    """HBM bytes per dispatch of the timed kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/), or None."""
+    path = ROOT / "profiles" / f"pm_step_{workload}_{grid}_hbm_traffic.json"
+    sym = _PROFILE_KERNELS.get(timer_name.split("[")[0])
+    if sym is None or not path.exists():
+        return None
+    table = json.loads(path.read_text()).get("hbm_bytes_per_dispatch", {})
+    hits = [(int(k.rsplit("@", 1)[1]), v) for k, v in table.items() if k.rsplit("@", 1)[0] == sym]
+    return max(hits)[1] if hits else None
+
+
 def phi_accuracy(chain, grav: float) -> dict:
     """phi of the last step against the exact discrete solution and the continuum FFT."""
     from galaxy_collision_amd.accuracy import phi_accuracy as acc
 
     return acc(chain.grid.rho, chain.grid.phi, chain.n, chain.dx, grav)
+
+
+def warm_start_ab(icr, w: dict, dx: float, grav: float, steps: int) -> dict:
+    """The same collision from the same initial state, stepped with each multigrid warm start
+    (previous phi as in the reference, time extrapolation, extrapolation plus the trained
+    warm-start network) at the same tolerance, and with the float32 reference multigrid at
+    its default tolerance: V-cycles and wall time per step, phi error of the last step."""
+    import numpy as np
+    import torch
+
+    from galaxy_collision_amd.accuracy import phi_rel_error_discrete
+    from galaxy_collision_amd.chain import PMChain
+    from galaxy_collision_amd.data import ParticleState
+    from galaxy_collision_amd.solver import make_solver
+    from galaxy_collision_amd.warmstart import WEIGHTS_DIR, WarmStartNet
+
+    pool = 4
+    variants = [("multigrid_mp", "previous"), ("multigrid_mp", "extrapolate")]
+    weights = WEIGHTS_DIR / f"wsnet_r{w['grid'] // pool}.npz"
+    net = WarmStartNet.load(weights) if weights.exists() else None
+    if net is not None:
+        variants.append(("multigrid_mp", "learned"))
+    variants.append(("multigrid", "previous"))
+    out = {}
+    for name, warm in variants:
+        parts = ParticleState(icr.n)
+        parts.load(icr.pos, icr.vel, icr.mass, icr.gid)
+        kw = {"dx": dx, "grav_constant": grav}
+        if name == "multigrid_mp":
+            kw.update(warm=warm, pool=pool, net=net if warm == "learned" else None)
+        solver = make_solver(name, w["grid"], **kw)
+        chain = PMChain(parts, w["grid"], dx, solver)
+        chain.prime()
+        for _ in range(3):
+            chain.step(w["dt"])
+        torch.cuda.synchronize()
+        chain.last_cycles.clear()
+        kept = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            chain.step(w["dt"])
+            lw = getattr(solver, "last_warm", None)
+            if warm == "learned" and lw and lw["extrapolated"]:
+                kept.append(lw["used_net"])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        key = f"{name}:{warm}" if name == "multigrid_mp" else f"{name}:tol{solver.tol:g}"
+        out[key] = {
+            "vcycles_per_step": float(np.mean(chain.last_cycles)),
+            "ms_per_step": wall / steps * 1e3,
+            "phi_rel_l2_vs_exact_discrete": phi_rel_error_discrete(
+                chain.grid.rho.to_numpy(), chain.grid.phi.to_numpy(), dx, grav),
+        }
+        if kept:
+            out[key]["net_kept_fraction"] = float(np.mean(kept))
+        print(f"[bench] A/B {key}: {out[key]}", file=sys.stderr, flush=True)
+        del chain, solver, parts
+        torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(icr, w: dict, dx: float, budget_s: float, threads: int | None) -> dict:
@@ -99,6 +178,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the phi accuracy check")
     ap.add_argument("--no-kernel-pass", action="store_true", help="skip the per-kernel pass")
+    ap.add_argument("--no-ab", action="store_true",
+                    help="skip the warm-start A/B pass (learned vs plain multigrid)")
+    ap.add_argument("--ab-steps", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy a 256 MiB buffer 3 times (a known-size kernel for scaling "
                          "profiler byte counters)")
@@ -196,7 +278,8 @@ def main() -> int:
         per_launch = r["bytes"] / r["launches"]
         achieved = per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(args.workload, w["grid"], name),
                     "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": avg_s * 1e6}
         print(f"[bench] pass 2: {pass2_ms:.3f} ms/step; dominant {name} {achieved:.0f} GB/s",
               file=sys.stderr, flush=True)
@@ -206,6 +289,22 @@ def main() -> int:
         acc = phi_accuracy(chain, grav)
         print(f"[bench] accuracy {acc}", file=sys.stderr, flush=True)
     stats = getattr(solver, "last_stats", None) or {}
+
+    ab, learned_vs_plain = None, None
+    if not args.no_ab and w["kind"] == "two_galaxy":
+        ab = warm_start_ab(icr, w, dx, grav, args.ab_steps)
+        plain, learned = ab.get("multigrid_mp:previous"), ab.get("multigrid_mp:learned")
+        if plain and learned:
+            learned_vs_plain = {
+                "plain": "mixed-precision multigrid warm-started from the previous phi (the "
+                         "reference's warm start), same tolerance",
+                "learned": "the same solver warm-started by extrapolation plus the trained "
+                           "warm-start network",
+                "vcycles_per_step": [learned["vcycles_per_step"], plain["vcycles_per_step"]],
+                "ms_per_step": [learned["ms_per_step"], plain["ms_per_step"]],
+                "vcycles_ratio": learned["vcycles_per_step"] / plain["vcycles_per_step"],
+                "wall_ratio": learned["ms_per_step"] / plain["ms_per_step"],
+            }
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -251,6 +350,8 @@ def main() -> int:
         "kernels": dict(sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"])[:16]),
         "mem_estimate_gb": estimate_memory(icr.n, w["grid"]).total_gb,
         **acc,
+        "learned_vs_plain_mg": learned_vs_plain,
+        "warm_start_ab": ab,
     }
     print(json.dumps(out), flush=True)
     return 0
