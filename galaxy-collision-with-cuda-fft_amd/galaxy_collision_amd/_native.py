@@ -161,6 +161,8 @@ _SIGNATURES = {
     "pm_reduce_grid_pe_f64": [_vp, _vp, _i64, ctypes.POINTER(_d), _vp],
     "pm_radial_histogram_f64": [_vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _i32, _vp, _vp],
     "pm_copy_f32": [_vp, _vp, _i64, _vp],
+    "pm_project_scratch_bytes": [_i32, ctypes.POINTER(_i64)],
+    "pm_project_mass": [_vp, _vp, _vp, _i64, _f, _f, _f, _f, _i32, _vp, _vp, _vp],
     "pm_conv3d_packed_size": [_i32, _i32, _i32, ctypes.POINTER(_i64)],
     "pm_conv3d_pack_weights": [_vp, _i32, _i32, _i32, _vp, _vp],
     "pm_conv3d_scratch_size": [_i32, _i32, _i32, ctypes.POINTER(_i64)],

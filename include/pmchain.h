@@ -260,6 +260,20 @@ pm_status pm_radial_histogram_f64(const float* pos_x, const float* pos_y, const 
                                   float cz, float inv_binw, int32_t nbins, double* hist_dev,
                                   hipStream_t stream);
 
+/* ------------------------------------------------------------------------------------
+ * Device 2D projection (reference viz/project.py:26-83 `_project` / `scatter_density`).
+ * img[ia*bins + ib] = mass in bin (ia, ib) (float32 [bins][bins], axis a major), ia =
+ * floor((pos_a - lo_a)/binw_a), likewise ib; particles outside [0, bins) on either axis are
+ * dropped. Deterministic: masses are summed as 64-bit fixed-point integers (absolute error
+ * per particle <= 2^-35 of the largest mass). scratch: pm_project_scratch_bytes(bins) bytes of
+ * device memory, 8-byte aligned. n <= 2^28.
+ * ---------------------------------------------------------------------------------- */
+pm_status pm_project_scratch_bytes(int32_t bins, int64_t* out);
+pm_status pm_project_mass(const float* pos_a, const float* pos_b, const float* mass,
+                          int64_t n_particles, float lo_a, float lo_b, float binw_a,
+                          float binw_b, int32_t bins, float* img, void* scratch,
+                          hipStream_t stream);
+
 /* dst = src (count floats): a plain streaming copy of known size, used to calibrate
  * profiler byte counters (no reference counterpart). */
 pm_status pm_copy_f32(float* dst, const float* src, int64_t count, hipStream_t stream);
