@@ -14,15 +14,10 @@ namespace {
 // g = -grad(phi) (reference deposit.py:132-174). One thread per node, k fastest, so each
 // wavefront reads and writes 64 consecutive floats of a z-row.
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi,
-                                                   float* __restrict__ gx,
-                                                   float* __restrict__ gy,
-                                                   float* __restrict__ gz, int n,
-                                                   float inv_dx, float inv_2dx) {
-  const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int64_t total = n2 * n;
-  const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= total) return;
+// -grad(phi) at node idx = (i, j, k): central differences inside, one-sided on the faces.
+__device__ __forceinline__ void grad_at(const float* __restrict__ phi, int64_t idx, int n,
+                                        int64_t n2, float inv_dx, float inv_2dx, float& vx,
+                                        float& vy, float& vz) {
   const pm::Idx3 q3 = pm::idx3(idx, n);
   const int i = q3.i, j = q3.j, k = q3.k;
   float a, b, s;
@@ -34,7 +29,7 @@ __global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi
   } else {
     a = phi[idx + n2]; b = phi[idx - n2]; s = inv_2dx;
   }
-  const float vx = -(a - b) * s;
+  vx = -(a - b) * s;
   // y
   if (j == 0) {
     a = phi[idx + n]; b = phi[idx]; s = inv_dx;
@@ -43,7 +38,7 @@ __global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi
   } else {
     a = phi[idx + n]; b = phi[idx - n]; s = inv_2dx;
   }
-  const float vy = -(a - b) * s;
+  vy = -(a - b) * s;
   // z
   if (k == 0) {
     a = phi[idx + 1]; b = phi[idx]; s = inv_dx;
@@ -52,10 +47,35 @@ __global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi
   } else {
     a = phi[idx + 1]; b = phi[idx - 1]; s = inv_2dx;
   }
-  const float vz = -(a - b) * s;
+  vz = -(a - b) * s;
+}
+
+__global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi,
+                                                   float* __restrict__ gx,
+                                                   float* __restrict__ gy,
+                                                   float* __restrict__ gz, int n,
+                                                   float inv_dx, float inv_2dx) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (idx >= n2 * n) return;
+  float vx, vy, vz;
+  grad_at(phi, idx, n, n2, inv_dx, inv_2dx, vx, vy, vz);
   gx[idx] = vx;
   gy[idx] = vy;
   gz[idx] = vz;
+}
+
+// The same gradient written interleaved, g4[node] = (gx, gy, gz, 0): one 16-byte load per
+// corner in the gather (the fused step's private layout).
+__global__ void __launch_bounds__(256) grad4_kernel(const float* __restrict__ phi,
+                                                    float4* __restrict__ g4, int n,
+                                                    float inv_dx, float inv_2dx) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (idx >= n2 * n) return;
+  float vx, vy, vz;
+  grad_at(phi, idx, n, n2, inv_dx, inv_2dx, vx, vy, vz);
+  g4[idx] = make_float4(vx, vy, vz, 0.0f);
 }
 
 // CIC interpolation of the three node fields at one particle (reference deposit.py:100-129).
@@ -95,6 +115,61 @@ __device__ __forceinline__ void cic_gather(float x, float y, float z,
       }
     }
   }
+}
+
+// CIC interpolation from the interleaved field (same weights, order and roundings as
+// cic_gather, so bitwise equal).
+__device__ __forceinline__ void cic_gather4(float x, float y, float z,
+                                            const float4* __restrict__ g4, int n, float dx,
+                                            float& ax, float& ay, float& az) {
+  ax = 0.0f;
+  ay = 0.0f;
+  az = 0.0f;
+  int ix, iy, iz;
+  float fx, fy, fz;
+  if (!pm::cic_axis(x, dx, n, ix, fx) || !pm::cic_axis(y, dx, n, iy, fy) ||
+      !pm::cic_axis(z, dx, n, iz, fz))
+    return;
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+#pragma unroll
+  for (int cx = 0; cx < 2; ++cx) {
+    const int jx = ix + cx;
+    const float wx = cx ? fx : 1.0f - fx;
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy) {
+      const int jy = iy + cy;
+      const float wy = cy ? fy : 1.0f - fy;
+#pragma unroll
+      for (int cz = 0; cz < 2; ++cz) {
+        const int jz = iz + cz;
+        const float wz = cz ? fz : 1.0f - fz;
+        if (jx >= 0 && jx < n && jy >= 0 && jy < n && jz >= 0 && jz < n) {
+          const float w = wx * wy * wz;
+          const float4 g = g4[jx * n2 + static_cast<int64_t>(jy) * n + jz];
+          ax += w * g.x;
+          ay += w * g.y;
+          az += w * g.z;
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) gather_kick4_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float4* __restrict__ g4, float* __restrict__ ax, float* __restrict__ ay,
+    float* __restrict__ az, float* __restrict__ vx, float* __restrict__ vy,
+    float* __restrict__ vz, int64_t np, int n, float dx, float h) {
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  float a0, a1, a2;
+  cic_gather4(px[p], py[p], pz[p], g4, n, dx, a0, a1, a2);
+  ax[p] = a0;
+  ay[p] = a1;
+  az[p] = a2;
+  vx[p] = vx[p] + a0 * h;
+  vy[p] = vy[p] + a1 * h;
+  vz[p] = vz[p] + a2 * h;
 }
 
 __global__ void __launch_bounds__(256) gather_kernel(
@@ -256,6 +331,43 @@ pm_status pm_grad(const float* phi, float* gx, float* gy, float* gz, int32_t n, 
     grad_kernel<<<pm::blocks_for(total), pm::kBlock, 0, stream>>>(phi, gx, gy, gz, n, inv_dx,
                                                                    inv_2dx);
     PM_CHECK_LAUNCH("grad_kernel");
+  }
+  return PM_OK;
+}
+
+pm_status pm_grad4(const float* phi, float* g4, int32_t n, float dx, hipStream_t stream) {
+  PM_REQUIRE(phi && g4, "pm_grad4: null pointer");
+  PM_REQUIRE((reinterpret_cast<uintptr_t>(g4) & 15u) == 0, "pm_grad4: g4 must be 16-byte aligned");
+  PM_REQUIRE(n >= 2 && n <= 1024, "pm_grad4: grid_size must be in [2, 1024]");
+  PM_REQUIRE(dx > 0.0f, "pm_grad4: dx must be positive");
+  const float inv_dx = 1.0f / dx;
+  const float inv_2dx = 1.0f / (2.0f * dx);
+  const int64_t total = static_cast<int64_t>(n) * n * n;
+  {
+    PM_TIMED("grad4", 20.0 * total, stream);
+    grad4_kernel<<<pm::blocks_for(total), pm::kBlock, 0, stream>>>(
+        phi, reinterpret_cast<float4*>(g4), n, inv_dx, inv_2dx);
+    PM_CHECK_LAUNCH("grad4_kernel");
+  }
+  return PM_OK;
+}
+
+pm_status pm_gather_kick4(const float* px, const float* py, const float* pz, const float* g4,
+                          float* ax, float* ay, float* az, float* vx, float* vy, float* vz,
+                          int64_t np, int32_t n, float dx, float h, hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_gather_kick4: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(px && py && pz && g4 && ax && ay && az && vx && vy && vz,
+             "pm_gather_kick4: null pointer");
+  PM_REQUIRE((reinterpret_cast<uintptr_t>(g4) & 15u) == 0,
+             "pm_gather_kick4: g4 must be 16-byte aligned");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_gather_kick4: grid_size must be in [1, 1024]");
+  PM_REQUIRE(dx > 0.0f, "pm_gather_kick4: dx must be positive");
+  {
+    PM_TIMED("gather_kick", 48.0 * np + 12.0 * n * n * n, stream);
+    gather_kick4_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+        px, py, pz, reinterpret_cast<const float4*>(g4), ax, ay, az, vx, vy, vz, np, n, dx, h);
+    PM_CHECK_LAUNCH("gather_kick4_kernel");
   }
   return PM_OK;
 }

@@ -141,6 +141,9 @@ _SIGNATURES = {
     "pm_kick_drift": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _vp],
     "pm_gather_kick": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                        _f, _f, _vp],
+    "pm_grad4": [_vp, _vp, _i32, _f, _vp],
+    "pm_gather_kick4": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _f,
+                        _vp],
     "pm_direct_accel": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _vp],
     "pm_mg_create": [ctypes.POINTER(MGDesc), ctypes.POINTER(_vp)],
     "pm_mg_destroy": [_vp],

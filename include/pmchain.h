@@ -98,6 +98,15 @@ pm_status pm_kick_drift(float* pos_x, float* pos_y, float* pos_z, float* vel_x, 
                         const float* acc_z, int64_t n_particles, float h, float dt,
                         hipStream_t stream);
 
+/* The fused step's interleaved variants (no reference counterpart as separate calls):
+ * pm_grad4 writes g4[node] = (gx, gy, gz, 0) (float32, 16-byte aligned n^3*4 floats), and
+ * pm_gather_kick4 reads it; the results are bitwise those of pm_grad + pm_gather_kick. */
+pm_status pm_grad4(const float* phi, float* g4, int32_t grid_size, float dx, hipStream_t stream);
+pm_status pm_gather_kick4(const float* pos_x, const float* pos_y, const float* pos_z,
+                          const float* g4, float* acc_x, float* acc_y, float* acc_z,
+                          float* vel_x, float* vel_y, float* vel_z, int64_t n_particles,
+                          int32_t grid_size, float dx, float h, hipStream_t stream);
+
 /* Fused gather then kick(h): bitwise the same as pm_gather followed by pm_kick
  * (the closing half of reference integrator.py:100-131 `kdk_step`). */
 pm_status pm_gather_kick(const float* pos_x, const float* pos_y, const float* pos_z,

@@ -37,6 +37,16 @@ def test_grad_bitwise(device, n, dx):
     ref = O.grad(phi, dx)
     for a in range(3):
         assert_bitwise(_np(g[a]), ref[a])
+    # interleaved variant of the fused step: g4[node] = (gx, gy, gz, 0)
+    from galaxy_collision_amd import _native as nat
+
+    phi_d = _t(phi, device)
+    g4 = torch.full((n, n, n, 4), 7.0, device=device)
+    nat.call("pm_grad4", nat.ptr(phi_d), nat.ptr(g4), n, dx, nat.stream())
+    g4h = _np(g4)
+    for a in range(3):
+        assert_bitwise(g4h[..., a], ref[a])
+    assert not np.any(g4h[..., 3])
 
 
 def _particles(rng, npart, n, dx, spread=1.5):
@@ -75,6 +85,67 @@ def test_gather_and_fused_gather_kick_bitwise(device, npart, n, dx):
     for a in range(3):
         assert_bitwise(_np(acc2[a]), ref[a])
         assert_bitwise(v1[:, a], O.kick(v0[:, a], ref[a], h))
+    # the interleaved-field variant used by the fused step
+    g4 = torch.stack([*gt, torch.zeros_like(gt[0])], -1).contiguous()
+    acc3 = [torch.empty(npart, device=device) for _ in range(3)]
+    nat.call("pm_gather_kick4", *(nat.ptr(f.tensor) for f in (parts.pos_x, parts.pos_y, parts.pos_z)),
+             nat.ptr(g4), *(nat.ptr(a) for a in acc3),
+             *(nat.ptr(f.tensor) for f in (parts.vel_x, parts.vel_y, parts.vel_z)),
+             npart, n, dx, h, nat.stream())
+    v2 = parts.velocities()
+    for a in range(3):
+        assert_bitwise(_np(acc3[a]), ref[a])
+        assert_bitwise(v2[:, a], O.kick(v1[:, a], ref[a], h))
+
+
+def test_fused_chain_step_equals_unfused_sequence(device):
+    """PMChain.step (kick+drift fused, interleaved gradient, gather+kick fused) equals the
+    reference step kick -> drift -> deposit -> solve -> grad -> gather -> kick bit for bit
+    (reference integrator.py:100-131 with sim.py:318-325's accel_fn)."""
+    announce("fused chain step vs unfused")
+    from galaxy_collision_amd.chain import PMChain
+    from galaxy_collision_amd.data import ParticleState
+    from galaxy_collision_amd.deposit import (deposit_density, gather_acceleration,
+                                              potential_to_accel)
+    from galaxy_collision_amd.ic import plummer_ic
+    from galaxy_collision_amd.integrator import drift, kick
+    from galaxy_collision_amd.solver import make_solver
+
+    n, dx, dt = 32, 0.5, 0.02
+    icr = plummer_ic(5000, total_mass=1.0, scale_a=1.0, centre=(8.0, 8.0, 8.0), seed=5)
+
+    def state():
+        p = ParticleState(icr.n)
+        p.load(icr.pos, icr.vel, icr.mass)
+        return p
+
+    a = state()
+    chain = PMChain(a, n, dx, make_solver("multigrid", n, dx=dx, grav_constant=1.0))
+    chain.prime()
+    for _ in range(3):
+        chain.step(dt)
+    b = state()
+    solver = make_solver("multigrid", n, dx=dx, grav_constant=1.0)
+    rho, phi = (torch.zeros((n, n, n), device=device) for _ in range(2))
+    g = [torch.zeros((n, n, n), device=device) for _ in range(3)]
+    acc = [torch.zeros(icr.n, device=device) for _ in range(3)]
+    fields = lambda p: (p.pos_x.tensor, p.pos_y.tensor, p.pos_z.tensor)  # noqa: E731
+    vels = lambda p: (p.vel_x.tensor, p.vel_y.tensor, p.vel_z.tensor)  # noqa: E731
+
+    def accel(warm):
+        deposit_density(b, rho, dx)
+        solver.solve(rho, phi, warm_start=warm)
+        potential_to_accel(phi, *g, dx=dx)
+        gather_acceleration(b, *g, *acc, dx=dx)
+
+    accel(False)
+    for _ in range(3):
+        kick(*vels(b), *acc, icr.n, 0.5 * dt)
+        drift(*fields(b), *vels(b), icr.n, dt)
+        accel(True)
+        kick(*vels(b), *acc, icr.n, 0.5 * dt)
+    assert_bitwise(a.positions(), b.positions(), "positions")
+    assert_bitwise(a.velocities(), b.velocities(), "velocities")
 
 
 def test_kick_drift_and_fused_bitwise(device):
