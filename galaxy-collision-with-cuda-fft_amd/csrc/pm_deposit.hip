@@ -17,6 +17,7 @@
 
 #include "pm_cic.h"
 #include "pm_common.h"
+#include "pm_deposit_fixed.h"
 #include "pm_timing.h"
 
 namespace {
@@ -532,6 +533,9 @@ struct pm_deposit_plan {
   bool timing = false;
   hipEvent_t ev[5] = {};
   float last_ms[4] = {0, 0, 0, 0};
+  int mode = PM_DEPOSIT_SORTED;
+  unsigned long long* acc64 = nullptr;  // PM_DEPOSIT_FIXED: n^3 accumulators (kept zeroed)
+  unsigned int* maxbits = nullptr;
 };
 
 namespace {
@@ -539,7 +543,8 @@ namespace {
 void free_plan(pm_deposit_plan* p) {
   if (!p) return;
   void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->rec, p->hist, p->digit,
-                  p->sums, p->head, p->tail, p->tail_slot, p->map, p->big, p->big_count};
+                  p->sums, p->head, p->tail, p->tail_slot, p->map, p->big, p->big_count,
+                  p->acc64, p->maxbits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
@@ -610,6 +615,24 @@ pm_status pm_deposit_plan_destroy(pm_deposit_plan* plan) {
   return PM_OK;
 }
 
+pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode) {
+  pm::clear_error();
+  PM_REQUIRE(plan, "pm_deposit_plan_set_mode: null plan");
+  PM_REQUIRE(mode == PM_DEPOSIT_SORTED || mode == PM_DEPOSIT_FIXED,
+             "pm_deposit_plan_set_mode: unknown mode");
+  if (mode == PM_DEPOSIT_FIXED && !plan->acc64) {
+    const int64_t nodes = static_cast<int64_t>(plan->n) * plan->n * plan->n;
+    pm_status st;
+    if ((st = alloc(&plan->acc64, nodes, "hipMalloc (deposit accumulators)")) ||
+        (st = alloc(&plan->maxbits, 1 + pm::depfix::kMaxBlocks, "hipMalloc (deposit scale)")))
+      return st;
+    if ((st = pm::depfix::zero(plan->acc64, nodes, nullptr))) return st;
+    PM_TRY_HIP(hipStreamSynchronize(nullptr));
+  }
+  plan->mode = mode;
+  return PM_OK;
+}
+
 pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable) {
   PM_REQUIRE(plan, "pm_deposit_plan_set_timing: null plan");
   plan->timing = enable != 0;
@@ -640,6 +663,11 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     return PM_ERR_CAPACITY;
   }
   PM_REQUIRE(np == 0 || (px && py && pz && mass), "pm_deposit_cic: null particle array");
+  if (plan->mode == PM_DEPOSIT_FIXED) {
+    PM_REQUIRE(!perm_out, "pm_deposit_cic: the particle order needs PM_DEPOSIT_SORTED mode");
+    return pm::depfix::deposit(px, py, pz, mass, np, plan->n, dx, rho, keys_out, plan->acc64,
+                               plan->maxbits, stream);
+  }
   const int n = plan->n;
   const int64_t nodes = static_cast<int64_t>(n) * n * n;
   const int64_t m = static_cast<int64_t>(n) + 1;

@@ -18,6 +18,8 @@ PM_OK = 0
 PM_ERR_CAPACITY = 4
 PM_MG_PLAIN = 0
 PM_MG_MIXED = 1
+PM_DEPOSIT_SORTED = 0
+PM_DEPOSIT_FIXED = 1
 PM_WARM_PREVIOUS = 0
 PM_WARM_EXTRAPOLATE = 1
 PM_WARM_LEARNED = 2
@@ -133,6 +135,7 @@ _SIGNATURES = {
     "pm_deposit_plan_destroy": [_vp],
     "pm_deposit_cic": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "pm_deposit_plan_set_timing": [_vp, _i32],
+    "pm_deposit_plan_set_mode": [_vp, _i32],
     "pm_deposit_last_timing": [_vp, ctypes.POINTER(_f)],
     "pm_grad": [_vp, _vp, _vp, _vp, _i32, _f, _vp],
     "pm_gather": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _vp],

@@ -3,7 +3,7 @@
 ``PMChain.step(dt)`` runs, on torch's current stream,
 
     kick(dt/2) + drift(dt)     (pm_kick_drift, one pass over the particles)
-    deposit                    (sort-based CIC deposit)
+    deposit                    (sort-based deterministic CIC deposit)
     solve                      (the configured Poisson solver, warm-started)
     g = -grad(phi)             (pm_grad4, interleaved (gx, gy, gz, 0) per node)
     gather + kick(dt/2)        (pm_gather_kick4, one pass over the particles)
@@ -25,7 +25,13 @@ from galaxy_collision_amd.deposit import deposit_plan
 
 
 class PMChain:
-    def __init__(self, parts, grid_size: int, dx: float, solver, grid: GridState | None = None):
+    """deposit_mode: "sorted" (default: cell-key sort + ordered segmented reduction) or
+    "fixed" (64-bit fixed-point integer atomics, order independent); both deterministic.
+    At 256^3 / 10M particles "fixed" is slower (its 64-bit device-scope atomics), see the
+    deposit mode notes in pmchain.h."""
+
+    def __init__(self, parts, grid_size: int, dx: float, solver, grid: GridState | None = None,
+                 deposit_mode: str = "sorted"):
         self.parts = parts
         self.n = int(grid_size)
         self.dx = float(dx)
@@ -35,6 +41,7 @@ class PMChain:
         self.acc = [torch.zeros(parts.n, device=dev) for _ in range(3)]
         self.g4 = torch.empty((self.n, self.n, self.n, 4), device=dev)
         self.plan = deposit_plan(self.n, parts.n)
+        self.deposit_mode = deposit_mode
         self.warm = False
         self.last_cycles: list[int] = []
 
@@ -43,6 +50,7 @@ class PMChain:
 
     def deposit(self) -> None:
         p = self.parts
+        self.plan.set_mode(self.deposit_mode)
         self.plan.deposit(p.pos_x, p.pos_y, p.pos_z, p.mass, p.n, self.dx, self.grid.rho)
 
     def solve(self) -> None:

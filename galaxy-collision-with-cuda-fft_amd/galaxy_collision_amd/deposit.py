@@ -1,7 +1,10 @@
 """The particle<->grid stages of the PM force chain on libpmchain (reference deposit.py).
 
-- ``deposit_density(parts, rho, dx)``: CIC deposit of mass density (M/dx^3), deterministic
-  (cell-key sort + segmented reduction, no float atomics). Reference deposit.py:177-182.
+- ``deposit_density(parts, rho, dx, mode="sorted")``: CIC deposit of mass density
+  (M/dx^3), deterministic either way: "sorted" (cell-key sort + segmented reduction in a
+  defined float32 order, also yields the keys and particle order) or "fixed" (64-bit
+  fixed-point integer sums, order independent, the fused step's default). Reference
+  deposit.py:177-182.
 - ``potential_to_accel(phi, ax, ay, az, dx)``: g = -grad(phi), central inside, one-sided on
   the faces. Reference deposit.py:204-207.
 - ``gather_acceleration(parts, ax, ay, az, acc_x, acc_y, acc_z, dx)``: CIC interpolation of
@@ -22,6 +25,9 @@ from galaxy_collision_amd import _native as nat
 from galaxy_collision_amd.data import as_tensor
 
 
+_MODES = {"sorted": nat.PM_DEPOSIT_SORTED, "fixed": nat.PM_DEPOSIT_FIXED}
+
+
 class DepositPlan:
     """Sort and reduction scratch for deposits on one grid size and device."""
 
@@ -33,6 +39,14 @@ class DepositPlan:
         h = ctypes.c_void_p()
         nat.call("pm_deposit_plan_create", self.grid_size, self.capacity, ctypes.byref(h))
         self.handle = h
+        self.mode = "sorted"
+
+    def set_mode(self, mode: str) -> None:
+        if mode not in _MODES:
+            raise ValueError(f"deposit mode must be one of {sorted(_MODES)}, got {mode!r}")
+        if mode != self.mode:
+            nat.call("pm_deposit_plan_set_mode", self.handle, _MODES[mode])
+            self.mode = mode
 
     def deposit(self, pos_x, pos_y, pos_z, mass, n: int, dx: float, rho,
                 keys_out: torch.Tensor | None = None, perm_out: torch.Tensor | None = None):
@@ -88,10 +102,11 @@ def _grid_n(field) -> int:
     return shape[0]
 
 
-def deposit_density(parts, rho, dx: float = 1.0) -> None:
+def deposit_density(parts, rho, dx: float = 1.0, mode: str = "sorted") -> None:
     """Deposit particle masses onto ``rho`` (mass per dx^3) by CIC. Overwrites ``rho``."""
     n = _grid_n(rho)
     plan = deposit_plan(n, parts.n)
+    plan.set_mode(mode)
     plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, parts.n, dx, rho)
 
 
@@ -102,6 +117,7 @@ def deposit_keys(parts, rho, dx: float = 1.0):
     keys = torch.empty(parts.n, dtype=torch.int32, device=t.device)
     perm = torch.empty(parts.n, dtype=torch.int32, device=t.device)
     plan = deposit_plan(n, parts.n)
+    plan.set_mode("sorted")
     plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, parts.n, dx, rho, keys, perm)
     return keys, perm
 

@@ -51,6 +51,19 @@ pm_status pm_deposit_plan_create(int32_t grid_size, int64_t max_particles,
                                  pm_deposit_plan** out);
 pm_status pm_deposit_plan_destroy(pm_deposit_plan* plan);
 
+/* Deposit modes. PM_DEPOSIT_SORTED (default): cell-key sort + ordered segmented
+ * reduction; also produces the keys and the stable permutation. PM_DEPOSIT_FIXED: every
+ * corner contribution (same float32 arithmetic) is rounded to a 64-bit fixed-point integer
+ * (scale 2^s, s = 61 - a - b for the largest m' = mass/dx^3 in [2^a, 2^(a+1)) and N < 2^b
+ * particles) and summed per node with integer atomics after a per-wave aggregation of
+ * particles that share a cell: deterministic in any order, rho = float(double(sum)*2^-s);
+ * keys_out is available, perm_out is not. Non-finite contributions are dropped. The first
+ * switch to PM_DEPOSIT_FIXED allocates n^3 64-bit accumulators and synchronizes. Measured at
+ * 256^3 with 10M cell-ordered particles: 5.2 ms (the 64-bit device-scope atomics, ~0.4 G/s
+ * across the XCDs) vs 0.95 ms for PM_DEPOSIT_SORTED, which stays the default. */
+enum { PM_DEPOSIT_SORTED = 0, PM_DEPOSIT_FIXED = 1 };
+pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode);
+
 /* keys_out (optional, n_particles uint32): extended cell key per particle in input order,
  *   key = ((ix+1)*(n+1) + (iy+1))*(n+1) + (iz+1) with i* = floor(pos/dx), or (n+1)^3 when
  *   no corner of the particle's cell lies in the box.

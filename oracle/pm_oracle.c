@@ -9,6 +9,7 @@
  *    written from its definition (stable counting sort, sequential pieces inside aligned
  *    16-position chunks, recursive pairwise tree over a cell's pieces, fixed corner order
  *    per node). Per-particle arithmetic follows reference deposit.py:58-82.
+ *    oracle_deposit_fixed: the fixed-point mode (integer sums of rounded corner terms).
  *
  * 2. cpu_*: an OpenMP port of the reference kernel sequence used as the CPU baseline:
  *    float-atomic CIC scatter (deposit.py:47-82), central-difference gradient
@@ -146,6 +147,55 @@ int oracle_deposit_cic(const float* px, const float* py, const float* pz, const 
     rho[idx] = acc;
   }
   free(key); free(rec); free(perm); free(count); free(map); free(sums); free(pieces);
+  return 0;
+}
+
+/* Fixed-point deposit: the same float32 corner terms, each rounded to a 64-bit integer at
+ * scale 2^s (s = 61 - a - b, largest finite positive m' in [2^a, 2^(a+1)), np < 2^b), summed
+ * per node as integers (any order gives the same sum), rho = (float)((double)sum * 2^-s).
+ * Non-finite or non-positive terms are dropped. keys (optional) as oracle_deposit_cic. */
+int oracle_deposit_fixed(const float* px, const float* py, const float* pz, const float* mass,
+                         int64_t np, int n, float dx, float* rho, uint32_t* keys_out) {
+  const int64_t m = (int64_t)n + 1;
+  const uint32_t sentinel = (uint32_t)(m * m * m);
+  const float inv_vol = 1.0f / ((dx * dx) * dx);
+  const int64_t nodes = (int64_t)n * n * n;
+  uint64_t* acc = (uint64_t*)calloc((size_t)(nodes > 0 ? nodes : 1), sizeof(uint64_t));
+  if (!acc) return -1;
+  float mmax = 0.0f;
+  for (int64_t p = 0; p < np; ++p) {
+    const float v = mass[p] * inv_vol;
+    if (v > 0.0f && isfinite(v) && v > mmax) mmax = v;
+  }
+  int b = 0;
+  for (int64_t v = np; v > 0; v >>= 1) ++b;
+  int s = 0;
+  if (mmax > 0.0f) {
+    int e;
+    (void)frexp((double)mmax, &e);
+    s = 61 - (e - 1) - b;
+  }
+  const int64_t n2 = (int64_t)n * n;
+  for (int64_t p = 0; p < np; ++p) {
+    int ix = 0, iy = 0, iz = 0;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const int in = cic_axis(px[p], dx, n, &ix, &fx) && cic_axis(py[p], dx, n, &iy, &fy) &&
+                   cic_axis(pz[p], dx, n, &iz, &fz);
+    if (keys_out)
+      keys_out[p] = in ? (uint32_t)(((int64_t)(ix + 1) * m + (iy + 1)) * m + (iz + 1)) : sentinel;
+    if (!in) continue;
+    const float r[4] = {fx, fy, fz, mass[p] * inv_vol};
+    float t[8];
+    corner_terms(r, t);
+    for (int c = 0; c < 8; ++c) {
+      const int jx = ix + (c >> 2), jy = iy + ((c >> 1) & 1), jz = iz + (c & 1);
+      if (jx < 0 || jx >= n || jy < 0 || jy >= n || jz < 0 || jz >= n) continue;
+      if (!(t[c] > 0.0f && isfinite(t[c]))) continue;
+      acc[jx * n2 + (int64_t)jy * n + jz] += (uint64_t)llrint(ldexp((double)t[c], s));
+    }
+  }
+  for (int64_t i = 0; i < nodes; ++i) rho[i] = (float)ldexp((double)acc[i], -s);
+  free(acc);
   return 0;
 }
 

@@ -49,6 +49,9 @@ def lib():
                                          ctypes.POINTER(ctypes.c_uint32),
                                          ctypes.POINTER(ctypes.c_int32)]
         L.oracle_deposit_cic.restype = ctypes.c_int
+        L.oracle_deposit_fixed.argtypes = [fp, fp, fp, fp, i64, ctypes.c_int, ctypes.c_float, fp,
+                                           ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_deposit_fixed.restype = ctypes.c_int
         L.cpu_deposit_atomic.argtypes = [fp, fp, fp, fp, i64, ctypes.c_int, ctypes.c_float, fp]
         L.cpu_grad.argtypes = [fp, fp, fp, fp, ctypes.c_int, ctypes.c_float]
         L.cpu_gather.argtypes = [fp, fp, fp, fp, fp, fp, fp, fp, fp, i64, ctypes.c_int,
@@ -118,6 +121,22 @@ def deposit_cic(pos: np.ndarray, mass: np.ndarray, n: int, dx: float):
     if rc != 0:
         raise MemoryError("oracle_deposit_cic allocation failed")
     return rho, keys[:npart], perm[:npart]
+
+
+def deposit_cic_fixed(pos: np.ndarray, mass: np.ndarray, n: int, dx: float):
+    """The fixed-point deposit (C). Returns (rho, keys)."""
+    pos = np.ascontiguousarray(pos, dtype=F32)
+    mass = np.ascontiguousarray(mass, dtype=F32)
+    x, y, z = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    npart = mass.shape[0]
+    rho = np.zeros((n, n, n), dtype=F32)
+    keys = np.zeros(max(npart, 1), dtype=np.uint32)
+    rc = lib().oracle_deposit_fixed(
+        _fp(x), _fp(y), _fp(z), _fp(mass), npart, n, dx, _fp(rho),
+        keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    if rc != 0:
+        raise MemoryError("oracle_deposit_fixed allocation failed")
+    return rho, keys[:npart]
 
 
 def deposit_cic_f64(pos: np.ndarray, mass: np.ndarray, n: int, dx: float) -> np.ndarray:

@@ -117,3 +117,76 @@ def test_deterministic_repeat(device):
     a, _, _ = _run(device, pos, mass, 64, 1.0)
     b, _, _ = _run(device, pos, mass, 64, 1.0)
     assert_bitwise(a, b)
+
+
+# ---------------------------------------------------------------- fixed-point mode
+
+def _run_fixed(device, pos, mass, n, dx):
+    from galaxy_collision_amd.data import ParticleState
+    from galaxy_collision_amd.deposit import deposit_plan
+
+    parts = ParticleState(max(len(mass), 1))
+    if len(mass):
+        parts.load(pos, None, mass)
+    parts.n = len(mass)
+    rho = torch.full((n, n, n), 7.0, device=device)
+    keys = torch.empty(max(parts.n, 1), dtype=torch.int32, device=device)
+    plan = deposit_plan(n, parts.n)
+    plan.set_mode("fixed")
+    plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, parts.n, dx, rho, keys)
+    return rho.cpu().numpy(), keys.cpu().numpy().view(np.uint32)[: parts.n]
+
+
+def _check_fixed(device, pos, mass, n, dx):
+    rho, keys = _run_fixed(device, pos, mass, n, dx)
+    ref_rho, ref_keys = O.deposit_cic_fixed(pos, mass, n, dx)
+    assert_bitwise(keys, ref_keys)
+    assert_bitwise(rho, ref_rho)
+    return rho
+
+
+def test_fixed_closed_forms_and_edges(device):
+    announce("fixed deposit closed forms / edges")
+    rho, _ = _run_fixed(device, np.array([[8.5, 8.5, 8.5]], F32), np.array([8.0], F32), 16, 1.0)
+    assert np.all(rho[8:10, 8:10, 8:10] == 1.0) and rho.sum() == 8.0
+    rho, _ = _run_fixed(device, np.array([[-3.0, 8, 8], [40.0, 8, 8], [15.5, 8, 8]], F32),
+                        np.ones(3, F32), 16, 1.0)
+    assert rho[15, 8, 8] == 0.5 and rho.sum() == 0.5
+    rho, _ = _run_fixed(device, np.zeros((0, 3), F32), np.zeros(0, F32), 8, 1.0)
+    assert not rho.any()
+    pos = np.array([[-5.0, 1, 1], [100.0, 2, 2], [np.nan, 3, 3], [3, np.inf, 3], [-1.0, 0, 0],
+                    [7.999, 7.999, 7.999], [8.0, 1, 1], [2.5, 2.5, 2.5]], F32)
+    _check_fixed(device, pos, np.ones(8, F32), 8, 1.0)
+
+
+@pytest.mark.parametrize("npart,n,dx,dense,sort", [
+    (1, 4, 1.0, 0, False), (4097, 8, 1.0, 0, False), (50_000, 32, 0.3125, 20_000, False),
+    (300_000, 64, 1.0, 150_000, True), (1_000_003, 128, 0.5, 0, True),
+    (1_000_003, 128, 0.5, 0, False)])
+def test_fixed_random_bitwise(device, npart, n, dx, dense, sort):
+    """Bitwise equal to the C oracle in random order and in cell order (long wave runs),
+    with a dense cell (one atomic target shared by many waves)."""
+    announce(f"fixed deposit npart={npart} n={n} dense={dense} sorted={sort}")
+    rng = np.random.default_rng(npart + 1)
+    pos = rng.normal(n * dx / 2, n * dx / 5, size=(npart, 3)).astype(F32)
+    if dense:
+        pos[:dense] = rng.uniform(0.3 * n * dx, 0.3 * n * dx + 0.9 * dx, size=(dense, 3))
+    pos = pos[rng.permutation(npart)]
+    if sort:
+        pos = pos[np.argsort(O.cell_keys(pos, n, dx), kind="stable")]
+    mass = rng.uniform(0.5, 2.0, npart).astype(F32)
+    rho = _check_fixed(device, pos, mass, n, dx)
+    # the sorted mode gives the same density to float32 rounding of a different order
+    ref_sorted, _, _ = O.deposit_cic(pos, mass, n, dx)
+    assert np.abs(rho - ref_sorted).max() <= 4e-6 * np.abs(ref_sorted).max()
+
+
+def test_fixed_order_independent(device):
+    announce("fixed deposit order independence")
+    rng = np.random.default_rng(5)
+    pos = rng.normal(32, 4, size=(400_000, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2.0, 400_000).astype(F32)
+    a, _ = _run_fixed(device, pos, mass, 64, 1.0)
+    order = rng.permutation(400_000)
+    b, _ = _run_fixed(device, pos[order], mass[order], 64, 1.0)
+    assert_bitwise(a, b)

@@ -98,11 +98,12 @@ def test_gather_and_fused_gather_kick_bitwise(device, npart, n, dx):
         assert_bitwise(v2[:, a], O.kick(v1[:, a], ref[a], h))
 
 
-def test_fused_chain_step_equals_unfused_sequence(device):
+@pytest.mark.parametrize("mode", ["fixed", "sorted"])
+def test_fused_chain_step_equals_unfused_sequence(device, mode):
     """PMChain.step (kick+drift fused, interleaved gradient, gather+kick fused) equals the
     reference step kick -> drift -> deposit -> solve -> grad -> gather -> kick bit for bit
     (reference integrator.py:100-131 with sim.py:318-325's accel_fn)."""
-    announce("fused chain step vs unfused")
+    announce(f"fused chain step vs unfused ({mode} deposit)")
     from galaxy_collision_amd.chain import PMChain
     from galaxy_collision_amd.data import ParticleState
     from galaxy_collision_amd.deposit import (deposit_density, gather_acceleration,
@@ -120,7 +121,8 @@ def test_fused_chain_step_equals_unfused_sequence(device):
         return p
 
     a = state()
-    chain = PMChain(a, n, dx, make_solver("multigrid", n, dx=dx, grav_constant=1.0))
+    chain = PMChain(a, n, dx, make_solver("multigrid", n, dx=dx, grav_constant=1.0),
+                    deposit_mode=mode)
     chain.prime()
     for _ in range(3):
         chain.step(dt)
@@ -133,7 +135,7 @@ def test_fused_chain_step_equals_unfused_sequence(device):
     vels = lambda p: (p.vel_x.tensor, p.vel_y.tensor, p.vel_z.tensor)  # noqa: E731
 
     def accel(warm):
-        deposit_density(b, rho, dx)
+        deposit_density(b, rho, dx, mode=mode)
         solver.solve(rho, phi, warm_start=warm)
         potential_to_accel(phi, *g, dx=dx)
         gather_acceleration(b, *g, *acc, dx=dx)

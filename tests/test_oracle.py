@@ -90,6 +90,66 @@ def test_deposit_empty():
     assert rho.shape == (8, 8, 8) and not rho.any() and keys.size == 0 and perm.size == 0
 
 
+def test_fixed_deposit_closed_forms():
+    """Reference tests/test_deposit.py closed forms for the fixed-point mode: on-node mass,
+    midpoint eighths, open-boundary drop (exact: these terms are exact in fixed point)."""
+    rho, _ = O.deposit_cic_fixed(np.array([[8.0, 8.0, 8.0]]), np.array([3.0]), 16, 1.0)
+    assert rho[8, 8, 8] == 3.0 and rho.sum() == 3.0
+    rho, _ = O.deposit_cic_fixed(np.array([[8.5, 8.5, 8.5]]), np.array([8.0]), 16, 1.0)
+    assert np.all(rho[8:10, 8:10, 8:10] == 1.0) and rho.sum() == 8.0
+    rho, keys = O.deposit_cic_fixed(np.array([[-3.0, 8, 8], [40.0, 8, 8], [15.5, 8, 8]]),
+                                    np.ones(3), 16, 1.0)
+    assert rho[15, 8, 8] == 0.5 and rho[0].sum() == 0.0 and rho.sum() == 0.5
+    assert keys[0] == keys[1] == 17 ** 3
+
+
+def test_fixed_deposit_is_the_rounded_exact_sum():
+    """Fixed point sums the float32 corner terms essentially exactly: the result is the
+    float64 scatter of the same terms rounded to float32 (the quantum is ~2^-37 of the
+    largest term, far below half a float32 ulp of any non-tiny node), it is independent
+    of the particle order, and the keys equal the sorted mode's."""
+    rng = np.random.default_rng(11)
+    n = 24
+    pos = rng.normal(n / 2, n / 6, size=(30000, 3)).astype(F32)
+    pos[:5000] = rng.uniform(11.0, 11.999, size=(5000, 3))
+    mass = rng.uniform(0.5, 2.0, 30000).astype(F32)
+    rho, keys = O.deposit_cic_fixed(pos, mass, n, 0.75)
+    # float64 sums of the exact float32 corner terms t = ((m'*wx)*wy)*wz (numpy float32
+    # arithmetic, one rounding per operation)
+    dx = F32(0.75)
+    g = pos / dx
+    ok = np.all((g >= -1) & (g < n), axis=1)
+    i = np.floor(g).astype(np.int64)
+    f = (g - i.astype(F32)).astype(F32)
+    mp = (mass * (F32(1.0) / ((dx * dx) * dx))).astype(F32)
+    ref = np.zeros(n ** 3)
+    for c in range(8):
+        bits = (c >> 2, (c >> 1) & 1, c & 1)
+        w = [f[:, a] if b else (F32(1.0) - f[:, a]) for a, b in enumerate(bits)]
+        t = ((mp * w[0]) * w[1]) * w[2]
+        j = i + np.array(bits)
+        inb = ok & np.all((j >= 0) & (j < n), axis=1)
+        np.add.at(ref, ((j[inb, 0] * n + j[inb, 1]) * n + j[inb, 2]), t[inb].astype(np.float64))
+    ref = ref.reshape(n, n, n)
+    big = ref > 1e-3 * ref.max()
+    ulp = np.spacing(ref[big].astype(F32)).astype(np.float64)
+    assert np.all(np.abs(rho[big].astype(np.float64) - ref[big]) <= ulp)
+    assert np.abs(rho - ref).max() <= 1e-7 * np.abs(ref).max()
+    order = rng.permutation(30000)
+    rho2, _ = O.deposit_cic_fixed(pos[order], mass[order], n, 0.75)
+    assert np.array_equal(rho, rho2)
+    _, keys_sorted, _ = O.deposit_cic(pos, mass, n, 0.75)
+    assert np.array_equal(keys, keys_sorted)
+
+
+def test_fixed_deposit_empty_and_dropped_nonfinite():
+    rho, keys = O.deposit_cic_fixed(np.zeros((0, 3)), np.zeros(0), 8, 1.0)
+    assert not rho.any() and keys.size == 0
+    pos = np.array([[2.0, 2.0, 2.0], [3.0, 3.0, 3.0]])
+    rho, _ = O.deposit_cic_fixed(pos, np.array([np.inf, 2.0]), 8, 1.0)
+    assert rho[3, 3, 3] == 2.0 and rho.sum() == 2.0
+
+
 # ---------------------------------------------------------------- grad / gather / kick
 
 def test_grad_linear_phi_and_faces():
