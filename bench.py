@@ -5,6 +5,7 @@ Workloads (seeded synthetic inputs, built on the host, resident in HBM before ti
                 mixed-precision multigrid to an estimated phi error of 1e-6, warm-started by
                 linear extrapolation of phi in time (default)
   collision2m   the same recipe with 2 x 1,048,576 particles on a 128^3 grid, box 40
+  collision100m the full-scale case: 2 x 50M particles (2:1 disk mass ratio), 256^3
   plummer       131,072-particle Plummer sphere, 64^3 grid over a box of 20a, dt 0.01
 
 A step is one fused KDK step (kick+drift, deposit, Poisson solve, -grad, gather+kick).
@@ -38,6 +39,8 @@ WORKLOADS = {
                          solver="multigrid_mp", warm="extrapolate"),
     "collision2m": dict(kind="two_galaxy", n_per=1_048_576, grid=128, box=40.0, dt=0.01,
                         solver="multigrid"),
+    "collision100m": dict(kind="two_galaxy", n_per=50_000_000, grid=256, box=40.0, dt=0.01,
+                          solver="multigrid_mp", warm="extrapolate", masses=(1.0, 0.5)),
     "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
 }
 
@@ -52,7 +55,8 @@ def build_ic(w: dict, seed: int):
         icr = ic.plummer_ic(w["n"], total_mass=1.0, scale_a=1.0, centre=(c, c, c), seed=seed)
     else:
         c = (n - 1) * dx / 2.0
-        icr = ic.two_galaxy_ic(w["n_per"], box_centre=(c, c, c), separation=20.0, seed=seed)
+        icr = ic.two_galaxy_ic(w["n_per"], box_centre=(c, c, c), separation=20.0, seed=seed,
+                               masses=w.get("masses", (1.0, 1.0)))
     return ic.sort_by_cell(icr, n, dx), dx
 
 
