@@ -8,7 +8,9 @@ Workloads (seeded synthetic inputs, built on the host, resident in HBM before ti
   collision100m the full-scale case: 2 x 50M particles (2:1 disk mass ratio), 256^3
   plummer       131,072-particle Plummer sphere, 64^3 grid over a box of 20a, dt 0.01
 
-A step is one fused KDK step (kick+drift, deposit, Poisson solve, -grad, gather+kick).
+A step is one fused KDK step (kick+drift, deposit, Poisson solve, -grad, gather+kick); the
+timed steps run back to back, so one step's closing kick shares a particle pass with the next
+step's opening kick and drift (bitwise the same results as separate steps).
 Pass 1 times --steps steps with no instrumentation (the reported value). Pass 2 repeats
 them with per-kernel HIP events on the launch stream, giving per-kernel times and the
 roofline figures of the dominant kernel. Pass 3 (collisions) restarts from the initial
@@ -237,16 +239,14 @@ def main() -> int:
     solver = make_solver(solver_name, w["grid"], **kw)
     chain = PMChain(parts, w["grid"], dx, solver)
     chain.prime()
-    for _ in range(args.warmup):
-        chain.step(w["dt"])
+    chain.run(args.warmup, w["dt"])
     torch.cuda.synchronize()
 
     # pass 1: throughput
     chain.last_cycles.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        chain.step(w["dt"])
+    chain.run(args.steps, w["dt"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     cycles = list(chain.last_cycles)
@@ -262,8 +262,7 @@ def main() -> int:
         nat.timing_enable(True)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        for _ in range(args.steps):
-            chain.step(w["dt"])
+        chain.run(args.steps, w["dt"])
         torch.cuda.synchronize()
         pass2_ms = (time.perf_counter() - t1) / args.steps * 1e3
         nat.timing_enable(False)

@@ -26,7 +26,7 @@ constexpr int kChunk = 16;           // positions per piece chunk (one thread ea
 constexpr int kSortWaves = 4;        // waves per radix workgroup
 constexpr int kSortItems = 16;       // keys per lane
 constexpr int kTile = kSortWaves * 64 * kSortItems;  // 4096 keys per radix tile
-constexpr int kMaxDigitBits = 8;
+constexpr int kMaxDigitBits = 9;     // 25-bit keys of a 256^3 grid sort in 3 passes
 constexpr int kSmallPieces = 8;      // open cells with <= this many pieces finish in one thread
 constexpr int kBigBatch = 1024;      // pieces per batch for large cells
 
@@ -144,14 +144,18 @@ __global__ void __launch_bounds__(256) radix_scan_kernel(uint32_t* __restrict__ 
   if (threadIdx.x == 0) digit_tot[blockIdx.x] = carry;
 }
 
-// digit_tot -> exclusive prefix (digit_start), one workgroup.
+// digit_tot -> exclusive prefix (digit_start), one workgroup; thread t owns digits 2t, 2t+1.
 __global__ void __launch_bounds__(256) radix_digit_start_kernel(uint32_t* __restrict__ digit_tot,
                                                                 int buckets) {
+  static_assert(2 * 256 >= (1 << kMaxDigitBits), "two digits per thread");
   __shared__ uint32_t s_wave[4];
-  const uint32_t v = threadIdx.x < static_cast<unsigned>(buckets) ? digit_tot[threadIdx.x] : 0u;
+  const int d0 = 2 * threadIdx.x;
+  const uint32_t v0 = d0 < buckets ? digit_tot[d0] : 0u;
+  const uint32_t v1 = d0 + 1 < buckets ? digit_tot[d0 + 1] : 0u;
   uint32_t total;
-  const uint32_t ex = block_exclusive_scan(v, total, s_wave);
-  if (threadIdx.x < static_cast<unsigned>(buckets)) digit_tot[threadIdx.x] = ex;
+  const uint32_t ex = block_exclusive_scan(v0 + v1, total, s_wave);
+  if (d0 < buckets) digit_tot[d0] = ex;
+  if (d0 + 1 < buckets) digit_tot[d0 + 1] = ex + v0;
 }
 
 // Stable scatter of one tile. Lane l of wave w, round r holds tile position
@@ -319,9 +323,19 @@ __global__ void __launch_bounds__(kPieceThreads) piece_kernel(
 }
 
 // First sorted position after `from` whose key differs from `key` (keys are sorted).
+// Galloping: probe from+1, +2, +4, ... until a different key (or the end), then bisect the
+// last gap, so a cell of L particles costs about 2 log2(L) mostly cached loads.
 __device__ __forceinline__ int64_t cell_end(const uint32_t* __restrict__ skey, int64_t from,
                                             int64_t np, uint32_t key) {
-  int64_t lo = from, hi = np;  // invariant: skey[lo] == key, answer in (lo, hi]
+  int64_t lo = from, step = 1, hi = np;  // invariant: skey[lo] == key, answer in (lo, hi]
+  while (lo + step < np) {
+    if (skey[lo + step] != key) {
+      hi = lo + step;
+      break;
+    }
+    lo += step;
+    step <<= 1;
+  }
   while (hi - lo > 1) {
     const int64_t mid = lo + (hi - lo) / 2;
     if (skey[mid] == key) lo = mid; else hi = mid;
@@ -374,13 +388,7 @@ __global__ void __launch_bounds__(256) join_kernel(
   map[key] = static_cast<int32_t>(slot);
 }
 
-// One workgroup (4 waves) per cell with more than kSmallPieces pieces. Batches of
-// kBigBatch = 1024 pieces, aligned to the cell's first piece: each lane reduces 4
-// consecutive pieces, the wave continues the aligned pairwise tree with shuffles (lane l
-// absorbs lane l+s when that lane holds pieces), and the 4 wave results are combined the
-// same way. Batch results are merged by a binary-counter cascade (push; merge while the two
-// top partials cover equal piece counts) and a final right-nested combine, which is the
-// same aligned tree one level up.
+// Piece q of a cell whose first piece closes chunk `chunk` (q = 0: that tail piece).
 __device__ __forceinline__ void load_piece(const float* __restrict__ head,
                                            const float* __restrict__ tail, int64_t chunk,
                                            int64_t q, float v[8]) {
@@ -391,6 +399,71 @@ __device__ __forceinline__ void load_piece(const float* __restrict__ head,
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 
+// Cells of kSmallPieces+1 .. kWavePieces pieces: one wave each, the first stage of
+// big_cell_kernel's batch (4 pieces per lane, aligned shuffle tree), whose result for a
+// batch of at most 256 pieces is the wave-0 value, so the sums are bitwise the same.
+constexpr int kWavePieces = 256;
+
+__global__ void __launch_bounds__(256) mid_cell_kernel(
+    const uint32_t* __restrict__ skey, const BigCell* __restrict__ big,
+    const uint32_t* __restrict__ big_count, const float* __restrict__ head,
+    const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
+    int32_t* __restrict__ map) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t count = *big_count;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t bcell = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); bcell < count;
+       bcell += waves) {
+    const BigCell cell = big[bcell];
+    if (cell.pieces > kWavePieces) continue;  // wave-uniform: big_cell_kernel's
+    const int64_t m = cell.pieces;
+    const int64_t q0 = 4 * lane;
+    float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (q0 < m) {
+      float w[8];
+      load_piece(head, tail, cell.chunk, q0, v);
+      if (q0 + 1 < m) {
+        load_piece(head, tail, cell.chunk, q0 + 1, w);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
+      }
+      if (q0 + 2 < m) {
+        float u[8];
+        load_piece(head, tail, cell.chunk, q0 + 2, w);
+        if (q0 + 3 < m) {
+          load_piece(head, tail, cell.chunk, q0 + 3, u);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) w[c] = w[c] + u[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
+      }
+    }
+#pragma unroll
+    for (int sft = 1; sft < 64; sft <<= 1) {
+      const bool take = (lane % (2 * sft)) == 0 && (q0 + 4 * sft) < m;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float o = __shfl_down(v[c], sft, 64);
+        if (take) v[c] = v[c] + o;
+      }
+    }
+    // lane 0 holds the 8 corner sums
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) sums[c * cap + cell.slot] = v[c];
+      map[skey[cell.slot]] = static_cast<int32_t>(cell.slot);
+    }
+  }
+}
+
+// One workgroup (4 waves) per cell with more than kWavePieces pieces. Batches of
+// kBigBatch = 1024 pieces, aligned to the cell's first piece: each lane reduces 4
+// consecutive pieces, the wave continues the aligned pairwise tree with shuffles (lane l
+// absorbs lane l+s when that lane holds pieces), and the 4 wave results are combined the
+// same way. Batch results are merged by a binary-counter cascade (push; merge while the two
+// top partials cover equal piece counts) and a final right-nested combine, which is the
+// same aligned tree one level up.
 __global__ void __launch_bounds__(256) big_cell_kernel(
     const uint32_t* __restrict__ skey, const BigCell* __restrict__ big,
     const uint32_t* __restrict__ big_count, const float* __restrict__ head,
@@ -406,6 +479,7 @@ __global__ void __launch_bounds__(256) big_cell_kernel(
   const uint32_t count = *big_count;
   for (uint32_t bcell = blockIdx.x; bcell < count; bcell += gridDim.x) {
     const BigCell cell = big[bcell];
+    if (cell.pieces <= kWavePieces) continue;  // mid_cell_kernel's
     int depth = 0;
     for (int64_t base = 0; base < cell.pieces; base += kBigBatch) {
       const int64_t m = cell.pieces - base < kBigBatch ? cell.pieces - base : kBigBatch;
@@ -480,26 +554,68 @@ __global__ void __launch_bounds__(256) big_cell_kernel(
 // ---------------------------------------------------------------------------------------
 // 4. node assembly
 // ---------------------------------------------------------------------------------------
+// One thread per 4 consecutive z-nodes (i, j, k0..k0+3): they share the 2x2x5 cells
+// (ex, ey, ez) in {i, i+1} x {j, j+1} x {k0 .. k0+4}, whose map entries are loaded once.
+// The 32 sums loads are issued together (absent cells read slot 0 and are not added), so
+// a wave waits for two rounds of memory latency per 4 nodes; a wave with no occupied cell
+// writes zeros without loading sums. Per node the order is that of step 5 above.
+constexpr int kAsmNodes = 4;
+
 __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ map,
                                                        const float* __restrict__ sums,
                                                        int64_t cap, int n,
-                                                       float* __restrict__ rho) {
-  const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int64_t idx = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= n2 * n) return;
-  const pm::Idx3 q3 = pm::idx3(idx, n);
-  const int i = q3.i, j = q3.j, k = q3.k;
+                                                       float* __restrict__ rho, bool vec4) {
+  const int kq = (n + kAsmNodes - 1) / kAsmNodes;
+  const int64_t items = static_cast<int64_t>(n) * n * kq;
+  const int64_t t = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= items) return;
+  const int k0 = static_cast<int>(t % kq) * kAsmNodes;
+  const int64_t ij = t / kq;
+  const int j = static_cast<int>(ij % n);
+  const int i = static_cast<int>(ij / n);
   const int64_t m = n + 1;
-  float acc = 0.0f;
+  int32_t cell[2][2][kAsmNodes + 1];
+  bool any = false;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int ex = i + 1 - (c >> 2);
-    const int ey = j + 1 - ((c >> 1) & 1);
-    const int ez = k + 1 - (c & 1);
-    const int32_t slot = map[(ex * m + ey) * m + ez];
-    if (slot >= 0) acc = acc + sums[c * cap + slot];
+  for (int ax = 0; ax < 2; ++ax) {
+#pragma unroll
+    for (int ay = 0; ay < 2; ++ay) {
+      const int32_t* row = map + ((i + ax) * m + (j + ay)) * m;
+#pragma unroll
+      for (int z = 0; z <= kAsmNodes; ++z) {
+        const int ez = k0 + z;
+        cell[ax][ay][z] = ez <= n ? row[ez] : -1;
+        any |= cell[ax][ay][z] >= 0;
+      }
+    }
   }
-  rho[idx] = acc;
+  float out[kAsmNodes] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (__ballot(any)) {
+#pragma unroll
+    for (int q = 0; q < kAsmNodes; ++q) {
+      float v[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int32_t slot = cell[1 - (c >> 2)][1 - ((c >> 1) & 1)][q + 1 - (c & 1)];
+        v[c] = sums[c * cap + (slot >= 0 ? slot : 0)];
+      }
+      float acc = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int32_t slot = cell[1 - (c >> 2)][1 - ((c >> 1) & 1)][q + 1 - (c & 1)];
+        acc = slot >= 0 ? acc + v[c] : acc;
+      }
+      out[q] = acc;
+    }
+  }
+  float* dst = rho + (static_cast<int64_t>(i) * n + j) * n + k0;
+  if (vec4) {  // n % 4 == 0 and rho 16-byte aligned
+    *reinterpret_cast<float4*>(dst) = make_float4(out[0], out[1], out[2], out[3]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < kAsmNodes; ++q)
+      if (k0 + q < n) dst[q] = out[q];
+  }
 }
 
 __global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* __restrict__ p, int64_t count,
@@ -672,6 +788,8 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   const int64_t nodes = static_cast<int64_t>(n) * n * n;
   const int64_t m = static_cast<int64_t>(n) + 1;
   const int64_t cells = m * m * m;
+  const int64_t asm_items = static_cast<int64_t>(n) * n * ((n + kAsmNodes - 1) / kAsmNodes);
+  const bool vec4 = (n % kAsmNodes) == 0 && (reinterpret_cast<uintptr_t>(rho) & 15u) == 0;
   const bool timing = plan->timing;
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[0], stream));
 
@@ -686,8 +804,8 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     }
     {
       PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
-      assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
-                                                                         plan->cap, n, rho);
+      assemble_kernel<<<pm::blocks_for(asm_items), pm::kBlock, 0, stream>>>(
+          plan->map, plan->sums, plan->cap, n, rho, vec4);
       PM_CHECK_LAUNCH("assemble_kernel");
     }
     return PM_OK;
@@ -766,6 +884,12 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     PM_CHECK_LAUNCH("join_kernel");
   }
   {
+    PM_TIMED("deposit.mid_cell", 0.0, stream);
+    mid_cell_kernel<<<1024, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
+                                              plan->tail, plan->sums, plan->cap, plan->map);
+    PM_CHECK_LAUNCH("mid_cell_kernel");
+  }
+  {
     PM_TIMED("deposit.big_cell", 0.0, stream);
     big_cell_kernel<<<256, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
                                              plan->tail, plan->sums, plan->cap, plan->map);
@@ -776,8 +900,8 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   // 4. nodes
   {
     PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
-    assemble_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, stream>>>(plan->map, plan->sums,
-                                                                       plan->cap, n, rho);
+    assemble_kernel<<<pm::blocks_for(asm_items), pm::kBlock, 0, stream>>>(
+        plan->map, plan->sums, plan->cap, n, rho, vec4);
     PM_CHECK_LAUNCH("assemble_kernel");
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[4], stream));

@@ -15,6 +15,7 @@
 // float32 resolution of phi (~1e-7 relative), far below the 1e-5 target. f is the same
 // float32 4*pi*G*rho as PLAIN mode and the faces are the same float32 values, so both
 // modes solve the identical discrete problem.
+#include <algorithm>
 #include <cmath>
 #include <utility>
 #include <vector>
@@ -249,15 +250,8 @@ __global__ void __launch_bounds__(256) restrict_kernel(const float* __restrict__
 // Trilinear prolongation of the coarse correction added into the fine interior
 // (multigrid.py:100-130): an even fine index takes coarse i/2 with weight 1, an odd one
 // takes i/2 and i/2+1 with weight 1/2 each; terms are summed in (di, dj, dk) order.
-__global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restrict__ phi_c,
-                                                          float* __restrict__ phi_f, int nf,
-                                                          int nc) {
-  const int64_t nf2 = static_cast<int64_t>(nf) * nf;
-  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= nf2 * nf) return;
-  const pm::Idx3 q3 = pm::idx3(c, nf);
-  const int i = q3.i, j = q3.j, k = q3.k;
-  if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) return;
+__device__ __forceinline__ float prolong_value(const float* __restrict__ phi_c, int nc, int i,
+                                               int j, int k) {
   const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
   const int oi = i & 1, oj = j & 1, ok = k & 1;
   const int64_t nc2 = static_cast<int64_t>(nc) * nc;
@@ -277,7 +271,19 @@ __global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restric
       }
     }
   }
-  phi_f[c] = phi_f[c] + val;
+  return val;
+}
+
+__global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restrict__ phi_c,
+                                                          float* __restrict__ phi_f, int nf,
+                                                          int nc) {
+  const int64_t nf2 = static_cast<int64_t>(nf) * nf;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= nf2 * nf) return;
+  const pm::Idx3 q3 = pm::idx3(c, nf);
+  const int i = q3.i, j = q3.j, k = q3.k;
+  if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) return;
+  phi_f[c] = phi_f[c] + prolong_value(phi_c, nc, i, j, k);
 }
 
 __global__ void __launch_bounds__(256) fill_f32_kernel(float* __restrict__ p, int64_t count,
@@ -485,7 +491,10 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
         s64_rhs[c] = rhs[c];
       }
       __syncthreads();
-      for (int q = 0; q < d.coarse64_sweeps; ++q) {
+      // a 3^3 level has one interior node, whose neighbours are all (zero) faces: the first
+      // sweep gives its exact value and every further sweep recomputes that same value
+      const int sweeps = nc == 3 && d.coarse64_sweeps > 1 ? 1 : d.coarse64_sweeps;
+      for (int q = 0; q < sweeps; ++q) {
         for (int color = 0; color < 2; ++color) {
           for (int t = threadIdx.x; t < nc * nc * half; t += blockDim.x) {
             const int kk = t % half, j = (t / half) % nc, i = t / (half * nc);
@@ -502,7 +511,7 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
       for (int c = threadIdx.x; c < nn; c += blockDim.x) phi[c] = static_cast<float>(s64_phi[c]);
       __syncthreads();
     } else {
-      lds_smooth(phi, rhs, nc, d.h2[L - 1], d.coarse_sweeps);
+      lds_smooth(phi, rhs, nc, d.h2[L - 1], nc == 3 && d.coarse_sweeps > 1 ? 1 : d.coarse_sweeps);
     }
   }
   // up
@@ -599,6 +608,479 @@ __global__ void __launch_bounds__(256) update_kernel(float* __restrict__ phi,
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Finest level of the MIXED correction in colour-split layout
+// ---------------------------------------------------------------------------------------
+// The correction e and the defect r of the finest level are internal to the solver, so they
+// are stored split by colour: node (i, j, k) of colour q = (i+j+k)&1 sits at
+//   q * plane + (i*n + j) * hn + (k >> 1),   hn = (n+1)/2, plane = n*n*hn
+// (odd n leaves one pad slot in half of the rows). Every neighbour of a node has the other
+// colour, and (i+-1, j, k), (i, j+-1, k) sit at the same k>>1, so a colour pass reads half
+// of e and half of r, contiguously, instead of all of both. Each kernel below evaluates the
+// same float expressions in the same order as its natural-layout counterpart, so the
+// solve is bit for bit that of the natural-layout V-cycle (fill, rb passes, residual,
+// prolongation, update), with three launches folded away:
+//   * the correction starts at zero, so the first colour-0 pass needs no neighbour loads
+//     (their sum is +0) and no fill precedes it: the first colour-0 and colour-1 passes
+//     rewrite every interior node, and the faces are zero from allocation on;
+//   * the colour-0 pass after the prolongation overwrites colour-0 nodes without reading
+//     them, so the prolongation is added to colour-1 nodes only;
+//   * the last colour-1 pass also applies phi += e to both colours and accumulates the
+//     norms, so e is not written back or read again.
+struct SplitIdx {
+  int i, j, kk;
+};
+__device__ __forceinline__ SplitIdx split_idx(int64_t t, int n, int hn) {
+  const uint32_t u = static_cast<uint32_t>(t), uh = static_cast<uint32_t>(hn);
+  const uint32_t q = u / uh;
+  SplitIdx r;
+  r.kk = static_cast<int>(u - q * uh);
+  r.j = static_cast<int>(q % static_cast<uint32_t>(n));
+  r.i = static_cast<int>(q / static_cast<uint32_t>(n));
+  return r;
+}
+
+// Sum of the six neighbours (other colour) of the node at position t, in the order of
+// rb_pass_kernel: i+1, i-1, j+1, j-1, k+1, k-1.
+__device__ __forceinline__ float split_nbr_sum(const float* __restrict__ eo, int64_t t, int kk,
+                                               int k, int64_t nh, int hn) {
+  const int64_t row = t - kk;
+  return eo[t + nh] + eo[t - nh] + eo[t + hn] + eo[t - hn] + eo[row + ((k + 1) >> 1)] +
+         eo[row + ((k - 1) >> 1)];
+}
+
+// One colour pass (rb_pass_kernel's update); ZERO: every neighbour is zero.
+template <bool ZERO>
+__global__ void __launch_bounds__(256) rbs_kernel(float* __restrict__ e,
+                                                  const float* __restrict__ r, int n, int hn,
+                                                  float h2, int color) {
+  const int64_t plane = static_cast<int64_t>(n) * n * hn;
+  const int64_t t = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= plane) return;
+  const SplitIdx q = split_idx(t, n, hn);
+  const int k = 2 * q.kk + ((q.i + q.j + color) & 1);
+  if (q.i < 1 || q.i > n - 2 || q.j < 1 || q.j > n - 2 || k < 1 || k > n - 2) return;
+  const int64_t own = static_cast<int64_t>(color) * plane;
+  const float sum =
+      ZERO ? 0.0f
+           : split_nbr_sum(e + (plane - own), t, q.kk, k, static_cast<int64_t>(n) * hn, hn);
+  e[own + t] = (sum - h2 * r[own + t]) / 6.0f;
+}
+
+// residual_kernel from the split e and r into the natural-layout res.
+__global__ void __launch_bounds__(256) residual_s_kernel(const float* __restrict__ e,
+                                                         const float* __restrict__ r,
+                                                         float* __restrict__ res, int n,
+                                                         int hn, float inv_h2) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= n2 * n) return;
+  const pm::Idx3 q3 = pm::idx3(c, n);
+  const int i = q3.i, j = q3.j, k = q3.k;
+  float out = 0.0f;
+  if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
+    const int64_t plane = n2 * hn;
+    const int64_t own = static_cast<int64_t>((i + j + k) & 1) * plane;
+    const int64_t t = (static_cast<int64_t>(i) * n + j) * hn + (k >> 1);
+    const float sum = split_nbr_sum(e + (plane - own), t, k >> 1, k,
+                                    static_cast<int64_t>(n) * hn, hn);
+    const float lap = (sum - 6.0f * e[own + t]) * inv_h2;
+    out = r[own + t] - lap;
+  }
+  res[c] = out;
+}
+
+// prolong_add_kernel on the colour-1 nodes of the split e.
+__global__ void __launch_bounds__(256) prolong_s_kernel(const float* __restrict__ phi_c,
+                                                        float* __restrict__ e, int n, int hn,
+                                                        int nc) {
+  const int64_t plane = static_cast<int64_t>(n) * n * hn;
+  const int64_t t = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= plane) return;
+  const SplitIdx q = split_idx(t, n, hn);
+  const int k = 2 * q.kk + ((q.i + q.j + 1) & 1);
+  if (q.i < 1 || q.i > n - 2 || q.j < 1 || q.j > n - 2 || k < 1 || k > n - 2) return;
+  e[plane + t] = e[plane + t] + prolong_value(phi_c, nc, q.i, q.j, k);
+}
+
+// The last colour-1 pass fused with update_kernel: phi += e over every node (e = 0 on the
+// faces), partial sums of e^2 and phi^2.
+__global__ void __launch_bounds__(256) black_update_kernel(const float* __restrict__ e,
+                                                           const float* __restrict__ r,
+                                                           float* __restrict__ phi, int n,
+                                                           int hn, float h2,
+                                                           double* __restrict__ partials) {
+  __shared__ double s_red[4 * 2];
+  double v[2] = {0.0, 0.0};
+  const int64_t plane = static_cast<int64_t>(n) * n * hn;
+  const int64_t nh = static_cast<int64_t>(n) * hn;
+  int64_t r_begin, r_end;
+  pm::block_range(plane, r_begin, r_end);
+  for (int64_t t = r_begin + threadIdx.x; t < r_end; t += blockDim.x) {
+    const SplitIdx q = split_idx(t, n, hn);
+    const int64_t row = (static_cast<int64_t>(q.i) * n + q.j) * n;
+    const int k0 = 2 * q.kk + ((q.i + q.j) & 1);
+    if (k0 < n) {
+      const float d = e[t];
+      const float p = phi[row + k0] + d;
+      phi[row + k0] = p;
+      v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
+      v[1] = v[1] + static_cast<double>(p) * static_cast<double>(p);
+    }
+    const int k1 = 2 * q.kk + ((q.i + q.j + 1) & 1);
+    if (k1 < n) {
+      float d = 0.0f;
+      if (q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2 && k1 >= 1 && k1 <= n - 2)
+        d = (split_nbr_sum(e, t, q.kk, k1, nh, hn) - h2 * r[plane + t]) / 6.0f;
+      const float p = phi[row + k1] + d;
+      phi[row + k1] = p;
+      v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
+      v[1] = v[1] + static_cast<double>(p) * static_cast<double>(p);
+    }
+  }
+  pm::block_sum<2>(v, s_red);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * pm::kRedMaxQ] = v[0];
+    partials[blockIdx.x * pm::kRedMaxQ + 1] = v[1];
+  }
+}
+
+// residual64_kernel writing r32 in split layout (same sums, same partials).
+__global__ void __launch_bounds__(256) residual64_s_kernel(const float* __restrict__ phi,
+                                                           const float* __restrict__ f, int n,
+                                                           int hn, double inv_h2,
+                                                           float* __restrict__ r32s,
+                                                           double* __restrict__ partials) {
+  __shared__ double s_red[4];
+  double v[1] = {0.0};
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t total = n2 * n;
+  const int64_t plane = n2 * hn;
+  int64_t r_begin, r_end;
+  pm::block_range(total, r_begin, r_end);
+  for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
+    const pm::Idx3 q3 = pm::idx3(c, n);
+    const int i = q3.i, j = q3.j, k = q3.k;
+    double r = 0.0;
+    if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
+      const double sum = static_cast<double>(phi[c + n2]) + static_cast<double>(phi[c - n2]) +
+                         static_cast<double>(phi[c + n]) + static_cast<double>(phi[c - n]) +
+                         static_cast<double>(phi[c + 1]) + static_cast<double>(phi[c - 1]);
+      const double lap = (sum - 6.0 * static_cast<double>(phi[c])) * inv_h2;
+      r = static_cast<double>(f[c]) - lap;
+    }
+    r32s[static_cast<int64_t>((i + j + k) & 1) * plane +
+         (static_cast<int64_t>(i) * n + j) * hn + (k >> 1)] = static_cast<float>(r);
+    v[0] = v[0] + r * r;
+  }
+  pm::block_sum<1>(v, s_red);
+  if (threadIdx.x == 0) partials[blockIdx.x * pm::kRedMaxQ] = v[0];
+}
+
+// --- 4-wide variants (n % 8 == 0): a thread owns positions kk0 .. kk0+3 of one row ------
+// The index math runs once per 4 nodes and the loads are 16-byte vectors: the (i+-1) and
+// (j+-1) rows, the node's own row of the other colour (its k+-1 neighbours, plus one
+// scalar past either end) and the rhs. The per-node expressions are those of the scalar
+// kernels, in the same order.
+struct Quad {
+  int i, j, kk0;
+  uint32_t t0;  // position of kk0 within a colour plane
+};
+__device__ __forceinline__ Quad quad_idx(uint32_t u, int n, int hq) {
+  // u = (i*n + j) * hq + kq, hq = hn / 4
+  const uint32_t row = u / static_cast<uint32_t>(hq);
+  Quad q;
+  q.kk0 = static_cast<int>(u - row * static_cast<uint32_t>(hq)) * 4;
+  q.j = static_cast<int>(row % static_cast<uint32_t>(n));
+  q.i = static_cast<int>(row / static_cast<uint32_t>(n));
+  q.t0 = u * 4u;
+  return q;
+}
+__device__ __forceinline__ float4 ld4(const float* __restrict__ p, uint32_t off) {
+  return *reinterpret_cast<const float4*>(p + off);
+}
+__device__ __forceinline__ float at4(const float4& v, int e) {
+  return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+}
+
+// Neighbour sums of the 4 nodes of parity p = (i+j+color)&1 in a row of the colour-plane
+// `eo` (the other colour), in rb_pass_kernel's order. m = the other colour's same-row quad,
+// edge = eo[t0-1] (p == 0) or eo[t0+4] (p == 1).
+__device__ __forceinline__ void quad_nbr_sums(const float* __restrict__ eo, uint32_t t0,
+                                              uint32_t nh, uint32_t hn, int p, float s[4]) {
+  const float4 a = ld4(eo, t0 + nh), b = ld4(eo, t0 - nh), c = ld4(eo, t0 + hn),
+               d = ld4(eo, t0 - hn), m = ld4(eo, t0);
+  const float edge = p ? eo[t0 + 4] : eo[t0 - 1];
+  // k = 2(kk0+e)+p: the k+1 neighbour sits at kk0+e+p, the k-1 neighbour at kk0+e+p-1
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const float kp = p ? (e < 3 ? at4(m, e + 1) : edge) : at4(m, e);
+    const float km = p ? at4(m, e) : (e > 0 ? at4(m, e - 1) : edge);
+    s[e] = at4(a, e) + at4(b, e) + at4(c, e) + at4(d, e) + kp + km;
+  }
+}
+
+template <bool ZERO>
+__global__ void __launch_bounds__(256) rbs4_kernel(float* __restrict__ e,
+                                                   const float* __restrict__ r, int n, int hn,
+                                                   float h2, int color) {
+  const int hq = hn >> 2;
+  const uint32_t quads = static_cast<uint32_t>(n) * n * hq;
+  const int64_t u = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (u >= quads) return;
+  const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
+  if (q.i < 1 || q.i > n - 2 || q.j < 1 || q.j > n - 2) return;
+  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
+  const uint32_t own = static_cast<uint32_t>(color) * plane;
+  const int p = (q.i + q.j + color) & 1;
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (!ZERO)
+    quad_nbr_sums(e + (plane - own), q.t0, static_cast<uint32_t>(n) * hn,
+                  static_cast<uint32_t>(hn), p, s);
+  const float4 rr = ld4(r, own + q.t0);
+  float out[4];
+#pragma unroll
+  for (int k4 = 0; k4 < 4; ++k4) {
+    const int k = 2 * (q.kk0 + k4) + p;
+    // faces (k = 0, k = n-1) and pad slots keep their zero
+    out[k4] = (k >= 1 && k <= n - 2) ? (s[k4] - h2 * at4(rr, k4)) / 6.0f : 0.0f;
+  }
+  *reinterpret_cast<float4*>(e + own + q.t0) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// residual_s_kernel, 4 positions of both colours (8 consecutive natural nodes) per thread.
+__global__ void __launch_bounds__(256) residual_s4_kernel(const float* __restrict__ e,
+                                                          const float* __restrict__ r,
+                                                          float* __restrict__ res, int n,
+                                                          int hn, float inv_h2) {
+  const int hq = hn >> 2;
+  const uint32_t quads = static_cast<uint32_t>(n) * n * hq;
+  const int64_t u = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (u >= quads) return;
+  const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
+  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
+  float out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2) {
+    const uint32_t nh = static_cast<uint32_t>(n) * hn;
+#pragma unroll
+    for (int color = 0; color < 2; ++color) {
+      const uint32_t own = static_cast<uint32_t>(color) * plane;
+      const int p = (q.i + q.j + color) & 1;
+      float s[4];
+      quad_nbr_sums(e + (plane - own), q.t0, nh, static_cast<uint32_t>(hn), p, s);
+      const float4 ev = ld4(e, own + q.t0), rv = ld4(r, own + q.t0);
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4) {
+        const int k = 2 * (q.kk0 + k4) + p;
+        if (k >= 1 && k <= n - 2) {
+          const float lap = (s[k4] - 6.0f * at4(ev, k4)) * inv_h2;
+          out[2 * k4 + p] = at4(rv, k4) - lap;
+        }
+      }
+    }
+  }
+  float* dst = res + (static_cast<int64_t>(q.i) * n + q.j) * n + 2 * q.kk0;
+  if (2 * q.kk0 + 8 <= n) {
+    *reinterpret_cast<float4*>(dst) = make_float4(out[0], out[1], out[2], out[3]);
+    *reinterpret_cast<float4*>(dst + 4) = make_float4(out[4], out[5], out[6], out[7]);
+  } else {
+    for (int k = 0; k < 8 && 2 * q.kk0 + k < n; ++k) dst[k] = out[k];
+  }
+}
+
+// prolong_s_kernel, 4 colour-1 positions per thread. The 4 nodes share i, j and the parity
+// of k, so their coarse terms come from the same (di, dj) rows at 4 or 5 consecutive coarse
+// k, loaded once; each node's sum keeps prolong_value's order.
+__global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict__ phi_c,
+                                                         float* __restrict__ e, int n, int hn,
+                                                         int nc) {
+  const int hq = hn >> 2;
+  const uint32_t quads = static_cast<uint32_t>(n) * n * hq;
+  const int64_t u = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (u >= quads) return;
+  const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
+  if (q.i < 1 || q.i > n - 2 || q.j < 1 || q.j > n - 2) return;
+  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
+  const int p = (q.i + q.j + 1) & 1;  // parity of k; k>>1 = kk0 + e
+  const int ci = q.i >> 1, cj = q.j >> 1, oi = q.i & 1, oj = q.j & 1;
+  const uint32_t nc2 = static_cast<uint32_t>(nc) * nc;
+  float c[2][2][5];
+#pragma unroll
+  for (int di = 0; di < 2; ++di) {
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const bool need = (di == 0 || oi) && (dj == 0 || oj);
+      const uint32_t row = (ci + di) * nc2 + static_cast<uint32_t>(cj + dj) * nc + q.kk0;
+#pragma unroll
+      for (int x = 0; x < 5; ++x)
+        c[di][dj][x] = (need && (x < 4 || p)) ? phi_c[row + x] : 0.0f;
+    }
+  }
+  const float4 v = ld4(e, plane + q.t0);
+  float out[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k4 = 0; k4 < 4; ++k4) {
+    const int k = 2 * (q.kk0 + k4) + p;
+    if (k < 1 || k > n - 2) continue;
+    float val = 0.0f;
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+      const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
+#pragma unroll
+      for (int dj = 0; dj < 2; ++dj) {
+        const float wj = oj ? 0.5f : (dj == 0 ? 1.0f : 0.0f);
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk) {
+          const float wk = p ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
+          const float w = wi * wj * wk;
+          if (w > 0.0f) val += w * c[di][dj][k4 + dk];
+        }
+      }
+    }
+    out[k4] = out[k4] + val;
+  }
+  *reinterpret_cast<float4*>(e + plane + q.t0) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// residual64_s_kernel, 8 consecutive natural nodes per thread (n % 8 == 0): the r values
+// are the same; the squared norm is summed in another order.
+__global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restrict__ phi,
+                                                            const float* __restrict__ f, int n,
+                                                            int hn, double inv_h2,
+                                                            float* __restrict__ r32s,
+                                                            double* __restrict__ partials) {
+  __shared__ double s_red[4];
+  double v[1] = {0.0};
+  const int hq = hn >> 2;
+  const int64_t quads = static_cast<int64_t>(n) * n * hq;
+  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
+  const uint32_t n2 = static_cast<uint32_t>(n) * n;
+  int64_t r_begin, r_end;
+  pm::block_range(quads, r_begin, r_end);
+  for (int64_t u = r_begin + threadIdx.x; u < r_end; u += blockDim.x) {
+    const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
+    const uint32_t c0 = (static_cast<uint32_t>(q.i) * n + q.j) * n + 2 * q.kk0;
+    float r8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2) {
+      float ctr[10], ip[8], im[8], jp[8], jm[8], fv[8];
+      const float4 c_a = ld4(phi, c0), c_b = ld4(phi, c0 + 4);
+      ctr[0] = phi[c0 - 1];
+      ctr[1] = c_a.x; ctr[2] = c_a.y; ctr[3] = c_a.z; ctr[4] = c_a.w;
+      ctr[5] = c_b.x; ctr[6] = c_b.y; ctr[7] = c_b.z; ctr[8] = c_b.w;
+      ctr[9] = phi[c0 + 8];
+      auto load8 = [&](float* dst, const float* src, uint32_t off) {
+        const float4 a = ld4(src, off), b = ld4(src, off + 4);
+        dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
+        dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
+      };
+      load8(ip, phi, c0 + n2);
+      load8(im, phi, c0 - n2);
+      load8(jp, phi, c0 + n);
+      load8(jm, phi, c0 - n);
+      load8(fv, f, c0);
+#pragma unroll
+      for (int x = 0; x < 8; ++x) {
+        const int k = 2 * q.kk0 + x;
+        if (k < 1 || k > n - 2) continue;
+        const double sum = static_cast<double>(ip[x]) + static_cast<double>(im[x]) +
+                           static_cast<double>(jp[x]) + static_cast<double>(jm[x]) +
+                           static_cast<double>(ctr[x + 2]) + static_cast<double>(ctr[x]);
+        const double lap = (sum - 6.0 * static_cast<double>(ctr[x + 1])) * inv_h2;
+        const double r = static_cast<double>(fv[x]) - lap;
+        r8[x] = static_cast<float>(r);
+        v[0] = v[0] + r * r;
+      }
+    }
+    // colour of natural k = 2*kk0 + x is (p0 + x) & 1; its position kk0 + (x >> 1)
+    const int p0 = (q.i + q.j) & 1;
+    float a0[4], a1[4];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      if (((p0 + x) & 1) == 0) a0[x >> 1] = r8[x]; else a1[x >> 1] = r8[x];
+    }
+    *reinterpret_cast<float4*>(r32s + q.t0) = make_float4(a0[0], a0[1], a0[2], a0[3]);
+    *reinterpret_cast<float4*>(r32s + plane + q.t0) = make_float4(a1[0], a1[1], a1[2], a1[3]);
+  }
+  pm::block_sum<1>(v, s_red);
+  if (threadIdx.x == 0) partials[blockIdx.x * pm::kRedMaxQ] = v[0];
+}
+
+// black_update_kernel, 4 positions of both colours (8 natural nodes) per thread.
+__global__ void __launch_bounds__(256) black_update4_kernel(const float* __restrict__ e,
+                                                            const float* __restrict__ r,
+                                                            float* __restrict__ phi, int n,
+                                                            int hn, float h2,
+                                                            double* __restrict__ partials) {
+  __shared__ double s_red[4 * 2];
+  double v[2] = {0.0, 0.0};
+  const int hq = hn >> 2;
+  const int64_t quads = static_cast<int64_t>(n) * n * hq;
+  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
+  const uint32_t nh = static_cast<uint32_t>(n) * hn;
+  int64_t r_begin, r_end;
+  pm::block_range(quads, r_begin, r_end);
+  for (int64_t u = r_begin + threadIdx.x; u < r_end; u += blockDim.x) {
+    const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
+    const bool inner_row = q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2;
+    const int p0 = (q.i + q.j) & 1, p1 = p0 ^ 1;
+    const float4 e0 = ld4(e, q.t0);
+    float d1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (inner_row) {
+      float s[4];
+      quad_nbr_sums(e, q.t0, nh, static_cast<uint32_t>(hn), p1, s);
+      const float4 rv = ld4(r, plane + q.t0);
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4) {
+        const int k = 2 * (q.kk0 + k4) + p1;
+        if (k >= 1 && k <= n - 2) d1[k4] = (s[k4] - h2 * at4(rv, k4)) / 6.0f;
+      }
+    }
+    float* dst = phi + (static_cast<int64_t>(q.i) * n + q.j) * n + 2 * q.kk0;
+    float ph[8];
+    const bool full = 2 * q.kk0 + 8 <= n;
+    if (full) {
+      const float4 a = *reinterpret_cast<const float4*>(dst);
+      const float4 b = *reinterpret_cast<const float4*>(dst + 4);
+      ph[0] = a.x; ph[1] = a.y; ph[2] = a.z; ph[3] = a.w;
+      ph[4] = b.x; ph[5] = b.y; ph[6] = b.z; ph[7] = b.w;
+    } else {
+      for (int k = 0; k < 8; ++k) ph[k] = 2 * q.kk0 + k < n ? dst[k] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (2 * q.kk0 + k >= n) continue;
+      const int k4 = k >> 1;
+      const float d = ((k & 1) == p0) ? at4(e0, k4) : d1[k4];
+      const float pv = ph[k] + d;
+      ph[k] = pv;
+      v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
+      v[1] = v[1] + static_cast<double>(pv) * static_cast<double>(pv);
+    }
+    if (full) {
+      *reinterpret_cast<float4*>(dst) = make_float4(ph[0], ph[1], ph[2], ph[3]);
+      *reinterpret_cast<float4*>(dst + 4) = make_float4(ph[4], ph[5], ph[6], ph[7]);
+    } else {
+      for (int k = 0; k < 8 && 2 * q.kk0 + k < n; ++k) dst[k] = ph[k];
+    }
+  }
+  pm::block_sum<2>(v, s_red);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * pm::kRedMaxQ] = v[0];
+    partials[blockIdx.x * pm::kRedMaxQ + 1] = v[1];
+  }
+}
+
+// natural -> split copy (the learned warm start hands over its defect in natural layout).
+__global__ void __launch_bounds__(256) to_split_kernel(const float* __restrict__ src,
+                                                       float* __restrict__ dst, int n, int hn) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= n2 * n) return;
+  const pm::Idx3 q3 = pm::idx3(c, n);
+  dst[static_cast<int64_t>((q3.i + q3.j + q3.k) & 1) * n2 * hn +
+      (static_cast<int64_t>(q3.i) * n + q3.j) * hn + (q3.k >> 1)] = src[c];
+}
+
 struct Level {
   int n = 0;
   double h = 0.0;
@@ -663,6 +1145,23 @@ struct pm_mg {
   float* net_out = nullptr;   // [R^3] network output
   float* scales = nullptr;    // [8] input/output scales
   std::vector<void*> owned;
+  // V-cycle launches replayed from HIP graphs (captured on a private stream, launched on the
+  // caller's): one graph per kind and set of buffer addresses it bakes in.
+  struct CycleGraph {
+    int kind;
+    const void* key[4];
+    hipGraphExec_t exec;
+  };
+  std::vector<CycleGraph> graphs;
+  hipStream_t capture = nullptr;
+  bool use_graphs = true;
+  // MIXED: the finest level's correction and defect in colour-split layout (allocated when
+  // the hierarchy has a coarser level below the finest and pre/post sweeps are >= 1)
+  float* e0s = nullptr;
+  float* r32s = nullptr;
+  int hn = 0;
+  bool use_split = true;
+  bool split_active() const { return e0s != nullptr && use_split; }
 };
 
 namespace {
@@ -676,6 +1175,8 @@ pm_status dalloc(pm_mg* mg, void** p, size_t bytes) {
 
 void destroy(pm_mg* mg) {
   if (!mg) return;
+  for (auto& g : mg->graphs) (void)hipGraphExecDestroy(g.exec);
+  if (mg->capture) (void)hipStreamDestroy(mg->capture);
   for (void* p : mg->owned) (void)hipFree(p);
   if (mg->host) (void)hipHostFree(mg->host);
   delete mg;
@@ -813,6 +1314,87 @@ pm_status vcycle(pm_mg* mg, size_t l, hipStream_t s) {
   return smooth(L, mg->d.post_sweeps, s);
 }
 
+// Workgroups of the split path's fused update (the partials the fold has to sum).
+unsigned split_update_blocks(const pm_mg* mg, const float* phi) {
+  const int n = mg->lv[0].n;
+  const int64_t plane = static_cast<int64_t>(n) * n * mg->hn;
+  const bool quad4 = n % 8 == 0 && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0;
+  return pm::red_blocks(quad4 ? plane / 4 : plane);
+}
+
+// The finest level of the MIXED correction in split layout: e0s = V(r32s) from a zero
+// start (pre/post sweeps >= 1), then phi += e0s with the norms in partials.
+pm_status vcycle_split0(pm_mg* mg, float* phi, hipStream_t s) {
+  const Level& L = mg->lv[0];
+  const Level& C = mg->lv[1];
+  const int n = L.n, hn = mg->hn;
+  const int64_t plane = static_cast<int64_t>(n) * n * hn;
+  const double nodes = static_cast<double>(L.nodes());
+  float* e = mg->e0s;
+  const float* r = mg->r32s;
+  // 4 positions per thread when rows split into aligned quads (n % 8 == 0)
+  const bool quad = n % 8 == 0;
+  const bool phi_aligned = (reinterpret_cast<uintptr_t>(phi) & 15u) == 0;
+  const int64_t quads = plane / 4;
+  auto pass = [&](int color, bool zero) -> pm_status {
+    PM_TIMED(level_name(zero ? "mg.rbs0" : "mg.rbs", n), (zero ? 4.0 : 6.0) * nodes, s);
+    if (quad && zero)
+      rbs4_kernel<true><<<pm::blocks_for(quads), pm::kBlock, 0, s>>>(e, r, n, hn, L.h2, color);
+    else if (quad)
+      rbs4_kernel<false><<<pm::blocks_for(quads), pm::kBlock, 0, s>>>(e, r, n, hn, L.h2, color);
+    else if (zero)
+      rbs_kernel<true><<<pm::blocks_for(plane), pm::kBlock, 0, s>>>(e, r, n, hn, L.h2, color);
+    else
+      rbs_kernel<false><<<pm::blocks_for(plane), pm::kBlock, 0, s>>>(e, r, n, hn, L.h2, color);
+    PM_CHECK_LAUNCH("rbs_kernel");
+    return PM_OK;
+  };
+  pm_status st;
+  for (int q = 0; q < mg->d.pre_sweeps; ++q) {
+    if ((st = pass(0, q == 0)) || (st = pass(1, false))) return st;
+  }
+  {
+    PM_TIMED(level_name("mg.residual_s", n), kResidualBytes * nodes, s);
+    if (quad)
+      residual_s4_kernel<<<pm::blocks_for(quads), pm::kBlock, 0, s>>>(e, r, L.res, n, hn,
+                                                                      L.inv_h2);
+    else
+      residual_s_kernel<<<pm::blocks_for(L.nodes()), pm::kBlock, 0, s>>>(e, r, L.res, n, hn,
+                                                                          L.inv_h2);
+    PM_CHECK_LAUNCH("residual_s_kernel");
+  }
+  {
+    PM_TIMED(level_name("mg.restrict", n), kRestrictBytes * nodes, s);
+    restrict_kernel<<<pm::blocks_for(C.nodes()), pm::kBlock, 0, s>>>(L.res, C.rhs, C.phi, n,
+                                                                      C.n);
+    PM_CHECK_LAUNCH("restrict_kernel");
+  }
+  if ((st = vcycle(mg, 1, s))) return st;
+  {
+    PM_TIMED(level_name("mg.prolong_s", n), 4.0 * nodes, s);
+    if (quad)
+      prolong_s4_kernel<<<pm::blocks_for(quads), pm::kBlock, 0, s>>>(C.phi, e, n, hn, C.n);
+    else
+      prolong_s_kernel<<<pm::blocks_for(plane), pm::kBlock, 0, s>>>(C.phi, e, n, hn, C.n);
+    PM_CHECK_LAUNCH("prolong_s_kernel");
+  }
+  for (int q = 0; q < mg->d.post_sweeps; ++q) {
+    if ((st = pass(0, false))) return st;
+    if (q + 1 < mg->d.post_sweeps && (st = pass(1, false))) return st;
+  }
+  {
+    PM_TIMED("mg.black_update", 12.0 * nodes, s);
+    if (quad && phi_aligned)
+      black_update4_kernel<<<pm::red_blocks(quads), pm::kBlock, 0, s>>>(e, r, phi, n, hn, L.h2,
+                                                                         mg->partials);
+    else
+      black_update_kernel<<<pm::red_blocks(plane), pm::kBlock, 0, s>>>(e, r, phi, n, hn,
+                                                                        L.h2, mg->partials);
+    PM_CHECK_LAUNCH("black_update_kernel");
+  }
+  return PM_OK;
+}
+
 // Copies `count` float64 results from the device (synchronizes the stream).
 pm_status read_scalars(pm_mg* mg, const double* dev, int count, double* out, hipStream_t s) {
   PM_TRY_HIP(hipMemcpyAsync(mg->host, dev, sizeof(double) * count, hipMemcpyDeviceToHost, s));
@@ -840,14 +1422,24 @@ pm_status plain_residual_norm(pm_mg* mg, const float* phi, double* out, hipStrea
   return PM_OK;
 }
 
-// r = f - A phi (float64 arithmetic) -> r32; its sum of squares lands in red[2] (no sync).
-pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s) {
+// r = f - A phi (float64 arithmetic) -> r32 (natural), or r32s (split layout) when `split`;
+// its sum of squares lands in red[2] (no sync).
+pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split = false) {
   const Level& L = mg->lv[0];
-  const unsigned b = pm::red_blocks(L.nodes());
+  // 8 nodes per thread on split grids with n % 8 == 0 and 16-byte aligned phi and f
+  const bool quad = split && L.n % 8 == 0 && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0;
+  const unsigned b = pm::red_blocks(quad ? L.nodes() / 8 : L.nodes());
   {
     PM_TIMED("mg.residual64", kRes64Bytes * L.nodes(), s);
-    residual64_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, 1.0 / (L.h * L.h),
-                                                mg->r32, mg->partials);
+    if (quad)
+      residual64_s4_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, mg->hn, 1.0 / (L.h * L.h),
+                                                     mg->r32s, mg->partials);
+    else if (split)
+      residual64_s_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, mg->hn, 1.0 / (L.h * L.h),
+                                                    mg->r32s, mg->partials);
+    else
+      residual64_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, 1.0 / (L.h * L.h),
+                                                  mg->r32, mg->partials);
     PM_CHECK_LAUNCH("residual64_kernel");
   }
   pm::fold_partials_kernel<1><<<1, pm::kBlock, 0, s>>>(mg->partials, static_cast<int>(b),
@@ -856,12 +1448,22 @@ pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s) {
   return PM_OK;
 }
 
-// One defect-correction iteration: e = V(r32); phi += e. Returns ||e||, ||phi||
-// (synchronizing).
-pm_status mixed_correct(pm_mg* mg, float* phi, double* e_norm, double* phi_norm,
-                        hipStream_t s) {
+// The launches of one V-cycle step. kCycleCorrect (MIXED): e0 = 0, e0 = V(r32), phi += e0,
+// red[0] = ||e0||^2, red[1] = ||phi||^2. kCyclePlain (PLAIN): phi = V(phi) on f0.
+enum { kCycleCorrect = 0, kCyclePlain = 1 };
+constexpr size_t kMaxGraphs = 4;
+
+pm_status enqueue_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
   Level& L0 = mg->lv[0];
   pm_status st;
+  if (kind == kCyclePlain) return vcycle(mg, 0, s);
+  if (mg->split_active()) {
+    if ((st = vcycle_split0(mg, phi, s))) return st;
+    pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(
+        mg->partials, static_cast<int>(split_update_blocks(mg, phi)), mg->red);
+    PM_CHECK_LAUNCH("fold_partials_kernel<2>");
+    return PM_OK;
+  }
   if ((st = fill(mg->e0, L0.nodes(), 0.0f, s))) return st;
   if ((st = vcycle(mg, 0, s))) return st;
   const unsigned b = pm::red_blocks(L0.nodes());
@@ -873,11 +1475,52 @@ pm_status mixed_correct(pm_mg* mg, float* phi, double* e_norm, double* phi_norm,
   pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(mg->partials, static_cast<int>(b),
                                                         mg->red);
   PM_CHECK_LAUNCH("fold_partials_kernel<2>");
-  double v[2];
-  if ((st = read_scalars(mg, mg->red, 2, v, s))) return st;
-  *e_norm = std::sqrt(v[0]);
-  *phi_norm = std::sqrt(v[1]);
   return PM_OK;
+}
+
+// Runs enqueue_cycle on stream s, from a HIP graph when graphs are on and per-kernel timing
+// is off (the timer's events need eager launches). A cycle is some 40 launches, most of
+// them on coarse levels that finish faster than the host can launch them; a graph replays
+// them back to back. The graph bakes in the addresses of phi and of the level-0 buffers
+// (the learned warm start swaps r32 with the level-0 residual), so it is keyed on them.
+pm_status run_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
+  if (!mg->use_graphs || pm::timing_enabled()) return enqueue_cycle(mg, kind, phi, s);
+  const Level& L0 = mg->lv[0];
+  const void* key[4] = {phi, L0.phi, L0.rhs, L0.res};
+  hipGraphExec_t exec = nullptr;
+  for (const auto& g : mg->graphs)
+    if (g.kind == kind && std::equal(key, key + 4, g.key)) exec = g.exec;
+  if (!exec) {
+    if (!mg->capture) PM_TRY_HIP(hipStreamCreateWithFlags(&mg->capture, hipStreamNonBlocking));
+    PM_TRY_HIP(hipStreamBeginCapture(mg->capture, hipStreamCaptureModeThreadLocal));
+    const pm_status st = enqueue_cycle(mg, kind, phi, mg->capture);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(mg->capture, &graph);
+    if (st) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    if (e != hipSuccess) return pm::hip_fail("hipStreamEndCapture (multigrid cycle)", e);
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return pm::hip_fail("hipGraphInstantiate (multigrid cycle)", ei);
+    if (mg->graphs.size() >= kMaxGraphs) {
+      (void)hipGraphExecDestroy(mg->graphs.front().exec);
+      mg->graphs.erase(mg->graphs.begin());
+    }
+    pm_mg::CycleGraph g{kind, {key[0], key[1], key[2], key[3]}, exec};
+    mg->graphs.push_back(g);
+  }
+  PM_TRY_HIP(hipGraphLaunch(exec, s));
+  return PM_OK;
+}
+
+// One defect-correction iteration: e = V(r32); phi += e. Returns (synchronizing) red[0..3]:
+// ||e||^2, ||phi||^2, the squared defect norm this correction started from, ||f||^2.
+pm_status mixed_correct(pm_mg* mg, float* phi, double v[4], hipStream_t s) {
+  pm_status st;
+  if ((st = run_cycle(mg, kCycleCorrect, phi, s))) return st;
+  return read_scalars(mg, mg->red, 4, v, s);
 }
 
 // Moments, f = 4*pi*G*rho, the initial guess (zero unless warm) with multipole faces, and
@@ -935,7 +1578,7 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
     L0.rhs = mg->f0;
     mg->last_phi = phi;
     for (int c = 0; c < mg->d.n_cycles; ++c) {
-      if ((st = vcycle(mg, 0, s))) return st;
+      if ((st = run_cycle(mg, kCyclePlain, phi, s))) return st;
       if (c + 1 >= mg->d.min_cycles && rhs_norm > 0.0) {
         if ((st = plain_residual_norm(mg, phi, &resid, s))) return st;
         if (resid < mg->d.tol * rhs_norm) {
@@ -945,14 +1588,26 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
       }
     }
   } else {
-    L0.phi = mg->e0;
-    L0.rhs = mg->r32;
+    const bool split = mg->split_active();
+    L0.phi = split ? mg->e0s : mg->e0;
+    L0.rhs = split ? mg->r32s : mg->r32;
     mg->last_phi = phi;
-    if (!residual_ready && (st = mixed_residual(mg, phi, s))) return st;
+    if (!residual_ready) {
+      if ((st = mixed_residual(mg, phi, s, split))) return st;
+    } else if (split) {
+      PM_TIMED("mg.to_split", 8.0 * L0.nodes(), s);
+      to_split_kernel<<<pm::blocks_for(L0.nodes()), pm::kBlock, 0, s>>>(mg->r32, mg->r32s,
+                                                                        L0.n, mg->hn);
+      PM_CHECK_LAUNCH("to_split_kernel");
+    }
     double prev_e = -1.0;
+    double v[4] = {0.0, 0.0, -1.0, 0.0};
+    bool read = false;
     for (int c = 0; c < mg->d.n_cycles; ++c) {
-      double e_norm, phi_norm;
-      if ((st = mixed_correct(mg, phi, &e_norm, &phi_norm, s))) return st;
+      if ((st = mixed_correct(mg, phi, v, s))) return st;
+      read = true;
+      const double e_norm = std::sqrt(v[0]);
+      const double phi_norm = std::sqrt(v[1]);
       // contraction estimate from successive corrections; 0.5 before two are known
       double rate = prev_e > 0.0 ? e_norm / prev_e : 0.5;
       if (rate > 0.95) rate = 0.95;
@@ -962,12 +1617,12 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
         cycles = c + 1;
         break;
       }
-      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s))) return st;
+      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s, split))) return st;
     }
-    double v[2];
-    if ((st = read_scalars(mg, mg->red + 2, 2, v, s))) return st;
-    resid = std::sqrt(v[0]);
-    rhs_norm = std::sqrt(v[1]);
+    // the last correction's read holds the defect it started from and ||f||^2
+    if (!read && (st = read_scalars(mg, mg->red, 4, v, s))) return st;
+    resid = std::sqrt(v[2]);
+    rhs_norm = std::sqrt(v[3]);
   }
   mg->solved = true;
   if (stats) {
@@ -1049,6 +1704,24 @@ pm_status pm_mg_create(const pm_mg_desc* desc, pm_mg** out) {
         (st = dalloc(mg, reinterpret_cast<void**>(&mg->r32), sizeof(float) * n0))) {
       destroy(mg);
       return st;
+    }
+    if (mg->lv.size() > 1 && mg->bottom_from != 0 && d.pre_sweeps >= 1 && d.post_sweeps >= 1) {
+      const int n = mg->lv[0].n;
+      mg->hn = (n + 1) / 2;
+      const size_t bytes = sizeof(float) * 2 * static_cast<size_t>(n) * n * mg->hn;
+      if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->e0s), bytes)) ||
+          (st = dalloc(mg, reinterpret_cast<void**>(&mg->r32s), bytes))) {
+        destroy(mg);
+        return st;
+      }
+      // faces and pad slots of e0s stay zero for good (no kernel writes them)
+      hipError_t ze = hipMemset(mg->e0s, 0, bytes);
+      if (ze == hipSuccess) ze = hipMemset(mg->r32s, 0, bytes);
+      if (ze == hipSuccess) ze = hipDeviceSynchronize();
+      if (ze != hipSuccess) {
+        destroy(mg);
+        return pm::hip_fail("hipMemset (multigrid split buffers)", ze);
+      }
     }
   }
   hipError_t he = hipHostMalloc(&mg->host, sizeof(double) * pm::kRedMaxQ, 0);
@@ -1201,12 +1874,28 @@ pm_status pm_mg_vcycle(pm_mg* mg, hipStream_t s) {
   PM_REQUIRE(mg, "pm_mg_vcycle: null handle");
   PM_REQUIRE(mg->solved, "pm_mg_vcycle: no solve has run on this handle");
   if (mg->d.mode == PM_MG_MIXED) {
-    pm_status st = mixed_residual(mg, mg->last_phi, s);
+    pm_status st = mixed_residual(mg, mg->last_phi, s, mg->split_active());
     if (st) return st;
-    double e, p;
-    return mixed_correct(mg, mg->last_phi, &e, &p, s);
+    double v[4];
+    return mixed_correct(mg, mg->last_phi, v, s);
   }
   return vcycle(mg, 0, s);
+}
+
+pm_status pm_mg_set_option(pm_mg* mg, int32_t option, int32_t value) {
+  pm::clear_error();
+  PM_REQUIRE(mg, "pm_mg_set_option: null handle");
+  switch (option) {
+    case PM_MG_OPT_GRAPHS:
+      mg->use_graphs = value != 0;
+      return PM_OK;
+    case PM_MG_OPT_SPLIT_FINE:
+      mg->use_split = value != 0;
+      return PM_OK;
+    default:
+      PM_REQUIRE(false, "pm_mg_set_option: unknown option");
+  }
+  return PM_OK;
 }
 
 pm_status pm_mg_residual_norm(pm_mg* mg, double* out, hipStream_t s) {

@@ -172,6 +172,33 @@ __global__ void __launch_bounds__(256) gather_kick4_kernel(
   vz[p] = vz[p] + a2 * h;
 }
 
+// Step t's closing gather+kick(h_close) joined with step t+1's opening kick(h_open) and
+// drift(dt): the same float operations in the same order as gather_kick4 followed by
+// kick_drift, without the round trip of a and v through memory.
+__global__ void __launch_bounds__(256) gather_kick_drift4_kernel(
+    float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
+    const float4* __restrict__ g4, float* __restrict__ vx, float* __restrict__ vy,
+    float* __restrict__ vz, int64_t np, int n, float dx, float h_close, float h_open,
+    float dt) {
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const float x = px[p], y = py[p], z = pz[p];
+  float a0, a1, a2;
+  cic_gather4(x, y, z, g4, n, dx, a0, a1, a2);
+  const float v0 = vx[p] + a0 * h_close;
+  const float v1 = vy[p] + a1 * h_close;
+  const float v2 = vz[p] + a2 * h_close;
+  const float w0 = v0 + a0 * h_open;
+  const float w1 = v1 + a1 * h_open;
+  const float w2 = v2 + a2 * h_open;
+  vx[p] = w0;
+  vy[p] = w1;
+  vz[p] = w2;
+  px[p] = x + w0 * dt;
+  py[p] = y + w1 * dt;
+  pz[p] = z + w2 * dt;
+}
+
 __global__ void __launch_bounds__(256) gather_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
     const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gz,
@@ -368,6 +395,26 @@ pm_status pm_gather_kick4(const float* px, const float* py, const float* pz, con
     gather_kick4_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
         px, py, pz, reinterpret_cast<const float4*>(g4), ax, ay, az, vx, vy, vz, np, n, dx, h);
     PM_CHECK_LAUNCH("gather_kick4_kernel");
+  }
+  return PM_OK;
+}
+
+pm_status pm_gather_kick_drift4(float* px, float* py, float* pz, const float* g4, float* vx,
+                                float* vy, float* vz, int64_t np, int32_t n, float dx,
+                                float h_close, float h_open, float dt, hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_gather_kick_drift4: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(px && py && pz && g4 && vx && vy && vz, "pm_gather_kick_drift4: null pointer");
+  PM_REQUIRE((reinterpret_cast<uintptr_t>(g4) & 15u) == 0,
+             "pm_gather_kick_drift4: g4 must be 16-byte aligned");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_gather_kick_drift4: grid_size must be in [1, 1024]");
+  PM_REQUIRE(dx > 0.0f, "pm_gather_kick_drift4: dx must be positive");
+  {
+    PM_TIMED("gather_kick_drift", 48.0 * np + 12.0 * n * n * n, stream);
+    gather_kick_drift4_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+        px, py, pz, reinterpret_cast<const float4*>(g4), vx, vy, vz, np, n, dx, h_close,
+        h_open, dt);
+    PM_CHECK_LAUNCH("gather_kick_drift4_kernel");
   }
   return PM_OK;
 }

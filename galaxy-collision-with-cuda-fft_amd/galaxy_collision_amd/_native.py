@@ -18,6 +18,8 @@ PM_OK = 0
 PM_ERR_CAPACITY = 4
 PM_MG_PLAIN = 0
 PM_MG_MIXED = 1
+PM_MG_OPT_GRAPHS = 0
+PM_MG_OPT_SPLIT_FINE = 1
 PM_DEPOSIT_SORTED = 0
 PM_DEPOSIT_FIXED = 1
 PM_WARM_PREVIOUS = 0
@@ -147,6 +149,8 @@ _SIGNATURES = {
     "pm_grad4": [_vp, _vp, _i32, _f, _vp],
     "pm_gather_kick4": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _f,
                         _vp],
+    "pm_gather_kick_drift4": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _f, _f, _f,
+                              _vp],
     "pm_direct_accel": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _vp],
     "pm_mg_create": [ctypes.POINTER(MGDesc), ctypes.POINTER(_vp)],
     "pm_mg_destroy": [_vp],
@@ -155,6 +159,7 @@ _SIGNATURES = {
     "pm_mg_solve_warm": [_vp, _vp, _vp, ctypes.POINTER(MGWarm), ctypes.POINTER(MGStats), _vp],
     "pm_mg_warm_reset": [_vp],
     "pm_mg_vcycle": [_vp, _vp],
+    "pm_mg_set_option": [_vp, _i32, _i32],
     "pm_mg_residual_norm": [_vp, ctypes.POINTER(_d), _vp],
     "pm_mg_moments": [_vp, _vp, ctypes.POINTER(_d), _vp],
     "pm_rb_pass": [_vp, _vp, _i32, _f, _i32, _vp],

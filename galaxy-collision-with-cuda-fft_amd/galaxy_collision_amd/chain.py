@@ -10,7 +10,10 @@
 
 which is bit for bit the reference step kick -> drift -> accel_fn -> kick
 (reference integrator.py:100-131 with the accel_fn of sim.py:318-325), with the two
-particle passes fused. The interleaved gradient turns the gather's 24 scattered loads per
+particle passes fused. ``PMChain.run(steps, dt)`` runs consecutive steps with the closing
+kick of one step, the opening kick and the drift of the next in one particle pass
+(pm_gather_kick_drift4), which leaves every particle and field value bitwise as ``steps``
+calls of ``step`` would. The interleaved gradient turns the gather's 24 scattered loads per
 particle into 8 16-byte loads; it lives in the chain's own buffer, so the grid's ax/ay/az
 fields hold the gradient of the last prime()/accel() call, not of the last step.
 """
@@ -80,18 +83,31 @@ class PMChain:
     prime = accel
 
     def step(self, dt: float) -> None:
+        self.run(1, dt)
+
+    def run(self, steps: int, dt: float) -> None:
+        """``steps`` consecutive KDK steps of size dt; the state afterwards (positions,
+        velocities, acc, rho, phi) is bitwise that of ``steps`` calls of ``step``."""
+        if steps <= 0:
+            return
         p, g = self.parts, self.grid
         h = 0.5 * dt
-        nat.call("pm_kick_drift", self._p(p.pos_x), self._p(p.pos_y), self._p(p.pos_z),
-                 self._p(p.vel_x), self._p(p.vel_y), self._p(p.vel_z),
-                 *(nat.ptr(a) for a in self.acc), p.n, h, dt, nat.stream())
-        self.deposit()
-        self.solve()
-        nat.call("pm_grad4", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx, nat.stream())
-        nat.call("pm_gather_kick4", self._p(p.pos_x), self._p(p.pos_y), self._p(p.pos_z),
-                 nat.ptr(self.g4), *(nat.ptr(a) for a in self.acc),
-                 self._p(p.vel_x), self._p(p.vel_y), self._p(p.vel_z), p.n, self.n, self.dx, h,
+        pos = (self._p(p.pos_x), self._p(p.pos_y), self._p(p.pos_z))
+        vel = (self._p(p.vel_x), self._p(p.vel_y), self._p(p.vel_z))
+        nat.call("pm_kick_drift", *pos, *vel, *(nat.ptr(a) for a in self.acc), p.n, h, dt,
                  nat.stream())
+        for s in range(steps):
+            self.deposit()
+            self.solve()
+            nat.call("pm_grad4", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx,
+                     nat.stream())
+            if s + 1 < steps:
+                nat.call("pm_gather_kick_drift4", *pos, nat.ptr(self.g4), *vel, p.n, self.n,
+                         self.dx, h, h, dt, nat.stream())
+            else:
+                nat.call("pm_gather_kick4", *pos, nat.ptr(self.g4),
+                         *(nat.ptr(a) for a in self.acc), *vel, p.n, self.n, self.dx, h,
+                         nat.stream())
 
 
 __all__ = ["PMChain"]

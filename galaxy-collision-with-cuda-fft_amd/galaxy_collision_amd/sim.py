@@ -184,8 +184,18 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
         write_snap(0, m0)
     if frame_cadence:
         capture_frame(0)
-    for step in range(1, config.steps + 1):
-        chain.step(config.dt)
+    def frame_due(step: int) -> bool:
+        return bool(frame_cadence) and (step % frame_cadence == 0 or step == config.steps)
+
+    step = 0
+    while step < config.steps:
+        # run up to the next step that is sampled, written or captured, in one fused run
+        nxt = step + 1
+        while not (nxt % cadence == 0 or nxt == config.steps or snapshot_due(nxt)
+                   or frame_due(nxt)):
+            nxt += 1
+        chain.run(nxt - step, config.dt)
+        step = nxt
         want_hist = step % cadence == 0 or step == config.steps
         want_snap = snapshot_due(step)
         if want_hist or want_snap:
@@ -194,7 +204,7 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
                 history.append(m)
             if want_snap:
                 write_snap(step, m)
-        if frame_cadence and (step % frame_cadence == 0 or step == config.steps):
+        if frame_due(step):
             capture_frame(step)
     torch.cuda.synchronize()
 

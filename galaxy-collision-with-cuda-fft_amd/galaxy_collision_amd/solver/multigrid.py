@@ -39,7 +39,7 @@ class _NativeMultigrid(PoissonSolver):
                  n_cycles: int = 20, pre_sweeps: int = 2, post_sweeps: int = 2,
                  coarse_sweeps: int = 40, tol: float = 1e-3, min_cycles: int = 1,
                  warm: str = "previous", net=None, pool: int = 4,
-                 fallback_ratio: float = 1.0):
+                 fallback_ratio: float = 1.0, graphs: bool = True, split_fine: bool = True):
         nat.require_device()
         if warm not in _WARM_MODES:
             raise ValueError(f"warm must be one of {sorted(_WARM_MODES)}, got {warm!r}")
@@ -67,6 +67,10 @@ class _NativeMultigrid(PoissonSolver):
         h = ctypes.c_void_p()
         nat.call("pm_mg_create", ctypes.byref(desc), ctypes.byref(h))
         self._h = h
+        # V-cycles replayed from HIP graphs, and (MIXED) the finest level in colour-split
+        # layout with fused launches: both bitwise the same as eager natural-layout cycles
+        nat.call("pm_mg_set_option", self._h, nat.PM_MG_OPT_GRAPHS, 1 if graphs else 0)
+        nat.call("pm_mg_set_option", self._h, nat.PM_MG_OPT_SPLIT_FINE, 1 if split_fine else 0)
         lv = ctypes.c_int32()
         nat.call("pm_mg_num_levels", self._h, ctypes.byref(lv))
         self.levels = lv.value
@@ -193,9 +197,10 @@ class MixedPrecisionMultigridSolver(_NativeMultigrid):
                  n_cycles: int = 40, pre_sweeps: int = 2, post_sweeps: int = 2,
                  coarse_sweeps: int = 40, tol: float = 1e-6, min_cycles: int = 1,
                  warm: str = "previous", net=None, pool: int = 4,
-                 fallback_ratio: float = 1.0):
+                 fallback_ratio: float = 1.0, graphs: bool = True, split_fine: bool = True):
         super().__init__(grid_size, dx, grav_constant, n_cycles, pre_sweeps, post_sweeps,
-                         coarse_sweeps, tol, min_cycles, warm, net, pool, fallback_ratio)
+                         coarse_sweeps, tol, min_cycles, warm, net, pool, fallback_ratio,
+                         graphs, split_fine)
 
 
 __all__ = ["MultigridPoissonSolver", "MixedPrecisionMultigridSolver"]

@@ -120,6 +120,16 @@ pm_status pm_gather_kick4(const float* pos_x, const float* pos_y, const float* p
                           float* vel_x, float* vel_y, float* vel_z, int64_t n_particles,
                           int32_t grid_size, float dx, float h, hipStream_t stream);
 
+/* Between two consecutive steps: gather from g4, closing kick(h_close) of step t, opening
+ * kick(h_open) and drift(dt) of step t+1, in one pass. Bitwise the same as pm_gather_kick4
+ * followed by pm_kick_drift (reference integrator.py:100-131 `kdk_step`, its closing kick
+ * followed by the next call's opening kick and drift); acc is not written, so the caller
+ * ends a run of steps with pm_gather_kick4. */
+pm_status pm_gather_kick_drift4(float* pos_x, float* pos_y, float* pos_z, const float* g4,
+                                float* vel_x, float* vel_y, float* vel_z, int64_t n_particles,
+                                int32_t grid_size, float dx, float h_close, float h_open,
+                                float dt, hipStream_t stream);
+
 /* Fused gather then kick(h): bitwise the same as pm_gather followed by pm_kick
  * (the closing half of reference integrator.py:100-131 `kdk_step`). */
 pm_status pm_gather_kick(const float* pos_x, const float* pos_y, const float* pos_z,
@@ -225,6 +235,21 @@ pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* 
                            pm_mg_stats* stats, hipStream_t stream);
 /* Forget the time history (the next warm solve starts from phi alone). */
 pm_status pm_mg_warm_reset(pm_mg* mg);
+
+/* Execution options of a solver handle (no reference counterpart: the reference's solver
+ * is a chain of Taichi kernel calls, multigrid.py:429-440). Results are bitwise the same
+ * with every option on or off.
+ *   PM_MG_OPT_GRAPHS (default 1): solves replay each V-cycle (with the MIXED correction's
+ *     fill, update and norms) from a HIP graph captured on first use for the buffers
+ *     involved, launched on the solve's stream; eager launches while per-kernel timing is
+ *     enabled.
+ *   PM_MG_OPT_SPLIT_FINE (default 1): MIXED solves keep the finest level's correction and
+ *     defect in colour-split layout, with the fill, the colour-0 prolongation and the
+ *     update folded into the smoothing passes (needs pre/post sweeps >= 1 and a level below
+ *     the finest; otherwise ignored). */
+#define PM_MG_OPT_GRAPHS 0
+#define PM_MG_OPT_SPLIT_FINE 1
+pm_status pm_mg_set_option(pm_mg* mg, int32_t option, int32_t value);
 
 /* One more V-cycle on the state of the last solve (reference multigrid.py:429-440
  * `_vcycle(0)` as called by tests). */

@@ -98,12 +98,13 @@ def test_gather_and_fused_gather_kick_bitwise(device, npart, n, dx):
         assert_bitwise(v2[:, a], O.kick(v1[:, a], ref[a], h))
 
 
-@pytest.mark.parametrize("mode", ["fixed", "sorted"])
-def test_fused_chain_step_equals_unfused_sequence(device, mode):
-    """PMChain.step (kick+drift fused, interleaved gradient, gather+kick fused) equals the
-    reference step kick -> drift -> deposit -> solve -> grad -> gather -> kick bit for bit
-    (reference integrator.py:100-131 with sim.py:318-325's accel_fn)."""
-    announce(f"fused chain step vs unfused ({mode} deposit)")
+@pytest.mark.parametrize("mode,how", [("fixed", "step"), ("sorted", "step"), ("sorted", "run")])
+def test_fused_chain_step_equals_unfused_sequence(device, mode, how):
+    """PMChain.step (kick+drift fused, interleaved gradient, gather+kick fused) and
+    PMChain.run (one step's closing kick, the next one's opening kick and drift in one pass)
+    equal the reference step kick -> drift -> deposit -> solve -> grad -> gather -> kick bit
+    for bit (reference integrator.py:100-131 with sim.py:318-325's accel_fn)."""
+    announce(f"fused chain {how} vs unfused ({mode} deposit)")
     from galaxy_collision_amd.chain import PMChain
     from galaxy_collision_amd.data import ParticleState
     from galaxy_collision_amd.deposit import (deposit_density, gather_acceleration,
@@ -124,8 +125,11 @@ def test_fused_chain_step_equals_unfused_sequence(device, mode):
     chain = PMChain(a, n, dx, make_solver("multigrid", n, dx=dx, grav_constant=1.0),
                     deposit_mode=mode)
     chain.prime()
-    for _ in range(3):
-        chain.step(dt)
+    if how == "run":
+        chain.run(3, dt)
+    else:
+        for _ in range(3):
+            chain.step(dt)
     b = state()
     solver = make_solver("multigrid", n, dx=dx, grav_constant=1.0)
     rho, phi = (torch.zeros((n, n, n), device=device) for _ in range(2))
@@ -148,6 +152,9 @@ def test_fused_chain_step_equals_unfused_sequence(device, mode):
         kick(*vels(b), *acc, icr.n, 0.5 * dt)
     assert_bitwise(a.positions(), b.positions(), "positions")
     assert_bitwise(a.velocities(), b.velocities(), "velocities")
+    for k in range(3):
+        assert_bitwise(_np(chain.acc[k]), _np(acc[k]), f"acc[{k}]")
+    assert_bitwise(chain.grid.phi.to_numpy(), _np(phi), "phi")
 
 
 def test_kick_drift_and_fused_bitwise(device):

@@ -178,3 +178,77 @@ def test_learned_warm_start_gain_over_extrapolation(device, seq_a, seq_b):
     # asserts that the network helps at all, keeps the accuracy and never costs cycles
     assert used >= 0.5 and np.median(ratio) < 0.99, (ratio, used)
     assert np.mean(c_l[2:]) <= np.mean(c_e[2:]) + 0.25
+
+
+@pytest.mark.parametrize("warm", ["previous", "extrapolate", "learned"])
+def test_graphs_and_split_fine_level_equal_eager_natural(device, seq_b, warm):
+    """Every solve replays its V-cycles from HIP graphs (captured once per set of buffers),
+    and MIXED solves run the finest level in colour-split layout with fused launches. Each
+    combination must give bitwise the phi and the cycle counts of eager natural-layout
+    cycles on every solve of a sequence (the norms only differ by summation order). The
+    learned mode with an always-accepting judge swaps the defect buffers every solve, so
+    its graphs are re-keyed and replayed alternately; the plain (float32) solver covers
+    the other graph kind."""
+    announce(f"multigrid graphs/split vs eager natural ({warm})")
+    from galaxy_collision_amd.solver.multigrid import (MixedPrecisionMultigridSolver,
+                                                       MultigridPoissonSolver)
+    from galaxy_collision_amd.warmstart import WarmStartNet
+
+    rhos, dx = seq_b
+    kw = dict(dx=dx, grav_constant=1.0)
+    if warm == "learned":
+        net = WarmStartNet(N_GRID // 4, seed=3)
+        kw.update(warm="learned", net=net, pool=4, fallback_ratio=1e30)
+    elif warm == "extrapolate":
+        kw.update(warm="extrapolate")
+    combos = [(False, False), (True, False), (False, True), (True, True)]
+    groups = [[MixedPrecisionMultigridSolver(N_GRID, graphs=g, split_fine=sp, **kw)
+               for g, sp in combos]]
+    if warm == "previous":
+        groups.append([MultigridPoissonSolver(N_GRID, dx=dx, grav_constant=1.0, graphs=g)
+                       for g in (False, True)])
+    used = 0
+    for t, rho in enumerate(rhos[:10]):
+        for group in groups:
+            phis = []
+            for s in group:
+                phi = getattr(s, "_test_phi", None)
+                if phi is None:
+                    phi = s._test_phi = torch.zeros((N_GRID,) * 3, device=device)
+                s.solve(rho, phi, warm_start=t > 0)
+                torch.cuda.synchronize()
+                phis.append(phi)
+            ref = group[0]
+            for s, phi in zip(group[1:], phis[1:]):
+                assert torch.equal(phi, phis[0]), (warm, t)
+                assert s.last_cycles == ref.last_cycles, (warm, t)
+                for key, v in ref.last_stats.items():
+                    assert s.last_stats[key] == pytest.approx(v, rel=1e-12, abs=0), (warm, t, key)
+        if warm == "learned" and groups[0][-1].last_warm:
+            used += groups[0][-1].last_warm["used_net"]
+    if warm == "learned":
+        assert used >= 5, used
+
+
+@pytest.mark.parametrize("n", [33, 37, 48])
+def test_split_fine_level_odd_and_even_grids(device, n):
+    """Colour-split finest level on odd grids (pad slot in half of the rows) and a grid whose
+    hierarchy reaches the bottom solver at its second level: bitwise the natural-layout
+    solve, cold and warm."""
+    announce(f"multigrid split fine level n={n}")
+    from galaxy_collision_amd.solver.multigrid import MixedPrecisionMultigridSolver
+
+    rng = np.random.default_rng(n)
+    rho = torch.zeros((n,) * 3, device=device)
+    inner = rng.uniform(0.0, 1.0, size=(n - 8,) * 3).astype(F32)
+    rho[4:-4, 4:-4, 4:-4] = torch.from_numpy(inner).to(device)
+    out = []
+    for split in (False, True):
+        s = MixedPrecisionMultigridSolver(n, dx=0.5, grav_constant=1.0, split_fine=split)
+        phi = torch.zeros((n,) * 3, device=device)
+        s.solve(rho, phi)
+        c0 = s.last_cycles
+        s.solve(rho * 1.01, phi, warm_start=True)
+        out.append((phi.clone(), c0, s.last_cycles))
+    assert torch.equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
