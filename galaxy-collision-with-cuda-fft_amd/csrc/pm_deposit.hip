@@ -39,22 +39,57 @@ struct BigCell {
 // ---------------------------------------------------------------------------------------
 // 1. keys and per-particle records
 // ---------------------------------------------------------------------------------------
+// One radix tile (kTile particles, item r of thread t at tile position r*256 + t) per
+// workgroup: keys, records and the tile's histogram of the first radix digit (low `bits`
+// bits), stored like radix_hist_kernel's, so the first sort pass skips its histogram.
+constexpr int kKeyItems = kTile / 256;
+
+__device__ __forceinline__ void tile_digit_count(uint32_t d, bool valid, int bits,
+                                                 uint64_t lanes_below, uint32_t* cnt) {
+  uint64_t match = __ballot(valid);
+  for (int bb = 0; bb < bits; ++bb) {
+    const bool bit = (d >> bb) & 1u;
+    const uint64_t bm = __ballot(bit);
+    match &= bit ? bm : ~bm;
+  }
+  if (valid && (match & lanes_below) == 0)
+    atomicAdd(&cnt[d], static_cast<uint32_t>(__popcll(match)));
+}
+
 __global__ void __launch_bounds__(256) key_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
     const float* __restrict__ mass, int64_t np, int n, float dx, float inv_vol,
-    uint32_t sentinel, uint32_t* __restrict__ keys, int32_t* __restrict__ vals,
-    float4* __restrict__ rec, uint32_t* __restrict__ keys_out) {
-  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  int ix = 0, iy = 0, iz = 0;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  const bool in = pm::cic_axis(px[p], dx, n, ix, fx) && pm::cic_axis(py[p], dx, n, iy, fy) &&
-                  pm::cic_axis(pz[p], dx, n, iz, fz);
-  const uint32_t key = in ? pm::cell_key(ix, iy, iz, n) : sentinel;
-  keys[p] = key;
-  vals[p] = static_cast<int32_t>(p);
-  rec[p] = make_float4(fx, fy, fz, mass[p] * inv_vol);
-  if (keys_out) keys_out[p] = key;
+    uint32_t sentinel, uint32_t* __restrict__ keys, float4* __restrict__ rec,
+    uint32_t* __restrict__ keys_out, int tiles, int bits, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[1 << kMaxDigitBits];
+  const int buckets = 1 << bits;
+  const uint32_t mask = static_cast<uint32_t>(buckets - 1);
+  for (int d = threadIdx.x; d < buckets; d += blockDim.x) cnt[d] = 0;
+  __syncthreads();
+  const uint64_t lanes_below = (1ull << (threadIdx.x & 63)) - 1ull;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
+  uint32_t kd[kKeyItems];
+#pragma unroll
+  for (int r = 0; r < kKeyItems; ++r) {
+    const int64_t p = base + r * 256 + threadIdx.x;
+    uint32_t key = 0;
+    if (p < np) {
+      int ix = 0, iy = 0, iz = 0;
+      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      const bool in = pm::cic_axis(px[p], dx, n, ix, fx) && pm::cic_axis(py[p], dx, n, iy, fy) &&
+                      pm::cic_axis(pz[p], dx, n, iz, fz);
+      key = in ? pm::cell_key(ix, iy, iz, n) : sentinel;
+      keys[p] = key;
+      rec[p] = make_float4(fx, fy, fz, mass[p] * inv_vol);
+      if (keys_out) keys_out[p] = key;
+    }
+    kd[r] = key & mask;
+  }
+  for (int r = 0; r < kKeyItems; ++r)
+    tile_digit_count(kd[r], base + r * 256 + threadIdx.x < np, bits, lanes_below, cnt);
+  __syncthreads();
+  for (int d = threadIdx.x; d < buckets; d += blockDim.x)
+    hist[static_cast<int64_t>(d) * tiles + blockIdx.x] = cnt[d];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -78,14 +113,7 @@ __global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restr
     const int64_t p = base + e * 256 + threadIdx.x;
     const bool valid = p < np;
     const uint32_t d = valid ? (keys[p] >> shift) & mask : 0u;
-    uint64_t match = __ballot(valid);
-    for (int bb = 0; bb < bits; ++bb) {
-      const bool bit = (d >> bb) & 1u;
-      const uint64_t bm = __ballot(bit);
-      match &= bit ? bm : ~bm;
-    }
-    if (valid && (match & lanes_below) == 0)
-      atomicAdd(&cnt[d], static_cast<uint32_t>(__popcll(match)));
+    tile_digit_count(d, valid, bits, lanes_below, cnt);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < buckets; d += blockDim.x)
@@ -159,7 +187,8 @@ __global__ void __launch_bounds__(256) radix_digit_start_kernel(uint32_t* __rest
 }
 
 // Stable scatter of one tile. Lane l of wave w, round r holds tile position
-// w*64*kSortItems + r*64 + l, so (wave, round, lane) order is input order.
+// w*64*kSortItems + r*64 + l, so (wave, round, lane) order is input order. A null vals_in
+// stands for the identity (the first pass sorts particle indices).
 __global__ void __launch_bounds__(256) radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
@@ -189,7 +218,7 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(
     const bool valid = p < np;
     const uint32_t key = valid ? keys_in[p] : 0u;
     k[r] = key;
-    v[r] = valid ? vals_in[p] : 0;
+    v[r] = valid ? (vals_in ? vals_in[p] : static_cast<int32_t>(p)) : 0;
     const uint32_t d = (key >> shift) & mask;
     uint64_t match = __ballot(valid);
     for (int b = 0; b < bits; ++b) {
@@ -229,6 +258,16 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(
 // ---------------------------------------------------------------------------------------
 // 3. segmented reduction
 // ---------------------------------------------------------------------------------------
+// A cell's 8 corner sums are one 32-byte record, sums[slot*8 + c] (slot = sorted position
+// of the cell's first particle): written with two 16-byte stores, and the 8 nodes that
+// read a cell's corners share its cache line.
+__device__ __forceinline__ void store_cell(float* __restrict__ sums, int64_t slot,
+                                           const float v[8]) {
+  float4* dst = reinterpret_cast<float4*>(sums + slot * 8);
+  dst[0] = make_float4(v[0], v[1], v[2], v[3]);
+  dst[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
 __device__ __forceinline__ void corner_terms(const float4 r, float t[8]) {
   const float wx0 = 1.0f - r.x, wy0 = 1.0f - r.y, wz0 = 1.0f - r.z;
 #pragma unroll
@@ -240,86 +279,77 @@ __device__ __forceinline__ void corner_terms(const float4 r, float t[8]) {
   }
 }
 
-// One thread per chunk: sequential piece sums. A workgroup stages its kPieceThreads
-// chunks (keys and the gathered particle records) in LDS with coalesced loads; the records
-// are stored structure-of-arrays with one pad slot per chunk so that the 64 lanes of a
-// wave, each walking its own chunk, hit 64 different banks. Complete cells are written
-// straight to sums/map; a chunk's continuation piece goes to head[chunk], its open last
-// piece to tail[chunk] (tail_slot = sorted position of that cell's first particle).
-constexpr int kPieceThreads = 128;
-constexpr int kPieceSpan = kPieceThreads * kChunk;       // sorted positions per workgroup
-constexpr int kPiecePadded = kPieceThreads * (kChunk + 1);
+// Piece sums with one lane per sorted position: a chunk of kChunk = 16 positions is one
+// 16-lane row of a wave, so keys, indices and the gathered records load coalesced and
+// nothing is staged in LDS. A run of equal keys inside a chunk is summed strictly in order
+// by a 15-step serial scan along the row (step s: lane s adds its terms to lane s-1's
+// running sum), which is the sequential piece sum of the contract above, bit for bit. The
+// lane where a run ends writes it: complete cells straight to sums/map, a chunk's
+// continuation piece to head[chunk], its open last piece to tail[chunk]; the chunk's last
+// position writes tail_slot (sorted position of the open cell's first particle, or -1) and
+// head_end (1 when the continued cell ends in this chunk).
+static_assert(kChunk == 16, "a chunk is one 16-lane row");
 
-__global__ void __launch_bounds__(kPieceThreads) piece_kernel(
+__global__ void __launch_bounds__(256) piece_kernel(
     const uint32_t* __restrict__ skey, const int32_t* __restrict__ sidx,
     const float4* __restrict__ rec, int64_t np, uint32_t sentinel, int64_t chunks,
     float* __restrict__ sums, int64_t cap, int32_t* __restrict__ map, float* __restrict__ head,
-    float* __restrict__ tail, int64_t* __restrict__ tail_slot) {
-  __shared__ uint32_t s_key[kPiecePadded + 1];
-  __shared__ float s_fx[kPiecePadded], s_fy[kPiecePadded], s_fz[kPiecePadded], s_m[kPiecePadded];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kPieceSpan;
-  // stage: position base+q -> padded slot q + q/kChunk; s_key[kPiecePadded] = key after span
-  for (int q = threadIdx.x; q < kPieceSpan; q += blockDim.x) {
-    const int64_t g = base + q;
-    const int slot = q + q / kChunk;
-    if (g < np) {
-      s_key[slot] = skey[g];
-      const float4 r = rec[sidx[g]];
-      s_fx[slot] = r.x;
-      s_fy[slot] = r.y;
-      s_fz[slot] = r.z;
-      s_m[slot] = r.w;
-    } else {
-      s_key[slot] = 0xFFFFFFFFu;
+    float* __restrict__ tail, int64_t* __restrict__ tail_slot, uint8_t* __restrict__ head_end) {
+  const int64_t g = pm::xcd_block() * blockDim.x + threadIdx.x;  // sorted position
+  const int lane = threadIdx.x & 63;
+  const int i = lane & (kChunk - 1);
+  const bool valid = g < np;
+  const uint32_t key = valid ? skey[g] : 0xFFFFFFFFu;
+  const uint32_t prevk = (valid && g > 0) ? skey[g - 1] : 0xFFFFFFFFu;
+  const uint32_t nextk = g + 1 < np ? skey[g + 1] : 0xFFFFFFFFu;
+  const bool live = valid && key < sentinel;
+  float t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (live) corner_terms(rec[sidx[g]], t);
+  // this position continues the run of the previous position of its chunk
+  const bool cont = i > 0 && live && prevk == key;
+  float acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = t[c];
+#pragma unroll
+  for (int st = 1; st < kChunk; ++st) {
+    const bool take = i == st && cont;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      // DPP row_shr:1: lane i of each 16-lane row reads lane i-1 of the same row
+      const float up = __int_as_float(
+          __builtin_amdgcn_update_dpp(0, __float_as_int(acc[c]), 0x111, 0xF, 0xF, true));
+      if (take) acc[c] = up + t[c];
     }
   }
-  if (threadIdx.x == 0)
-    s_key[kPiecePadded] = base + kPieceSpan < np ? skey[base + kPieceSpan] : 0xFFFFFFFFu;
-  __syncthreads();
-  const int64_t ch = static_cast<int64_t>(blockIdx.x) * kPieceThreads + threadIdx.x;
-  if (ch >= chunks) return;
-  const int64_t s = ch * kChunk;
-  const int e = static_cast<int>((s + kChunk < np ? s + kChunk : np) - s);  // positions here
-  const int row = threadIdx.x * (kChunk + 1);
-  const uint32_t prev = s == 0 ? 0xFFFFFFFFu
-                               : (threadIdx.x == 0 ? skey[s - 1] : s_key[row - 2]);
-  // key just after this chunk (next thread's first, or the staged one after the span)
-  const uint32_t after = e < kChunk ? 0xFFFFFFFFu
-                                    : (threadIdx.x + 1 < kPieceThreads ? s_key[row + kChunk + 1]
-                                                                       : s_key[kPiecePadded]);
-  int64_t open_slot = -1;
-  int i = 0;
-  while (i < e) {
-    const uint32_t key = s_key[row + i];
-    if (key >= sentinel) break;
-    float acc[8];
-    corner_terms(make_float4(s_fx[row + i], s_fy[row + i], s_fz[row + i], s_m[row + i]), acc);
-    int j = i + 1;
-    while (j < e && s_key[row + j] == key) {
-      float t[8];
-      corner_terms(make_float4(s_fx[row + j], s_fy[row + j], s_fz[row + j], s_m[row + j]), t);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) acc[c] = acc[c] + t[c];
-      ++j;
-    }
-    const bool starts = (i > 0) || (prev != key);
-    const bool ends = (j < e) || (after != key);
-    const int64_t gi = s + i;
+  // first lane of this position's run (within the chunk)
+  const uint64_t run_starts = __ballot(live && !cont);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const uint64_t below = run_starts & upto;
+  const int ls = below ? 63 - __builtin_clzll(below) : lane;
+  const uint32_t prev_at_start = __shfl(prevk, ls, 64);
+  const int64_t g_start = g - (lane - ls);
+  const bool end_here = live && (i == kChunk - 1 || g + 1 >= np || nextk != key);
+  const bool starts = g_start == 0 || prev_at_start != key;
+  const bool ends = g + 1 >= np || nextk != key;
+  const int64_t ch = g / kChunk;
+  if (end_here) {
     if (starts && ends) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) sums[c * cap + gi] = acc[c];
-      map[key] = static_cast<int32_t>(gi);
+      store_cell(sums, g_start, acc);
+      map[key] = static_cast<int32_t>(g_start);
     } else if (starts) {
 #pragma unroll
       for (int c = 0; c < 8; ++c) tail[ch * 8 + c] = acc[c];
-      open_slot = gi;
     } else {
 #pragma unroll
       for (int c = 0; c < 8; ++c) head[ch * 8 + c] = acc[c];
     }
-    i = j;
   }
-  tail_slot[ch] = open_slot;
+  // head_end[ch] = 1 when the run continued from the previous chunk ends in this one
+  const uint64_t closing = __ballot(end_here && !starts && ends);
+  if (valid && (i == kChunk - 1 || g + 1 == np)) {
+    tail_slot[ch] = (end_here && starts && !ends) ? g_start : -1;
+    head_end[ch] = ((closing >> (lane & ~(kChunk - 1))) & 0xFFFFull) != 0 ? 1 : 0;
+  }
 }
 
 // First sorted position after `from` whose key differs from `key` (keys are sorted).
@@ -347,7 +377,8 @@ __device__ __forceinline__ int64_t cell_end(const uint32_t* __restrict__ skey, i
 // unrolled aligned pairwise tree; larger ones are listed for big_cell_kernel.
 __global__ void __launch_bounds__(256) join_kernel(
     const uint32_t* __restrict__ skey, int64_t np, int64_t chunks,
-    const int64_t* __restrict__ tail_slot, const float* __restrict__ head,
+    const int64_t* __restrict__ tail_slot, const uint8_t* __restrict__ head_end,
+    const float* __restrict__ head,
     const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
     int32_t* __restrict__ map, BigCell* __restrict__ big, uint32_t* __restrict__ big_count) {
   const int64_t ch = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -355,9 +386,20 @@ __global__ void __launch_bounds__(256) join_kernel(
   const int64_t slot = tail_slot[ch];
   if (slot < 0) return;
   const uint32_t key = skey[slot];
-  const int64_t end = cell_end(skey, slot, np, key);
-  const int64_t last = (end - 1) / kChunk;
-  const int64_t pieces = last - ch + 1;
+  // the cell's chunks ch+1 .. last continue it; last is the first with head_end set, read
+  // as 8 bytes (head_end carries 16 bytes of padding)
+  const uint64_t base = static_cast<uint64_t>(ch) + 1;
+  const uint64_t* words = reinterpret_cast<const uint64_t*>(head_end);
+  const unsigned sh = static_cast<unsigned>(base & 7u) * 8u;
+  const uint64_t w0 = words[base >> 3];
+  const uint64_t win = sh == 0 ? w0 : (w0 >> sh) | (words[(base >> 3) + 1] << (64u - sh));
+  int64_t pieces;
+  if (win != 0 && __builtin_ctzll(win) / 8 + 2 <= kSmallPieces) {
+    pieces = __builtin_ctzll(win) / 8 + 2;
+  } else {
+    const int64_t end = cell_end(skey, slot, np, key);
+    pieces = (end - 1) / kChunk - ch + 1;
+  }
   if (pieces > kSmallPieces) {
     const uint32_t at = atomicAdd(big_count, 1u);
     big[at] = BigCell{slot, ch, pieces};
@@ -383,8 +425,7 @@ __global__ void __launch_bounds__(256) join_kernel(
       }
     }
   }
-#pragma unroll
-  for (int c = 0; c < 8; ++c) sums[c * cap + slot] = p[0][c];
+  store_cell(sums, slot, p[0]);
   map[key] = static_cast<int32_t>(slot);
 }
 
@@ -450,8 +491,7 @@ __global__ void __launch_bounds__(256) mid_cell_kernel(
     }
     // lane 0 holds the 8 corner sums
     if (lane == 0) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) sums[c * cap + cell.slot] = v[c];
+      store_cell(sums, cell.slot, v);
       map[skey[cell.slot]] = static_cast<int32_t>(cell.slot);
     }
   }
@@ -544,7 +584,7 @@ __global__ void __launch_bounds__(256) big_cell_kernel(
     if (tid < 8) {
       float acc = stack[depth - 1][tid];
       for (int d = depth - 2; d >= 0; --d) acc = stack[d][tid] + acc;
-      sums[tid * cap + cell.slot] = acc;
+      sums[cell.slot * 8 + tid] = acc;
     }
     if (tid == 0) map[skey[cell.slot]] = static_cast<int32_t>(cell.slot);
     __syncthreads();
@@ -597,7 +637,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const int32_t slot = cell[1 - (c >> 2)][1 - ((c >> 1) & 1)][q + 1 - (c & 1)];
-        v[c] = sums[c * cap + (slot >= 0 ? slot : 0)];
+        v[c] = sums[static_cast<int64_t>(slot >= 0 ? slot : 0) * 8 + c];
       }
       float acc = 0.0f;
 #pragma unroll
@@ -643,6 +683,7 @@ struct pm_deposit_plan {
   float* head = nullptr;
   float* tail = nullptr;
   int64_t* tail_slot = nullptr;
+  uint8_t* head_end = nullptr;  // chunks + 16 bytes (join reads 16-byte windows)
   int32_t* map = nullptr;
   BigCell* big = nullptr;
   uint32_t* big_count = nullptr;
@@ -659,8 +700,8 @@ namespace {
 void free_plan(pm_deposit_plan* p) {
   if (!p) return;
   void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->rec, p->hist, p->digit,
-                  p->sums, p->head, p->tail, p->tail_slot, p->map, p->big, p->big_count,
-                  p->acc64, p->maxbits};
+                  p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->map, p->big,
+                  p->big_count, p->acc64, p->maxbits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
@@ -709,6 +750,7 @@ pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) 
       (st = alloc(&p->head, 8 * chunks, "hipMalloc head")) ||
       (st = alloc(&p->tail, 8 * chunks, "hipMalloc tail")) ||
       (st = alloc(&p->tail_slot, chunks, "hipMalloc tail_slot")) ||
+      (st = alloc(&p->head_end, chunks + 16, "hipMalloc head_end")) ||
       (st = alloc(&p->map, m * m * m, "hipMalloc map")) ||
       (st = alloc(&p->big, chunks, "hipMalloc big")) ||
       (st = alloc(&p->big_count, 1, "hipMalloc big_count"))) {
@@ -813,22 +855,22 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
 
   // 1. keys
   const float inv_vol = 1.0f / ((dx * dx) * dx);
+  const int tiles = static_cast<int>((np + kTile - 1) / kTile);
   {
-    PM_TIMED("deposit.key", 40.0 * np, stream);
-    key_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
-        px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel, plan->keys[0], plan->vals[0],
-        plan->rec, keys_out);
+    PM_TIMED("deposit.key", 32.0 * np, stream);
+    key_kernel<<<tiles, 256, 0, stream>>>(px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel,
+                                          plan->keys[0], plan->rec, keys_out, tiles,
+                                          plan->digit_bits[0], plan->hist);
     PM_CHECK_LAUNCH("key_kernel");
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[1], stream));
 
-  // 2. sort
-  const int tiles = static_cast<int>((np + kTile - 1) / kTile);
+  // 2. sort (the first pass's histogram came with the keys)
   int src = 0, shift = 0;
   for (int s = 0; s < plan->passes; ++s) {
     const int bits = plan->digit_bits[s];
     const int buckets = 1 << bits;
-    {
+    if (s > 0) {
       PM_TIMED("deposit.radix_hist", 4.0 * np, stream);
       radix_hist_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], np, tiles, shift, bits,
                                                     plan->hist);
@@ -846,7 +888,8 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     }
     {
       PM_TIMED("deposit.radix_scatter", 16.0 * np, stream);
-      radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], plan->vals[src],
+      radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src],
+                                                       s == 0 ? nullptr : plan->vals[src],
                                                        plan->keys[1 - src], plan->vals[1 - src],
                                                        plan->hist, plan->digit, np, tiles, shift,
                                                        bits);
@@ -866,9 +909,9 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   const int64_t chunks = (np + kChunk - 1) / kChunk;
   {
     PM_TIMED("deposit.piece", 24.0 * np, stream);
-    piece_kernel<<<pm::blocks_for(chunks, kPieceThreads), kPieceThreads, 0, stream>>>(
+    piece_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
         skey, sidx, plan->rec, np, plan->sentinel, chunks, plan->sums, plan->cap, plan->map,
-        plan->head, plan->tail, plan->tail_slot);
+        plan->head, plan->tail, plan->tail_slot, plan->head_end);
     PM_CHECK_LAUNCH("piece_kernel");
   }
   {
@@ -879,8 +922,8 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   {
     PM_TIMED("deposit.join", 8.0 * chunks, stream);
     join_kernel<<<pm::blocks_for(chunks), pm::kBlock, 0, stream>>>(
-        skey, np, chunks, plan->tail_slot, plan->head, plan->tail, plan->sums, plan->cap,
-        plan->map, plan->big, plan->big_count);
+        skey, np, chunks, plan->tail_slot, plan->head_end, plan->head, plan->tail, plan->sums,
+        plan->cap, plan->map, plan->big, plan->big_count);
     PM_CHECK_LAUNCH("join_kernel");
   }
   {
