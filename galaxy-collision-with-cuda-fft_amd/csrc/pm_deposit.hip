@@ -39,11 +39,26 @@ struct BigCell {
 // ---------------------------------------------------------------------------------------
 // 1. keys and per-particle records
 // ---------------------------------------------------------------------------------------
-// One radix tile (kTile particles, item r of thread t at tile position r*256 + t) per
-// workgroup: keys, records and the tile's histogram of the first radix digit (low `bits`
-// bits), stored like radix_hist_kernel's, so the first sort pass skips its histogram.
-constexpr int kKeyItems = kTile / 256;
+__global__ void __launch_bounds__(256) key_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float* __restrict__ mass, int64_t np, int n, float dx, float inv_vol,
+    uint32_t sentinel, uint32_t* __restrict__ keys, float4* __restrict__ rec,
+    uint32_t* __restrict__ keys_out) {
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  int ix = 0, iy = 0, iz = 0;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  const bool in = pm::cic_axis(px[p], dx, n, ix, fx) && pm::cic_axis(py[p], dx, n, iy, fy) &&
+                  pm::cic_axis(pz[p], dx, n, iz, fz);
+  const uint32_t key = in ? pm::cell_key(ix, iy, iz, n) : sentinel;
+  keys[p] = key;
+  rec[p] = make_float4(fx, fy, fz, mass[p] * inv_vol);
+  if (keys_out) keys_out[p] = key;
+}
 
+// Lanes holding the same digit are grouped with one ballot per digit bit and only the
+// lowest such lane adds the group's count, so nearly sorted keys do not serialize on one
+// LDS counter.
 __device__ __forceinline__ void tile_digit_count(uint32_t d, bool valid, int bits,
                                                  uint64_t lanes_below, uint32_t* cnt) {
   uint64_t match = __ballot(valid);
@@ -56,48 +71,10 @@ __device__ __forceinline__ void tile_digit_count(uint32_t d, bool valid, int bit
     atomicAdd(&cnt[d], static_cast<uint32_t>(__popcll(match)));
 }
 
-__global__ void __launch_bounds__(256) key_kernel(
-    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    const float* __restrict__ mass, int64_t np, int n, float dx, float inv_vol,
-    uint32_t sentinel, uint32_t* __restrict__ keys, float4* __restrict__ rec,
-    uint32_t* __restrict__ keys_out, int tiles, int bits, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t cnt[1 << kMaxDigitBits];
-  const int buckets = 1 << bits;
-  const uint32_t mask = static_cast<uint32_t>(buckets - 1);
-  for (int d = threadIdx.x; d < buckets; d += blockDim.x) cnt[d] = 0;
-  __syncthreads();
-  const uint64_t lanes_below = (1ull << (threadIdx.x & 63)) - 1ull;
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
-  uint32_t kd[kKeyItems];
-#pragma unroll
-  for (int r = 0; r < kKeyItems; ++r) {
-    const int64_t p = base + r * 256 + threadIdx.x;
-    uint32_t key = 0;
-    if (p < np) {
-      int ix = 0, iy = 0, iz = 0;
-      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-      const bool in = pm::cic_axis(px[p], dx, n, ix, fx) && pm::cic_axis(py[p], dx, n, iy, fy) &&
-                      pm::cic_axis(pz[p], dx, n, iz, fz);
-      key = in ? pm::cell_key(ix, iy, iz, n) : sentinel;
-      keys[p] = key;
-      rec[p] = make_float4(fx, fy, fz, mass[p] * inv_vol);
-      if (keys_out) keys_out[p] = key;
-    }
-    kd[r] = key & mask;
-  }
-  for (int r = 0; r < kKeyItems; ++r)
-    tile_digit_count(kd[r], base + r * 256 + threadIdx.x < np, bits, lanes_below, cnt);
-  __syncthreads();
-  for (int d = threadIdx.x; d < buckets; d += blockDim.x)
-    hist[static_cast<int64_t>(d) * tiles + blockIdx.x] = cnt[d];
-}
-
 // ---------------------------------------------------------------------------------------
 // 2. stable LSD radix sort (reduce-then-scan; no inter-workgroup waiting)
 // ---------------------------------------------------------------------------------------
-// Per-tile digit histogram, stored digit-major: hist[d * tiles + tile]. Lanes holding the
-// same digit are grouped with one ballot per digit bit and only the lowest such lane adds
-// the group's count, so nearly sorted keys do not serialize on one LDS counter.
+// Per-tile digit histogram, stored digit-major: hist[d * tiles + tile].
 __global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restrict__ keys,
                                                          int64_t np, int tiles, int shift,
                                                          int bits, uint32_t* __restrict__ hist) {
@@ -855,22 +832,22 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
 
   // 1. keys
   const float inv_vol = 1.0f / ((dx * dx) * dx);
-  const int tiles = static_cast<int>((np + kTile - 1) / kTile);
   {
-    PM_TIMED("deposit.key", 32.0 * np, stream);
-    key_kernel<<<tiles, 256, 0, stream>>>(px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel,
-                                          plan->keys[0], plan->rec, keys_out, tiles,
-                                          plan->digit_bits[0], plan->hist);
+    PM_TIMED("deposit.key", 36.0 * np, stream);
+    key_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+        px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel, plan->keys[0], plan->rec,
+        keys_out);
     PM_CHECK_LAUNCH("key_kernel");
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[1], stream));
 
-  // 2. sort (the first pass's histogram came with the keys)
+  // 2. sort (the first pass sorts particle indices: no index array is read)
+  const int tiles = static_cast<int>((np + kTile - 1) / kTile);
   int src = 0, shift = 0;
   for (int s = 0; s < plan->passes; ++s) {
     const int bits = plan->digit_bits[s];
     const int buckets = 1 << bits;
-    if (s > 0) {
+    {
       PM_TIMED("deposit.radix_hist", 4.0 * np, stream);
       radix_hist_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], np, tiles, shift, bits,
                                                     plan->hist);

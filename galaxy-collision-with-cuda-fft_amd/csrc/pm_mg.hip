@@ -292,14 +292,6 @@ __global__ void __launch_bounds__(256) fill_f32_kernel(float* __restrict__ p, in
   if (i < count) p[i] = v;
 }
 
-// rhs = factor * rho (multigrid.py:249-252)
-__global__ void __launch_bounds__(256) scale_kernel(const float* __restrict__ src,
-                                                    float* __restrict__ dst, int64_t count,
-                                                    float factor) {
-  const int64_t i = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (i < count) dst[i] = factor * src[i];
-}
-
 // Multipole face potential (multigrid.py:261-295); p = [G, M, cx, cy, cz, sxx, syy, szz,
 // sxy, sxz, syz]. Thread (face, a, b) with face = 2*axis + side.
 __device__ __forceinline__ float face_value(int i, int j, int k, float dx, const float* p) {
@@ -1530,19 +1522,15 @@ pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStr
   const int n = L0.n;
   const int64_t nodes = L0.nodes();
   pm_status st;
-  // boundary moments (multigrid.py:458) -> float32 face parameters, on the device
-  {
-    PM_TIMED("mg.moments", 4.0 * nodes, s);
-    if ((st = pm::launch_moments(rho, n, mg->d.dx, mg->d.grav, mg->partials, mg->moments,
-                                 mg->face_params, s)))
-      return st;
-  }
-  // f = 4*pi*G*rho (multigrid.py:463)
+  // boundary moments (multigrid.py:458) -> float32 face parameters, on the device, with
+  // f = 4*pi*G*rho (multigrid.py:463) and ||f||^2 (multigrid.py:477, into red[3]) from the
+  // same pass over rho
   {
     const float factor = static_cast<float>(4.0 * kPi * mg->d.grav);
-    PM_TIMED("mg.scale", 8.0 * nodes, s);
-    scale_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(rho, mg->f0, nodes, factor);
-    PM_CHECK_LAUNCH("scale_kernel");
+    PM_TIMED("mg.moments", 8.0 * nodes, s);
+    if ((st = pm::launch_moments(rho, n, mg->d.dx, mg->d.grav, mg->partials, mg->moments,
+                                 mg->face_params, s, mg->f0, factor, mg->red + 3)))
+      return st;
   }
   if (!warm && (st = fill(phi, nodes, 0.0f, s))) return st;
   {
@@ -1550,11 +1538,6 @@ pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStr
     faces_kernel<<<pm::blocks_for(6 * static_cast<int64_t>(n) * n), pm::kBlock, 0, s>>>(
         phi, n, mg->d.dx, mg->face_params);
     PM_CHECK_LAUNCH("faces_kernel");
-  }
-  // ||rhs|| (multigrid.py:477)
-  {
-    PM_TIMED("mg.rhs_norm", 4.0 * nodes, s);
-    if ((st = pm::launch_sumsq_f32(mg->f0, nodes, mg->partials, mg->red + 3, s))) return st;
   }
   return PM_OK;
 }

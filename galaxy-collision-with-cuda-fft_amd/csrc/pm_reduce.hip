@@ -38,13 +38,19 @@ __global__ void __launch_bounds__(256) sumsq_f64_kernel(const double* __restrict
 }
 
 // Ten raw moments (reference multigrid.py:133-178): m = rho*dx^3, (x,y,z) = (i,j,k)*dx.
+// With F, the same pass also writes f = factor * rho (multigrid.py:249-252) and sums f^2
+// into partial slot 10 with sumsq_f32_kernel's partition and tree, so that sum is bitwise the
+// one a separate sumsq pass over f would give.
+template <bool F>
 __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ rho, int n,
                                                       double dxd, double vol,
-                                                      double* __restrict__ partials) {
-  __shared__ double s_red[4 * 10];
-  double v[10];
+                                                      double* __restrict__ partials,
+                                                      float* __restrict__ f, float factor) {
+  constexpr int Q = F ? 11 : 10;
+  __shared__ double s_red[4 * Q];
+  double v[Q];
 #pragma unroll
-  for (int q = 0; q < 10; ++q) v[q] = 0.0;
+  for (int q = 0; q < Q; ++q) v[q] = 0.0;
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const int64_t total = n2 * n;
   int64_t r_begin, r_end;
@@ -52,7 +58,14 @@ __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ 
   for (int64_t idx = r_begin + threadIdx.x; idx < r_end; idx += blockDim.x) {
     const pm::Idx3 q3 = pm::idx3(idx, n);
     const int i = q3.i, j = q3.j, k = q3.k;
-    const double m = static_cast<double>(rho[idx]) * vol;
+    const float r = rho[idx];
+    if (F) {
+      const float fv = factor * r;
+      f[idx] = fv;
+      const double t = static_cast<double>(fv);
+      v[Q - 1] = v[Q - 1] + t * t;
+    }
+    const double m = static_cast<double>(r) * vol;
     const double x = static_cast<double>(i) * dxd;
     const double y = static_cast<double>(j) * dxd;
     const double z = static_cast<double>(k) * dxd;
@@ -67,10 +80,10 @@ __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ 
     v[8] = v[8] + m * x * z;
     v[9] = v[9] + m * y * z;
   }
-  pm::block_sum<10>(v, s_red);
+  pm::block_sum<Q>(v, s_red);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < 10; ++q) partials[blockIdx.x * pm::kRedMaxQ + q] = v[q];
+    for (int q = 0; q < Q; ++q) partials[blockIdx.x * pm::kRedMaxQ + q] = v[q];
   }
 }
 
@@ -248,14 +261,26 @@ pm_status launch_sumsq_f64(const double* x, int64_t count, double* partials, dou
 }
 
 pm_status launch_moments(const float* rho, int n, float dx, double grav, double* partials,
-                         double* sums_dev, float* face_params, hipStream_t stream) {
+                         double* sums_dev, float* face_params, hipStream_t stream, float* f,
+                         float factor, double* f_sumsq_dev) {
   const int64_t total = static_cast<int64_t>(n) * n * n;
   const unsigned b = red_blocks(total);
   const double dxd = static_cast<double>(dx);
-  moments_kernel<<<b, kBlock, 0, stream>>>(rho, n, dxd, dxd * dxd * dxd, partials);
+  if (f) {
+    moments_kernel<true><<<b, kBlock, 0, stream>>>(rho, n, dxd, dxd * dxd * dxd, partials, f,
+                                                   factor);
+  } else {
+    moments_kernel<false><<<b, kBlock, 0, stream>>>(rho, n, dxd, dxd * dxd * dxd, partials,
+                                                    nullptr, 0.0f);
+  }
   PM_CHECK_LAUNCH("moments_kernel");
   fold_partials_kernel<10><<<1, kBlock, 0, stream>>>(partials, static_cast<int>(b), sums_dev);
   PM_CHECK_LAUNCH("fold_partials_kernel<10>");
+  if (f && f_sumsq_dev) {
+    fold_partials_kernel<1><<<1, kBlock, 0, stream>>>(partials + 10, static_cast<int>(b),
+                                                       f_sumsq_dev);
+    PM_CHECK_LAUNCH("fold_partials_kernel<1>");
+  }
   if (face_params) {
     moments_finalize_kernel<<<1, 1, 0, stream>>>(sums_dev, grav, face_params);
     PM_CHECK_LAUNCH("moments_finalize_kernel");
