@@ -28,6 +28,7 @@ constexpr int kSortItems = 16;       // keys per lane
 constexpr int kTile = kSortWaves * 64 * kSortItems;  // 4096 keys per radix tile
 constexpr int kMaxDigitBits = 9;     // 25-bit keys of a 256^3 grid sort in 3 passes
 constexpr int kSmallPieces = 8;      // open cells with <= this many pieces finish in one thread
+constexpr int kWavePieces = 256;     // ... in one wave; larger ones in a workgroup
 constexpr int kBigBatch = 1024;      // pieces per batch for large cells
 
 struct BigCell {
@@ -351,16 +352,20 @@ __device__ __forceinline__ int64_t cell_end(const uint32_t* __restrict__ skey, i
 }
 
 // Cells whose pieces span chunks: small ones (<= kSmallPieces pieces) finish here with an
-// unrolled aligned pairwise tree; larger ones are listed for big_cell_kernel.
+// unrolled aligned pairwise tree; up to kWavePieces pieces are marked in mid_pieces[chunk]
+// for mid_cell_kernel (no shared counter: 10^4 such cells would serialize on one
+// device-scope atomic); larger ones are listed for big_cell_kernel.
 __global__ void __launch_bounds__(256) join_kernel(
     const uint32_t* __restrict__ skey, int64_t np, int64_t chunks,
     const int64_t* __restrict__ tail_slot, const uint8_t* __restrict__ head_end,
     const float* __restrict__ head,
     const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
-    int32_t* __restrict__ map, BigCell* __restrict__ big, uint32_t* __restrict__ big_count) {
+    int32_t* __restrict__ map, BigCell* __restrict__ big, uint32_t* __restrict__ big_count,
+    int32_t* __restrict__ mid_pieces) {
   const int64_t ch = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (ch >= chunks) return;
   const int64_t slot = tail_slot[ch];
+  mid_pieces[ch] = 0;
   if (slot < 0) return;
   const uint32_t key = skey[slot];
   // the cell's chunks ch+1 .. last continue it; last is the first with head_end set, read
@@ -377,9 +382,13 @@ __global__ void __launch_bounds__(256) join_kernel(
     const int64_t end = cell_end(skey, slot, np, key);
     pieces = (end - 1) / kChunk - ch + 1;
   }
-  if (pieces > kSmallPieces) {
+  if (pieces > kWavePieces) {  // rare: listed for big_cell_kernel
     const uint32_t at = atomicAdd(big_count, 1u);
     big[at] = BigCell{slot, ch, pieces};
+    return;
+  }
+  if (pieces > kSmallPieces) {  // found by mid_cell_kernel's scan
+    mid_pieces[ch] = static_cast<int32_t>(pieces);
     return;
   }
   float p[kSmallPieces][8];
@@ -420,56 +429,59 @@ __device__ __forceinline__ void load_piece(const float* __restrict__ head,
 // Cells of kSmallPieces+1 .. kWavePieces pieces: one wave each, the first stage of
 // big_cell_kernel's batch (4 pieces per lane, aligned shuffle tree), whose result for a
 // batch of at most 256 pieces is the wave-0 value, so the sums are bitwise the same.
-constexpr int kWavePieces = 256;
-
 __global__ void __launch_bounds__(256) mid_cell_kernel(
-    const uint32_t* __restrict__ skey, const BigCell* __restrict__ big,
-    const uint32_t* __restrict__ big_count, const float* __restrict__ head,
-    const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
-    int32_t* __restrict__ map) {
+    const uint32_t* __restrict__ skey, const int64_t* __restrict__ tail_slot,
+    const int32_t* __restrict__ mid_pieces, int64_t chunks, const float* __restrict__ head,
+    const float* __restrict__ tail, float* __restrict__ sums, int32_t* __restrict__ map) {
   const int lane = threadIdx.x & 63;
-  const uint32_t count = *big_count;
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  for (uint32_t bcell = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); bcell < count;
-       bcell += waves) {
-    const BigCell cell = big[bcell];
-    if (cell.pieces > kWavePieces) continue;  // wave-uniform: big_cell_kernel's
-    const int64_t m = cell.pieces;
-    const int64_t q0 = 4 * lane;
-    float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (q0 < m) {
-      float w[8];
-      load_piece(head, tail, cell.chunk, q0, v);
-      if (q0 + 1 < m) {
-        load_piece(head, tail, cell.chunk, q0 + 1, w);
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // each wave scans 64 chunks at a time and takes the marked ones in turn
+  for (int64_t base = wave * 64; base < chunks; base += waves * 64) {
+    const int32_t mine = base + lane < chunks ? mid_pieces[base + lane] : 0;
+    uint64_t todo = __ballot(mine > 0);
+    while (todo) {
+      const int l = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int64_t chunk = base + l;
+      const int64_t m = __shfl(mine, l, 64);
+      const int64_t slot = tail_slot[chunk];
+      const int64_t q0 = 4 * lane;
+      float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (q0 < m) {
+        float w[8];
+        load_piece(head, tail, chunk, q0, v);
+        if (q0 + 1 < m) {
+          load_piece(head, tail, chunk, q0 + 1, w);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
-      }
-      if (q0 + 2 < m) {
-        float u[8];
-        load_piece(head, tail, cell.chunk, q0 + 2, w);
-        if (q0 + 3 < m) {
-          load_piece(head, tail, cell.chunk, q0 + 3, u);
-#pragma unroll
-          for (int c = 0; c < 8; ++c) w[c] = w[c] + u[c];
+          for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
         }
+        if (q0 + 2 < m) {
+          float u[8];
+          load_piece(head, tail, chunk, q0 + 2, w);
+          if (q0 + 3 < m) {
+            load_piece(head, tail, chunk, q0 + 3, u);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
+            for (int c = 0; c < 8; ++c) w[c] = w[c] + u[c];
+          }
+#pragma unroll
+          for (int c = 0; c < 8; ++c) v[c] = v[c] + w[c];
+        }
       }
-    }
 #pragma unroll
-    for (int sft = 1; sft < 64; sft <<= 1) {
-      const bool take = (lane % (2 * sft)) == 0 && (q0 + 4 * sft) < m;
+      for (int sft = 1; sft < 64; sft <<= 1) {
+        const bool take = (lane % (2 * sft)) == 0 && (q0 + 4 * sft) < m;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const float o = __shfl_down(v[c], sft, 64);
-        if (take) v[c] = v[c] + o;
+        for (int c = 0; c < 8; ++c) {
+          const float o = __shfl_down(v[c], sft, 64);
+          if (take) v[c] = v[c] + o;
+        }
       }
-    }
-    // lane 0 holds the 8 corner sums
-    if (lane == 0) {
-      store_cell(sums, cell.slot, v);
-      map[skey[cell.slot]] = static_cast<int32_t>(cell.slot);
+      // lane 0 holds the 8 corner sums
+      if (lane == 0) {
+        store_cell(sums, slot, v);
+        map[skey[slot]] = static_cast<int32_t>(slot);
+      }
     }
   }
 }
@@ -496,7 +508,6 @@ __global__ void __launch_bounds__(256) big_cell_kernel(
   const uint32_t count = *big_count;
   for (uint32_t bcell = blockIdx.x; bcell < count; bcell += gridDim.x) {
     const BigCell cell = big[bcell];
-    if (cell.pieces <= kWavePieces) continue;  // mid_cell_kernel's
     int depth = 0;
     for (int64_t base = 0; base < cell.pieces; base += kBigBatch) {
       const int64_t m = cell.pieces - base < kBigBatch ? cell.pieces - base : kBigBatch;
@@ -661,6 +672,7 @@ struct pm_deposit_plan {
   float* tail = nullptr;
   int64_t* tail_slot = nullptr;
   uint8_t* head_end = nullptr;  // chunks + 16 bytes (join reads 16-byte windows)
+  int32_t* mid_pieces = nullptr;
   int32_t* map = nullptr;
   BigCell* big = nullptr;
   uint32_t* big_count = nullptr;
@@ -677,7 +689,7 @@ namespace {
 void free_plan(pm_deposit_plan* p) {
   if (!p) return;
   void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->rec, p->hist, p->digit,
-                  p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->map, p->big,
+                  p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->mid_pieces, p->map, p->big,
                   p->big_count, p->acc64, p->maxbits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -728,6 +740,7 @@ pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) 
       (st = alloc(&p->tail, 8 * chunks, "hipMalloc tail")) ||
       (st = alloc(&p->tail_slot, chunks, "hipMalloc tail_slot")) ||
       (st = alloc(&p->head_end, chunks + 16, "hipMalloc head_end")) ||
+      (st = alloc(&p->mid_pieces, chunks, "hipMalloc mid_pieces")) ||
       (st = alloc(&p->map, m * m * m, "hipMalloc map")) ||
       (st = alloc(&p->big, chunks, "hipMalloc big")) ||
       (st = alloc(&p->big_count, 1, "hipMalloc big_count"))) {
@@ -900,13 +913,13 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     PM_TIMED("deposit.join", 8.0 * chunks, stream);
     join_kernel<<<pm::blocks_for(chunks), pm::kBlock, 0, stream>>>(
         skey, np, chunks, plan->tail_slot, plan->head_end, plan->head, plan->tail, plan->sums,
-        plan->cap, plan->map, plan->big, plan->big_count);
+        plan->cap, plan->map, plan->big, plan->big_count, plan->mid_pieces);
     PM_CHECK_LAUNCH("join_kernel");
   }
   {
     PM_TIMED("deposit.mid_cell", 0.0, stream);
-    mid_cell_kernel<<<1024, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
-                                              plan->tail, plan->sums, plan->cap, plan->map);
+    mid_cell_kernel<<<1024, 256, 0, stream>>>(skey, plan->tail_slot, plan->mid_pieces, chunks,
+                                              plan->head, plan->tail, plan->sums, plan->map);
     PM_CHECK_LAUNCH("mid_cell_kernel");
   }
   {
