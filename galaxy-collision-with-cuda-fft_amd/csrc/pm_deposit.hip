@@ -268,11 +268,32 @@ __device__ __forceinline__ void corner_terms(const float4 r, float t[8]) {
 // head_end (1 when the continued cell ends in this chunk).
 static_assert(kChunk == 16, "a chunk is one 16-lane row");
 
+// Node bricks of kBrick^3 nodes: bricks[b] != 0 when some occupied cell has a corner node in
+// brick b. Cell (ex, ey, ez) (extended indices, key (ex*m + ey)*m + ez, m = n+1) holds
+// corner c of node (ex-1+cx, ey-1+cy, ez-1+cz); node assembly skips unmarked bricks (all
+// their nodes are 0).
+constexpr int kBrick = 8;
+
+__device__ __forceinline__ void mark_bricks(uint32_t key, int n, uint8_t* __restrict__ bricks) {
+  const uint32_t m = static_cast<uint32_t>(n) + 1u;
+  const int ez = static_cast<int>(key % m);
+  const int ey = static_cast<int>((key / m) % m);
+  const int ex = static_cast<int>(key / (m * m));
+  const int nb = (n + kBrick - 1) / kBrick;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int i = ex - 1 + (c >> 2), j = ey - 1 + ((c >> 1) & 1), k = ez - 1 + (c & 1);
+    if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n)
+      bricks[((i / kBrick) * nb + j / kBrick) * nb + k / kBrick] = 1;
+  }
+}
+
 __global__ void __launch_bounds__(256) piece_kernel(
     const uint32_t* __restrict__ skey, const int32_t* __restrict__ sidx,
     const float4* __restrict__ rec, int64_t np, uint32_t sentinel, int64_t chunks,
     float* __restrict__ sums, int64_t cap, int32_t* __restrict__ map, float* __restrict__ head,
-    float* __restrict__ tail, int64_t* __restrict__ tail_slot, uint8_t* __restrict__ head_end) {
+    float* __restrict__ tail, int64_t* __restrict__ tail_slot, uint8_t* __restrict__ head_end,
+    int n, uint8_t* __restrict__ bricks) {
   const int64_t g = pm::xcd_block() * blockDim.x + threadIdx.x;  // sorted position
   const int lane = threadIdx.x & 63;
   const int i = lane & (kChunk - 1);
@@ -322,6 +343,8 @@ __global__ void __launch_bounds__(256) piece_kernel(
       for (int c = 0; c < 8; ++c) head[ch * 8 + c] = acc[c];
     }
   }
+  // the first particle of a cell marks the bricks of the cell's corner nodes
+  if (live && lane == ls && starts) mark_bricks(key, n, bricks);
   // head_end[ch] = 1 when the run continued from the previous chunk ends in this one
   const uint64_t closing = __ballot(end_here && !starts && ends);
   if (valid && (i == kChunk - 1 || g + 1 == np)) {
@@ -592,7 +615,8 @@ constexpr int kAsmNodes = 4;
 __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ map,
                                                        const float* __restrict__ sums,
                                                        int64_t cap, int n,
-                                                       float* __restrict__ rho, bool vec4) {
+                                                       float* __restrict__ rho, bool vec4,
+                                                       const uint8_t* __restrict__ bricks) {
   const int kq = (n + kAsmNodes - 1) / kAsmNodes;
   const int64_t items = static_cast<int64_t>(n) * n * kq;
   const int64_t t = pm::xcd_block() * blockDim.x + threadIdx.x;
@@ -602,6 +626,9 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
   const int j = static_cast<int>(ij % n);
   const int i = static_cast<int>(ij / n);
   const int64_t m = n + 1;
+  const int nb = (n + kBrick - 1) / kBrick;
+  // k0 .. k0+3 lie in one brick (kBrick is a multiple of kAsmNodes)
+  const bool marked = bricks[((i / kBrick) * nb + j / kBrick) * nb + k0 / kBrick] != 0;
   int32_t cell[2][2][kAsmNodes + 1];
   bool any = false;
 #pragma unroll
@@ -612,7 +639,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
 #pragma unroll
       for (int z = 0; z <= kAsmNodes; ++z) {
         const int ez = k0 + z;
-        cell[ax][ay][z] = ez <= n ? row[ez] : -1;
+        cell[ax][ay][z] = (marked && ez <= n) ? row[ez] : -1;
         any |= cell[ax][ay][z] >= 0;
       }
     }
@@ -646,10 +673,13 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
   }
 }
 
-__global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* __restrict__ p, int64_t count,
-                                                       int32_t value) {
+// map = -1 for every extended cell, bricks = 0.
+__global__ void __launch_bounds__(256) reset_map_kernel(int32_t* __restrict__ map, int64_t cells,
+                                                        uint8_t* __restrict__ bricks,
+                                                        int64_t nbricks) {
   const int64_t i = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = value;
+  if (i < cells) map[i] = -1;
+  if (i < nbricks) bricks[i] = 0;
 }
 
 __global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
@@ -673,6 +703,7 @@ struct pm_deposit_plan {
   int64_t* tail_slot = nullptr;
   uint8_t* head_end = nullptr;  // chunks + 16 bytes (join reads 16-byte windows)
   int32_t* mid_pieces = nullptr;
+  uint8_t* bricks = nullptr;    // ceil(n/8)^3 node-brick marks
   int32_t* map = nullptr;
   BigCell* big = nullptr;
   uint32_t* big_count = nullptr;
@@ -689,13 +720,18 @@ namespace {
 void free_plan(pm_deposit_plan* p) {
   if (!p) return;
   void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->rec, p->hist, p->digit,
-                  p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->mid_pieces, p->map, p->big,
+                  p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->mid_pieces, p->bricks, p->map, p->big,
                   p->big_count, p->acc64, p->maxbits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
     if (e) (void)hipEventDestroy(e);
   delete p;
+}
+
+int64_t nbricks_of(int n) {
+  const int64_t nb = (n + kBrick - 1) / kBrick;
+  return nb * nb * nb;
 }
 
 template <typename T>
@@ -741,6 +777,7 @@ pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) 
       (st = alloc(&p->tail_slot, chunks, "hipMalloc tail_slot")) ||
       (st = alloc(&p->head_end, chunks + 16, "hipMalloc head_end")) ||
       (st = alloc(&p->mid_pieces, chunks, "hipMalloc mid_pieces")) ||
+      (st = alloc(&p->bricks, nbricks_of(n), "hipMalloc bricks")) ||
       (st = alloc(&p->map, m * m * m, "hipMalloc map")) ||
       (st = alloc(&p->big, chunks, "hipMalloc big")) ||
       (st = alloc(&p->big_count, 1, "hipMalloc big_count"))) {
@@ -827,8 +864,9 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
 
   {
     PM_TIMED("deposit.fill_map", 4.0 * cells, stream);
-    fill_i32_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(plan->map, cells, -1);
-    PM_CHECK_LAUNCH("fill_i32_kernel");
+    reset_map_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(
+        plan->map, cells, plan->bricks, nbricks_of(n));
+    PM_CHECK_LAUNCH("reset_map_kernel");
   }
   if (np == 0) {
     if (timing) {
@@ -837,7 +875,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     {
       PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
       assemble_kernel<<<pm::blocks_for(asm_items), pm::kBlock, 0, stream>>>(
-          plan->map, plan->sums, plan->cap, n, rho, vec4);
+          plan->map, plan->sums, plan->cap, n, rho, vec4, plan->bricks);
       PM_CHECK_LAUNCH("assemble_kernel");
     }
     return PM_OK;
@@ -901,7 +939,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     PM_TIMED("deposit.piece", 24.0 * np, stream);
     piece_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
         skey, sidx, plan->rec, np, plan->sentinel, chunks, plan->sums, plan->cap, plan->map,
-        plan->head, plan->tail, plan->tail_slot, plan->head_end);
+        plan->head, plan->tail, plan->tail_slot, plan->head_end, n, plan->bricks);
     PM_CHECK_LAUNCH("piece_kernel");
   }
   {
@@ -934,7 +972,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   {
     PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
     assemble_kernel<<<pm::blocks_for(asm_items), pm::kBlock, 0, stream>>>(
-        plan->map, plan->sums, plan->cap, n, rho, vec4);
+        plan->map, plan->sums, plan->cap, n, rho, vec4, plan->bricks);
     PM_CHECK_LAUNCH("assemble_kernel");
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[4], stream));
