@@ -36,6 +36,9 @@ __device__ __forceinline__ bool cic_axis(float x, float dx, int n, int& i, float
 
 // Extended-cell key: cells with base index -1..n-1 per axis shifted by +1, C order.
 // Keys lie in [0, (n+1)^3); the value (n+1)^3 marks a particle with no in-box corner.
+// Edge (in nodes) of the node bricks the deposit marks for the gradient and assembly passes.
+constexpr int kNodeBrick = 8;
+
 __device__ __forceinline__ uint32_t cell_key(int ix, int iy, int iz, int n) {
   const uint32_t m = static_cast<uint32_t>(n) + 1u;
   return (static_cast<uint32_t>(ix + 1) * m + static_cast<uint32_t>(iy + 1)) * m +

@@ -272,7 +272,7 @@ static_assert(kChunk == 16, "a chunk is one 16-lane row");
 // brick b. Cell (ex, ey, ez) (extended indices, key (ex*m + ey)*m + ez, m = n+1) holds
 // corner c of node (ex-1+cx, ey-1+cy, ez-1+cz); node assembly skips unmarked bricks (all
 // their nodes are 0).
-constexpr int kBrick = 8;
+constexpr int kBrick = pm::kNodeBrick;
 
 __device__ __forceinline__ void mark_bricks(uint32_t key, int n, uint8_t* __restrict__ bricks) {
   const uint32_t m = static_cast<uint32_t>(n) + 1u;
@@ -704,6 +704,7 @@ struct pm_deposit_plan {
   uint8_t* head_end = nullptr;  // chunks + 16 bytes (join reads 16-byte windows)
   int32_t* mid_pieces = nullptr;
   uint8_t* bricks = nullptr;    // ceil(n/8)^3 node-brick marks
+  bool bricks_valid = false;    // marks of the last deposit (sorted mode only)
   int32_t* map = nullptr;
   BigCell* big = nullptr;
   uint32_t* big_count = nullptr;
@@ -818,6 +819,14 @@ pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode) {
   return PM_OK;
 }
 
+pm_status pm_deposit_plan_node_bricks(pm_deposit_plan* plan, const uint8_t** bricks,
+                                      int32_t* brick_edge) {
+  PM_REQUIRE(plan && bricks, "pm_deposit_plan_node_bricks: null pointer");
+  *bricks = plan->bricks_valid ? plan->bricks : nullptr;
+  if (brick_edge) *brick_edge = kBrick;
+  return PM_OK;
+}
+
 pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable) {
   PM_REQUIRE(plan, "pm_deposit_plan_set_timing: null plan");
   plan->timing = enable != 0;
@@ -848,6 +857,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     return PM_ERR_CAPACITY;
   }
   PM_REQUIRE(np == 0 || (px && py && pz && mass), "pm_deposit_cic: null particle array");
+  plan->bricks_valid = plan->mode == PM_DEPOSIT_SORTED;
   if (plan->mode == PM_DEPOSIT_FIXED) {
     PM_REQUIRE(!perm_out, "pm_deposit_cic: the particle order needs PM_DEPOSIT_SORTED mode");
     return pm::depfix::deposit(px, py, pz, mass, np, plan->n, dx, rho, keys_out, plan->acc64,

@@ -15,11 +15,9 @@ namespace {
 // wavefront reads and writes 64 consecutive floats of a z-row.
 // ---------------------------------------------------------------------------------------
 // -grad(phi) at node idx = (i, j, k): central differences inside, one-sided on the faces.
-__device__ __forceinline__ void grad_at(const float* __restrict__ phi, int64_t idx, int n,
-                                        int64_t n2, float inv_dx, float inv_2dx, float& vx,
-                                        float& vy, float& vz) {
-  const pm::Idx3 q3 = pm::idx3(idx, n);
-  const int i = q3.i, j = q3.j, k = q3.k;
+__device__ __forceinline__ void grad_at_ijk(const float* __restrict__ phi, int64_t idx, int i,
+                                            int j, int k, int n, int64_t n2, float inv_dx,
+                                            float inv_2dx, float& vx, float& vy, float& vz) {
   float a, b, s;
   // x
   if (i == 0) {
@@ -48,6 +46,13 @@ __device__ __forceinline__ void grad_at(const float* __restrict__ phi, int64_t i
     a = phi[idx + 1]; b = phi[idx - 1]; s = inv_2dx;
   }
   vz = -(a - b) * s;
+}
+
+__device__ __forceinline__ void grad_at(const float* __restrict__ phi, int64_t idx, int n,
+                                        int64_t n2, float inv_dx, float inv_2dx, float& vx,
+                                        float& vy, float& vz) {
+  const pm::Idx3 q3 = pm::idx3(idx, n);
+  grad_at_ijk(phi, idx, q3.i, q3.j, q3.k, n, n2, inv_dx, inv_2dx, vx, vy, vz);
 }
 
 __global__ void __launch_bounds__(256) grad_kernel(const float* __restrict__ phi,
@@ -113,6 +118,35 @@ __device__ __forceinline__ void cic_gather(float x, float y, float z,
           az += w * gz[node];
         }
       }
+    }
+  }
+}
+
+// grad4_kernel on the nodes of marked 8^3 bricks: one workgroup per brick (2 nodes per
+// thread, k fastest); unmarked bricks return at once and keep their old g4.
+constexpr int kGradBrick = pm::kNodeBrick;
+static_assert(kGradBrick == 8, "a workgroup of 256 threads covers 8^3 nodes, 2 per thread");
+
+__global__ void __launch_bounds__(256) grad4_bricks_kernel(const float* __restrict__ phi,
+                                                           float4* __restrict__ g4, int n,
+                                                           float inv_dx, float inv_2dx,
+                                                           const uint8_t* __restrict__ bricks) {
+  const int nb = (n + kGradBrick - 1) / kGradBrick;
+  const int64_t b = pm::xcd_block();
+  if (b >= static_cast<int64_t>(nb) * nb * nb || bricks[b] == 0) return;
+  const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
+            bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int q = threadIdx.x + 256 * h;
+    const int i = bi * kGradBrick + (q >> 6), j = bj * kGradBrick + ((q >> 3) & 7),
+              k = bk * kGradBrick + (q & 7);
+    if (i < n && j < n && k < n) {
+      const int64_t idx = i * n2 + static_cast<int64_t>(j) * n + k;
+      float vx, vy, vz;
+      grad_at_ijk(phi, idx, i, j, k, n, n2, inv_dx, inv_2dx, vx, vy, vz);
+      g4[idx] = make_float4(vx, vy, vz, 0.0f);
     }
   }
 }
@@ -375,6 +409,25 @@ pm_status pm_grad4(const float* phi, float* g4, int32_t n, float dx, hipStream_t
     grad4_kernel<<<pm::blocks_for(total), pm::kBlock, 0, stream>>>(
         phi, reinterpret_cast<float4*>(g4), n, inv_dx, inv_2dx);
     PM_CHECK_LAUNCH("grad4_kernel");
+  }
+  return PM_OK;
+}
+
+pm_status pm_grad4_bricks(const float* phi, float* g4, int32_t n, float dx,
+                         const uint8_t* bricks, hipStream_t stream) {
+  PM_REQUIRE(phi && g4 && bricks, "pm_grad4_bricks: null pointer");
+  PM_REQUIRE((reinterpret_cast<uintptr_t>(g4) & 15u) == 0,
+             "pm_grad4_bricks: g4 must be 16-byte aligned");
+  PM_REQUIRE(n >= 2 && n <= 1024, "pm_grad4_bricks: grid_size must be in [2, 1024]");
+  PM_REQUIRE(dx > 0.0f, "pm_grad4_bricks: dx must be positive");
+  const float inv_dx = 1.0f / dx;
+  const float inv_2dx = 1.0f / (2.0f * dx);
+  const int64_t nb = (n + kGradBrick - 1) / kGradBrick;
+  {
+    PM_TIMED("grad4", 0.0, stream);
+    grad4_bricks_kernel<<<static_cast<unsigned>(nb * nb * nb), 256, 0, stream>>>(
+        phi, reinterpret_cast<float4*>(g4), n, inv_dx, inv_2dx, bricks);
+    PM_CHECK_LAUNCH("grad4_bricks_kernel");
   }
   return PM_OK;
 }

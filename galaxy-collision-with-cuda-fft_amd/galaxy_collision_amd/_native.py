@@ -147,6 +147,8 @@ _SIGNATURES = {
     "pm_gather_kick": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                        _f, _f, _vp],
     "pm_grad4": [_vp, _vp, _i32, _f, _vp],
+    "pm_grad4_bricks": [_vp, _vp, _i32, _f, _vp, _vp],
+    "pm_deposit_plan_node_bricks": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i32)],
     "pm_gather_kick4": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _f,
                         _vp],
     "pm_gather_kick_drift4": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _f, _f, _f,

@@ -5,7 +5,10 @@
     kick(dt/2) + drift(dt)     (pm_kick_drift, one pass over the particles)
     deposit                    (sort-based deterministic CIC deposit)
     solve                      (the configured Poisson solver, warm-started)
-    g = -grad(phi)             (pm_grad4, interleaved (gx, gy, gz, 0) per node)
+    g = -grad(phi)             (pm_grad4_bricks: interleaved (gx, gy, gz, 0) at the nodes of
+                               the 8^3 bricks the deposit marked, i.e. every node the
+                               gather reads; pm_grad4 on every node after a fixed-mode
+                               deposit)
     gather + kick(dt/2)        (pm_gather_kick4, one pass over the particles)
 
 which is bit for bit the reference step kick -> drift -> accel_fn -> kick
@@ -19,6 +22,8 @@ fields hold the gradient of the last prime()/accel() call, not of the last step.
 """
 
 from __future__ import annotations
+
+import ctypes
 
 import torch
 
@@ -99,8 +104,13 @@ class PMChain:
         for s in range(steps):
             self.deposit()
             self.solve()
-            nat.call("pm_grad4", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx,
-                     nat.stream())
+            bricks = self.plan.node_bricks()
+            if bricks:  # only the nodes the gather below reads
+                nat.call("pm_grad4_bricks", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx,
+                         ctypes.c_void_p(bricks), nat.stream())
+            else:
+                nat.call("pm_grad4", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx,
+                         nat.stream())
             if s + 1 < steps:
                 nat.call("pm_gather_kick_drift4", *pos, nat.ptr(self.g4), *vel, p.n, self.n,
                          self.dx, h, h, dt, nat.stream())

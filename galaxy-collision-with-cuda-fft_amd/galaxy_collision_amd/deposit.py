@@ -57,6 +57,14 @@ class DepositPlan:
                  nat.ptr(perm_out, torch.int32) if perm_out is not None else None,
                  nat.stream())
 
+    def node_bricks(self) -> int | None:
+        """Device address of the last deposit's node-brick marks (8^3 node bricks holding a
+        CIC corner of some deposited particle), or None after a fixed-mode deposit."""
+        ptr, edge = ctypes.c_void_p(), ctypes.c_int32()
+        nat.call("pm_deposit_plan_node_bricks", self.handle, ctypes.byref(ptr),
+                 ctypes.byref(edge))
+        return ptr.value
+
     def set_timing(self, on: bool) -> None:
         nat.call("pm_deposit_plan_set_timing", self.handle, 1 if on else 0)
 

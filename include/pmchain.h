@@ -75,6 +75,14 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* pos_x, const float*
 /* Timing hook: per-phase device times (ms) of the last pm_deposit_cic on this plan when
  * pm_deposit_plan_set_timing(plan, 1) was set before it; phases = key, sort, reduce, assemble. */
 pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable);
+
+/* Node bricks of the last deposit with this plan: bricks[b] != 0 when some deposited
+ * particle has a CIC corner node in the brick_edge^3 node brick b = (bi*nb + bj)*nb + bk,
+ * nb = ceil(grid_size / brick_edge). Device memory owned by the plan, valid until its next
+ * deposit; *bricks is NULL after a PM_DEPOSIT_FIXED deposit. For pm_grad4_bricks. No
+ * reference counterpart (the reference's gradient covers every node, deposit.py:204-207). */
+pm_status pm_deposit_plan_node_bricks(pm_deposit_plan* plan, const uint8_t** bricks,
+                                      int32_t* brick_edge);
 pm_status pm_deposit_last_timing(pm_deposit_plan* plan, float out_ms[4]);
 
 /* ------------------------------------------------------------------------------------
@@ -115,6 +123,11 @@ pm_status pm_kick_drift(float* pos_x, float* pos_y, float* pos_z, float* vel_x, 
  * pm_grad4 writes g4[node] = (gx, gy, gz, 0) (float32, 16-byte aligned n^3*4 floats), and
  * pm_gather_kick4 reads it; the results are bitwise those of pm_grad + pm_gather_kick. */
 pm_status pm_grad4(const float* phi, float* g4, int32_t grid_size, float dx, hipStream_t stream);
+/* pm_grad4 on the nodes of the bricks marked in `bricks` (pm_deposit_plan_node_bricks):
+ * every node the gather of those deposited particles reads; g4 elsewhere is left as it was.
+ * Bitwise pm_grad4 on those nodes. */
+pm_status pm_grad4_bricks(const float* phi, float* g4, int32_t grid_size, float dx,
+                          const uint8_t* bricks, hipStream_t stream);
 pm_status pm_gather_kick4(const float* pos_x, const float* pos_y, const float* pos_z,
                           const float* g4, float* acc_x, float* acc_y, float* acc_z,
                           float* vel_x, float* vel_y, float* vel_z, int64_t n_particles,

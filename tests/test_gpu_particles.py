@@ -251,3 +251,59 @@ def test_kdk_step_one_eval_and_exact_inline_equality(device):
     ef = energy(pos_k.astype(np.float64), vel_k.astype(np.float64))
     assert abs((ef - e0) / e0) < 1e-3
     assert np.linalg.norm(pos_k[0] - pos0[0]) / r < 0.02
+
+
+@pytest.mark.parametrize("n,npart", [(32, 3000), (37, 5000), (64, 40_000)])
+def test_grad4_on_deposit_bricks(device, n, npart):
+    """pm_grad4_bricks writes pm_grad4's values (bitwise) on every node of the 8^3 bricks the
+    last deposit marked, leaves the other nodes alone, and the marked bricks cover every CIC
+    corner node of every deposited particle (so the gather reads only written nodes)."""
+    announce(f"grad4 on deposit bricks n={n}")
+    import ctypes
+
+    from galaxy_collision_amd import _native as nat
+    from galaxy_collision_amd.data import ParticleState
+    from galaxy_collision_amd.deposit import DepositPlan
+
+    dx = 0.5
+    rng = np.random.default_rng(n)
+    # a compact blob, a uniform slab over the lower 40% of x (some particles outside the
+    # box), so that some bricks stay unmarked
+    slab = rng.uniform(-1.0, n * dx + 1.0, size=(npart - npart // 2, 3))
+    slab[:, 0] = rng.uniform(-1.0, 0.4 * n * dx, size=len(slab))
+    pos = np.concatenate([rng.normal(n * dx * 0.3, 1.0, size=(npart // 2, 3)), slab])
+    pos = pos.astype(F32)
+    parts = ParticleState(npart)
+    parts.load(pos, None, np.ones(npart, F32))
+    rho = torch.zeros((n, n, n), device=device)
+    plan = DepositPlan(n, npart)
+    plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, npart, dx, rho)
+    ptr = plan.node_bricks()
+    assert ptr
+    nb = -(-n // 8)
+    torch.cuda.synchronize()
+    marks = np.zeros(nb ** 3, dtype=np.uint8)
+    hip = ctypes.CDLL("libamdhip64.so")
+    assert hip.hipMemcpy(ctypes.c_void_p(marks.ctypes.data), ctypes.c_void_p(ptr),
+                         ctypes.c_size_t(marks.nbytes), 2) == 0  # device to host
+    phi = torch.from_numpy(rng.normal(size=(n, n, n)).astype(F32)).to(device)
+    full = torch.empty((n, n, n, 4), device=device)
+    part = torch.full((n, n, n, 4), 12345.0, device=device)
+    nat.call("pm_grad4", nat.ptr(phi), nat.ptr(full), n, dx, nat.stream())
+    nat.call("pm_grad4_bricks", nat.ptr(phi), nat.ptr(part), n, dx, ctypes.c_void_p(ptr),
+             nat.stream())
+    full_np, part_np = _np(full), _np(part)
+    m = marks.reshape(nb, nb, nb).repeat(8, 0).repeat(8, 1).repeat(8, 2)[:n, :n, :n] != 0
+    assert m.any() and not m.all()
+    assert_bitwise(part_np[m], full_np[m])
+    assert np.all(part_np[~m] == 12345.0)
+    # every in-box CIC corner node of every particle lies in a marked brick
+    base = np.floor(pos.astype(np.float64) / dx).astype(np.int64)
+    for c in range(8):
+        node = base + np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1])
+        nd = node[np.all((node >= 0) & (node < n), axis=1)]
+        assert np.all(m[nd[:, 0], nd[:, 1], nd[:, 2]]), c
+    plan.set_mode("fixed")
+    plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, npart, dx, rho)
+    assert plan.node_bricks() is None
+    plan.close()
