@@ -57,20 +57,6 @@ __global__ void __launch_bounds__(256) key_kernel(
   if (keys_out) keys_out[p] = key;
 }
 
-// Lanes holding the same digit are grouped with one ballot per digit bit and only the
-// lowest such lane adds the group's count, so nearly sorted keys do not serialize on one
-// LDS counter.
-__device__ __forceinline__ void tile_digit_count(uint32_t d, bool valid, int bits,
-                                                 uint64_t lanes_below, uint32_t* cnt) {
-  uint64_t match = __ballot(valid);
-  for (int bb = 0; bb < bits; ++bb) {
-    const bool bit = (d >> bb) & 1u;
-    const uint64_t bm = __ballot(bit);
-    match &= bit ? bm : ~bm;
-  }
-  if (valid && (match & lanes_below) == 0)
-    atomicAdd(&cnt[d], static_cast<uint32_t>(__popcll(match)));
-}
 
 // ---------------------------------------------------------------------------------------
 // 2. stable LSD radix sort (reduce-then-scan; no inter-workgroup waiting)
@@ -82,20 +68,30 @@ __global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restr
   __shared__ uint32_t cnt[1 << kMaxDigitBits];
   const int buckets = 1 << bits;
   const uint32_t mask = static_cast<uint32_t>(buckets - 1);
-  const int lane = threadIdx.x & 63;
   for (int d = threadIdx.x; d < buckets; d += blockDim.x) cnt[d] = 0;
   __syncthreads();
-  const uint64_t lanes_below = (1ull << lane) - 1ull;
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
-  for (int e = 0; e < kTile / 256; ++e) {
-    const int64_t p = base + e * 256 + threadIdx.x;
-    const bool valid = p < np;
-    const uint32_t d = valid ? (keys[p] >> shift) & mask : 0u;
-    tile_digit_count(d, valid, bits, lanes_below, cnt);
+  // tile number in XCD-aware order: neighbouring tiles, whose counts share cache lines of
+  // the digit-major histogram, run on one XCD
+  const int64_t tile = pm::xcd_block();
+  const int64_t base = tile * kTile;
+  // counts do not depend on the order of the adds: plain LDS atomics
+  if (base + kTile <= np) {
+    const uint4* q4 = reinterpret_cast<const uint4*>(keys + base);
+#pragma unroll
+    for (int e = 0; e < kTile / 1024; ++e) {
+      const uint4 q = q4[e * 256 + threadIdx.x];
+      atomicAdd(&cnt[(q.x >> shift) & mask], 1u);
+      atomicAdd(&cnt[(q.y >> shift) & mask], 1u);
+      atomicAdd(&cnt[(q.z >> shift) & mask], 1u);
+      atomicAdd(&cnt[(q.w >> shift) & mask], 1u);
+    }
+  } else {
+    for (int64_t p = base + threadIdx.x; p < np; p += blockDim.x)
+      atomicAdd(&cnt[(keys[p] >> shift) & mask], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < buckets; d += blockDim.x)
-    hist[static_cast<int64_t>(d) * tiles + blockIdx.x] = cnt[d];
+    hist[static_cast<int64_t>(d) * tiles + tile] = cnt[d];
 }
 
 // Block-wide exclusive scan of one value per thread (256 threads = 4 waves).
@@ -178,15 +174,16 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(
   const int wave = threadIdx.x >> 6;
   const int buckets = 1 << bits;
   const uint32_t mask = static_cast<uint32_t>(buckets - 1);
+  const int64_t tile = pm::xcd_block();  // as in radix_hist_kernel
   for (int d = threadIdx.x; d < kSortWaves * (1 << kMaxDigitBits); d += blockDim.x)
     (&wave_cnt[0][0])[d] = 0;
   for (int d = threadIdx.x; d < buckets; d += blockDim.x)
-    tile_base[d] = digit_start[d] + hist[static_cast<int64_t>(d) * tiles + blockIdx.x];
+    tile_base[d] = digit_start[d] + hist[static_cast<int64_t>(d) * tiles + tile];
   __syncthreads();
 
   const uint64_t lanes_below = (1ull << lane) - 1ull;
   const int64_t wave_start =
-      static_cast<int64_t>(blockIdx.x) * kTile + static_cast<int64_t>(wave) * 64 * kSortItems;
+      tile * kTile + static_cast<int64_t>(wave) * 64 * kSortItems;
   uint32_t k[kSortItems];
   int32_t v[kSortItems];
   uint32_t rank[kSortItems];
