@@ -2,9 +2,10 @@
 
 Workloads (seeded synthetic inputs, built on the host, resident in HBM before timing):
   collision10m  two-galaxy collision, 10M particles, 256^3 grid, box 40, dt 0.01,
-                mixed-precision multigrid to an estimated phi error of 2e-6 (measured error
-                against the exact discrete solution ~1e-7, reported with every run), warm-
-                started by linear extrapolation of phi in time (default)
+                mixed-precision multigrid, V(1,1) cycles, to an estimated phi error of 2e-6
+                (measured error against the exact discrete solution ~1e-7, the float32
+                floor, reported with every run), warm-started by linear extrapolation of
+                phi in time (default)
   collision2m   the same recipe with 2 x 1,048,576 particles on a 128^3 grid, box 40
   collision100m the full-scale case: 2 x 50M particles (2:1 disk mass ratio), 256^3
   plummer       131,072-particle Plummer sphere, 64^3 grid over a box of 20a, dt 0.01
@@ -39,12 +40,12 @@ REFERENCE_STEPS_PER_S = 24.8  # reference docs/performance.md:17 (RTX 3070, 256^
 
 WORKLOADS = {
     "collision10m": dict(kind="two_galaxy", n_per=5_000_000, grid=256, box=40.0, dt=0.01,
-                         solver="multigrid_mp", warm="extrapolate", tol=2e-6),
+                         solver="multigrid_mp", warm="extrapolate", tol=2e-6, sweeps=(1, 1)),
     "collision2m": dict(kind="two_galaxy", n_per=1_048_576, grid=128, box=40.0, dt=0.01,
                         solver="multigrid"),
     "collision100m": dict(kind="two_galaxy", n_per=50_000_000, grid=256, box=40.0, dt=0.01,
                           solver="multigrid_mp", warm="extrapolate", masses=(1.0, 0.5),
-                          tol=2e-6),
+                          tol=2e-6, sweeps=(1, 1)),
     "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
 }
 
@@ -237,6 +238,8 @@ def main() -> int:
         kw["tol"] = args.tol
     elif "tol" in w and solver_name == w["solver"]:
         kw["tol"] = w["tol"]
+    if "sweeps" in w and solver_name == w["solver"]:
+        kw["pre_sweeps"], kw["post_sweeps"] = w["sweeps"]
     warm = args.warm or (w.get("warm", "previous") if solver_name == w["solver"] else "previous")
     if solver_name in ("multigrid", "multigrid_mp") and warm != "previous":
         kw["warm"] = warm
@@ -342,6 +345,7 @@ def main() -> int:
             "dt": w["dt"],
             "solver": solver_name,
             "solver_tol": getattr(solver, "tol", None),
+            "vcycle": (f"V({solver.nu1},{solver.nu2})" if hasattr(solver, "nu1") else None),
             "warm_start": getattr(solver, "warm", None),
             "precision": "f32 particles/V-cycles; f64 residual, coarsest solve, reductions"
             if solver_name == "multigrid_mp" else "f32",
