@@ -66,8 +66,11 @@ def build_ic(w: dict, seed: int):
 
 
 # rocprofv3 kernel symbol of a timed kernel name (the finest-grid dispatch is the timed one)
-_PROFILE_KERNELS = {"mg.rb_pass": "rb_pass_kernel", "gather_kick": "gather_kick_kernel",
-                    "mg.residual": "residual_kernel", "mg.prolong": "prolong_add_kernel",
+_PROFILE_KERNELS = {"mg.rb_pass": "rb_pass_kernel", "gather_kick": "gather_kick4_kernel",
+                    "gather_kick_drift": "gather_kick_drift4_kernel",
+                    "mg.rbs": "rbs4_kernel<false>", "mg.residual": "residual_kernel",
+                    "mg.prolong": "prolong_add_kernel", "deposit.piece": "piece_kernel",
+                    "deposit.key": "key_kernel", "deposit.radix_hist": "radix_hist_kernel",
                     "deposit.radix_scatter": "radix_scatter_kernel"}
 
 

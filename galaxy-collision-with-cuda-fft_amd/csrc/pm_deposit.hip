@@ -922,7 +922,8 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
       PM_CHECK_LAUNCH("radix_digit_start_kernel");
     }
     {
-      PM_TIMED("deposit.radix_scatter", 16.0 * np, stream);
+      // read key (+ index after the first pass), write key + index
+      PM_TIMED("deposit.radix_scatter", (s == 0 ? 12.0 : 16.0) * np, stream);
       radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src],
                                                        s == 0 ? nullptr : plan->vals[src],
                                                        plan->keys[1 - src], plan->vals[1 - src],
