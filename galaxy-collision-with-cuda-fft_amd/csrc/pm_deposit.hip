@@ -74,16 +74,42 @@ __global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restr
   // the digit-major histogram, run on one XCD
   const int64_t tile = pm::xcd_block();
   const int64_t base = tile * kTile;
-  // counts do not depend on the order of the adds: plain LDS atomics
+  // Counts do not depend on the order of the adds (plain LDS atomics). Cell-ordered input
+  // is highly clustered, so each wave first collapses runs of equal digits over its 256
+  // consecutive keys (lane l holds keys 4l .. 4l+3) and adds each run's length once.
   if (base + kTile <= np) {
     const uint4* q4 = reinterpret_cast<const uint4*>(keys + base);
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int e = 0; e < kTile / 1024; ++e) {
       const uint4 q = q4[e * 256 + threadIdx.x];
-      atomicAdd(&cnt[(q.x >> shift) & mask], 1u);
-      atomicAdd(&cnt[(q.y >> shift) & mask], 1u);
-      atomicAdd(&cnt[(q.z >> shift) & mask], 1u);
-      atomicAdd(&cnt[(q.w >> shift) & mask], 1u);
+      const uint32_t d[4] = {(q.x >> shift) & mask, (q.y >> shift) & mask,
+                             (q.z >> shift) & mask, (q.w >> shift) & mask};
+      const uint32_t prev = __shfl_up(d[3], 1, 64);   // digit before this lane's first key
+      const uint32_t next = __shfl_down(d[0], 1, 64); // digit after this lane's last key
+      bool start[4], end[4];
+      start[0] = lane == 0 || prev != d[0];
+      end[3] = lane == 63 || next != d[3];
+#pragma unroll
+      for (int j = 1; j < 4; ++j) start[j] = d[j] != d[j - 1];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) end[j] = d[j] != d[j + 1];
+      int last = -1;  // last run start in this lane
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (start[j]) last = j;
+      // start of the run that enters this lane: the last run start of the nearest earlier
+      // lane that has one
+      const uint64_t has = __ballot(last >= 0);
+      const uint64_t below = has & ((1ull << lane) - 1ull);
+      const int src = below ? 63 - __builtin_clzll(below) : 0;
+      const int src_last = __shfl(last, src, 64);
+      int run_start = 4 * src + src_last;  // valid when the lane's first key continues a run
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (start[j]) run_start = 4 * lane + j;
+        if (end[j]) atomicAdd(&cnt[d[j]], static_cast<uint32_t>(4 * lane + j - run_start + 1));
+      }
     }
   } else {
     for (int64_t p = base + threadIdx.x; p < np; p += blockDim.x)
@@ -191,21 +217,50 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(
   for (int r = 0; r < kSortItems; ++r) {
     const int64_t p = wave_start + r * 64 + lane;
     const bool valid = p < np;
-    const uint32_t key = valid ? keys_in[p] : 0u;
-    k[r] = key;
+    k[r] = valid ? keys_in[p] : 0u;
     v[r] = valid ? (vals_in ? vals_in[p] : static_cast<int32_t>(p)) : 0;
-    const uint32_t d = (key >> shift) & mask;
-    uint64_t match = __ballot(valid);
-    for (int b = 0; b < bits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bm = __ballot(bit);
-      match &= bit ? bm : ~bm;
-    }
-    const uint32_t before = static_cast<uint32_t>(__popcll(match & lanes_below));
-    const uint32_t base = wave_cnt[wave][d];
-    rank[r] = base + before;
-    if (valid && before == 0) wave_cnt[wave][d] = base + static_cast<uint32_t>(__popcll(match));
   }
+  // Rounds whose valid lanes all hold one digit (common for cell-ordered input) need no
+  // per-bit ballots, and a run of such rounds keeps that digit's wave count in a register
+  // (c_digit, c_count: wave-uniform) instead of an LDS write-then-read per round; the
+  // count goes back to LDS before any other digit's count is touched.
+  uint32_t c_digit = 0xFFFFFFFFu, c_count = 0;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const bool valid = wave_start + r * 64 + lane < np;
+    const uint32_t d = (k[r] >> shift) & mask;
+    const uint64_t vmask = __ballot(valid);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const bool uniform = __ballot(valid && d == d0) == vmask;
+    const uint32_t nvalid = static_cast<uint32_t>(__popcll(vmask));
+    if (uniform) {
+      if (d0 != c_digit) {
+        if (c_digit != 0xFFFFFFFFu) wave_cnt[wave][c_digit] = c_count;
+        c_digit = d0;
+        c_count = wave_cnt[wave][d0];
+      }
+      rank[r] = c_count + static_cast<uint32_t>(__popcll(vmask & lanes_below));
+      c_count += nvalid;
+    } else {
+      if (c_digit != 0xFFFFFFFFu) {
+        wave_cnt[wave][c_digit] = c_count;
+        c_digit = 0xFFFFFFFFu;
+      }
+      // lanes with the same digit: one ballot per digit bit
+      uint64_t match = vmask;
+      for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bm = __ballot(bit);
+        match &= bit ? bm : ~bm;
+      }
+      const uint32_t before = static_cast<uint32_t>(__popcll(match & lanes_below));
+      const uint32_t base = wave_cnt[wave][d];
+      rank[r] = base + before;
+      if (valid && before == 0)
+        wave_cnt[wave][d] = base + static_cast<uint32_t>(__popcll(match));
+    }
+  }
+  if (c_digit != 0xFFFFFFFFu) wave_cnt[wave][c_digit] = c_count;
   __syncthreads();
   // exclusive prefix of the per-wave digit counts, in wave order
   for (int d = threadIdx.x; d < buckets; d += blockDim.x) {
