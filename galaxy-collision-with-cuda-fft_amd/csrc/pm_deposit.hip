@@ -47,13 +47,15 @@ __global__ void __launch_bounds__(256) key_kernel(
     uint32_t* __restrict__ keys_out) {
   const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
+  // all four loads first (the && below would otherwise order them)
+  const float x = px[p], y = py[p], z = pz[p], m = mass[p];
   int ix = 0, iy = 0, iz = 0;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  const bool in = pm::cic_axis(px[p], dx, n, ix, fx) && pm::cic_axis(py[p], dx, n, iy, fy) &&
-                  pm::cic_axis(pz[p], dx, n, iz, fz);
+  const bool in = pm::cic_axis(x, dx, n, ix, fx) && pm::cic_axis(y, dx, n, iy, fy) &&
+                  pm::cic_axis(z, dx, n, iz, fz);
   const uint32_t key = in ? pm::cell_key(ix, iy, iz, n) : sentinel;
   keys[p] = key;
-  rec[p] = make_float4(fx, fy, fz, mass[p] * inv_vol);
+  rec[p] = make_float4(fx, fy, fz, m * inv_vol);
   if (keys_out) keys_out[p] = key;
 }
 

@@ -15,37 +15,24 @@ namespace {
 // wavefront reads and writes 64 consecutive floats of a z-row.
 // ---------------------------------------------------------------------------------------
 // -grad(phi) at node idx = (i, j, k): central differences inside, one-sided on the faces.
+// Central differences inside, one-sided on the faces; the neighbour offsets and the scale
+// are selected without branches so the six loads issue together (same a, b, s as the
+// branch-per-face form).
 __device__ __forceinline__ void grad_at_ijk(const float* __restrict__ phi, int64_t idx, int i,
                                             int j, int k, int n, int64_t n2, float inv_dx,
                                             float inv_2dx, float& vx, float& vy, float& vz) {
-  float a, b, s;
-  // x
-  if (i == 0) {
-    a = phi[idx + n2]; b = phi[idx]; s = inv_dx;
-  } else if (i == n - 1) {
-    a = phi[idx]; b = phi[idx - n2]; s = inv_dx;
-  } else {
-    a = phi[idx + n2]; b = phi[idx - n2]; s = inv_2dx;
-  }
-  vx = -(a - b) * s;
-  // y
-  if (j == 0) {
-    a = phi[idx + n]; b = phi[idx]; s = inv_dx;
-  } else if (j == n - 1) {
-    a = phi[idx]; b = phi[idx - n]; s = inv_dx;
-  } else {
-    a = phi[idx + n]; b = phi[idx - n]; s = inv_2dx;
-  }
-  vy = -(a - b) * s;
-  // z
-  if (k == 0) {
-    a = phi[idx + 1]; b = phi[idx]; s = inv_dx;
-  } else if (k == n - 1) {
-    a = phi[idx]; b = phi[idx - 1]; s = inv_dx;
-  } else {
-    a = phi[idx + 1]; b = phi[idx - 1]; s = inv_2dx;
-  }
-  vz = -(a - b) * s;
+  const int64_t px = i == n - 1 ? 0 : n2, mx = i == 0 ? 0 : n2;
+  const int64_t py = j == n - 1 ? 0 : n, my = j == 0 ? 0 : n;
+  const int64_t pz = k == n - 1 ? 0 : 1, mz = k == 0 ? 0 : 1;
+  const float ax = phi[idx + px], bx = phi[idx - mx];
+  const float ay = phi[idx + py], by = phi[idx - my];
+  const float az = phi[idx + pz], bz = phi[idx - mz];
+  const float sx = (i == 0 || i == n - 1) ? inv_dx : inv_2dx;
+  const float sy = (j == 0 || j == n - 1) ? inv_dx : inv_2dx;
+  const float sz = (k == 0 || k == n - 1) ? inv_dx : inv_2dx;
+  vx = -(ax - bx) * sx;
+  vy = -(ay - by) * sy;
+  vz = -(az - bz) * sz;
 }
 
 __device__ __forceinline__ void grad_at(const float* __restrict__ phi, int64_t idx, int n,
@@ -84,6 +71,9 @@ __global__ void __launch_bounds__(256) grad4_kernel(const float* __restrict__ ph
 }
 
 // CIC interpolation of the three node fields at one particle (reference deposit.py:100-129).
+// The 8 corner loads are issued together (an out-of-box corner reads node 0 and is not
+// added), so a particle waits for one round of memory latency instead of eight; the terms
+// are added in (cx, cy, cz) order exactly as before.
 __device__ __forceinline__ void cic_gather(float x, float y, float z,
                                            const float* __restrict__ gx,
                                            const float* __restrict__ gy,
@@ -98,27 +88,27 @@ __device__ __forceinline__ void cic_gather(float x, float y, float z,
       !pm::cic_axis(z, dx, n, iz, fz))
     return;
   const int64_t n2 = static_cast<int64_t>(n) * n;
+  bool ok[8];
+  float vx[8], vy[8], vz[8];
 #pragma unroll
-  for (int cx = 0; cx < 2; ++cx) {
-    const int jx = ix + cx;
-    const float wx = cx ? fx : 1.0f - fx;
+  for (int c = 0; c < 8; ++c) {
+    const int jx = ix + (c >> 2), jy = iy + ((c >> 1) & 1), jz = iz + (c & 1);
+    ok[c] = jx >= 0 && jx < n && jy >= 0 && jy < n && jz >= 0 && jz < n;
+    const int64_t node = ok[c] ? jx * n2 + static_cast<int64_t>(jy) * n + jz : 0;
+    vx[c] = gx[node];
+    vy[c] = gy[node];
+    vz[c] = gz[node];
+  }
 #pragma unroll
-    for (int cy = 0; cy < 2; ++cy) {
-      const int jy = iy + cy;
-      const float wy = cy ? fy : 1.0f - fy;
-#pragma unroll
-      for (int cz = 0; cz < 2; ++cz) {
-        const int jz = iz + cz;
-        const float wz = cz ? fz : 1.0f - fz;
-        if (jx >= 0 && jx < n && jy >= 0 && jy < n && jz >= 0 && jz < n) {
-          const float w = wx * wy * wz;
-          const int64_t node = jx * n2 + static_cast<int64_t>(jy) * n + jz;
-          ax += w * gx[node];
-          ay += w * gy[node];
-          az += w * gz[node];
-        }
-      }
-    }
+  for (int c = 0; c < 8; ++c) {
+    const float wx = (c >> 2) ? fx : 1.0f - fx;
+    const float wy = ((c >> 1) & 1) ? fy : 1.0f - fy;
+    const float wz = (c & 1) ? fz : 1.0f - fz;
+    const float w = wx * wy * wz;
+    const float tx = ax + w * vx[c], ty = ay + w * vy[c], tz = az + w * vz[c];
+    ax = ok[c] ? tx : ax;
+    ay = ok[c] ? ty : ay;
+    az = ok[c] ? tz : az;
   }
 }
 
@@ -165,27 +155,24 @@ __device__ __forceinline__ void cic_gather4(float x, float y, float z,
       !pm::cic_axis(z, dx, n, iz, fz))
     return;
   const int64_t n2 = static_cast<int64_t>(n) * n;
+  bool ok[8];
+  float4 g[8];
 #pragma unroll
-  for (int cx = 0; cx < 2; ++cx) {
-    const int jx = ix + cx;
-    const float wx = cx ? fx : 1.0f - fx;
+  for (int c = 0; c < 8; ++c) {
+    const int jx = ix + (c >> 2), jy = iy + ((c >> 1) & 1), jz = iz + (c & 1);
+    ok[c] = jx >= 0 && jx < n && jy >= 0 && jy < n && jz >= 0 && jz < n;
+    g[c] = g4[ok[c] ? jx * n2 + static_cast<int64_t>(jy) * n + jz : 0];
+  }
 #pragma unroll
-    for (int cy = 0; cy < 2; ++cy) {
-      const int jy = iy + cy;
-      const float wy = cy ? fy : 1.0f - fy;
-#pragma unroll
-      for (int cz = 0; cz < 2; ++cz) {
-        const int jz = iz + cz;
-        const float wz = cz ? fz : 1.0f - fz;
-        if (jx >= 0 && jx < n && jy >= 0 && jy < n && jz >= 0 && jz < n) {
-          const float w = wx * wy * wz;
-          const float4 g = g4[jx * n2 + static_cast<int64_t>(jy) * n + jz];
-          ax += w * g.x;
-          ay += w * g.y;
-          az += w * g.z;
-        }
-      }
-    }
+  for (int c = 0; c < 8; ++c) {
+    const float wx = (c >> 2) ? fx : 1.0f - fx;
+    const float wy = ((c >> 1) & 1) ? fy : 1.0f - fy;
+    const float wz = (c & 1) ? fz : 1.0f - fz;
+    const float w = wx * wy * wz;
+    const float tx = ax + w * g[c].x, ty = ay + w * g[c].y, tz = az + w * g[c].z;
+    ax = ok[c] ? tx : ax;
+    ay = ok[c] ? ty : ay;
+    az = ok[c] ? tz : az;
   }
 }
 
