@@ -255,6 +255,15 @@ __device__ __forceinline__ float prolong_value(const float* __restrict__ phi_c, 
   const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
   const int oi = i & 1, oj = j & 1, ok = k & 1;
   const int64_t nc2 = static_cast<int64_t>(nc) * nc;
+  // the 8 loads issue together; a zero-weight term (possibly one past the coarse grid's
+  // edge, so its index is clamped) is loaded but not added
+  float cv[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int a = ci + (c >> 2), b = cj + ((c >> 1) & 1), d = ck + (c & 1);
+    cv[c] = phi_c[(a < nc ? a : nc - 1) * nc2 + static_cast<int64_t>(b < nc ? b : nc - 1) * nc +
+                  (d < nc ? d : nc - 1)];
+  }
   float val = 0.0f;
 #pragma unroll
   for (int di = 0; di < 2; ++di) {
@@ -266,8 +275,8 @@ __device__ __forceinline__ float prolong_value(const float* __restrict__ phi_c, 
       for (int dk = 0; dk < 2; ++dk) {
         const float wk = ok ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
         const float w = wi * wj * wk;
-        if (w > 0.0f)
-          val += w * phi_c[(ci + di) * nc2 + static_cast<int64_t>(cj + dj) * nc + (ck + dk)];
+        const float t = val + w * cv[di * 4 + dj * 2 + dk];
+        val = w > 0.0f ? t : val;
       }
     }
   }
