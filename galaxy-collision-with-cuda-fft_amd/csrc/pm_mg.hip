@@ -1162,6 +1162,10 @@ struct pm_mg {
   float* r32s = nullptr;
   int hn = 0;
   bool use_split = true;
+  // the 4-wide split reductions (float64 defect, fused update) run one quad per thread:
+  // quad_blocks workgroups with their own partial sums
+  unsigned quad_blocks = 0;
+  double* partials_q = nullptr;
   bool split_active() const { return e0s != nullptr && use_split; }
 };
 
@@ -1316,11 +1320,15 @@ pm_status vcycle(pm_mg* mg, size_t l, hipStream_t s) {
 }
 
 // Workgroups of the split path's fused update (the partials the fold has to sum).
-unsigned split_update_blocks(const pm_mg* mg, const float* phi) {
+unsigned split_update_blocks(const pm_mg* mg, const float* phi, double** partials) {
   const int n = mg->lv[0].n;
   const int64_t plane = static_cast<int64_t>(n) * n * mg->hn;
-  const bool quad4 = n % 8 == 0 && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0;
-  return pm::red_blocks(quad4 ? plane / 4 : plane);
+  if (mg->quad_blocks && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0) {
+    *partials = mg->partials_q;
+    return mg->quad_blocks;
+  }
+  *partials = mg->partials;
+  return pm::red_blocks(plane);
 }
 
 // The finest level of the MIXED correction in split layout: e0s = V(r32s) from a zero
@@ -1385,12 +1393,12 @@ pm_status vcycle_split0(pm_mg* mg, float* phi, hipStream_t s) {
   }
   {
     PM_TIMED("mg.black_update", 12.0 * nodes, s);
+    double* partials;
+    const unsigned blocks = split_update_blocks(mg, phi, &partials);
     if (quad && phi_aligned)
-      black_update4_kernel<<<pm::red_blocks(quads), pm::kBlock, 0, s>>>(e, r, phi, n, hn, L.h2,
-                                                                         mg->partials);
+      black_update4_kernel<<<blocks, pm::kBlock, 0, s>>>(e, r, phi, n, hn, L.h2, partials);
     else
-      black_update_kernel<<<pm::red_blocks(plane), pm::kBlock, 0, s>>>(e, r, phi, n, hn,
-                                                                        L.h2, mg->partials);
+      black_update_kernel<<<blocks, pm::kBlock, 0, s>>>(e, r, phi, n, hn, L.h2, partials);
     PM_CHECK_LAUNCH("black_update_kernel");
   }
   return PM_OK;
@@ -1428,13 +1436,14 @@ pm_status plain_residual_norm(pm_mg* mg, const float* phi, double* out, hipStrea
 pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split = false) {
   const Level& L = mg->lv[0];
   // 8 nodes per thread on split grids with n % 8 == 0 and 16-byte aligned phi and f
-  const bool quad = split && L.n % 8 == 0 && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0;
-  const unsigned b = pm::red_blocks(quad ? L.nodes() / 8 : L.nodes());
+  const bool quad = split && mg->quad_blocks && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0;
+  const unsigned b = quad ? mg->quad_blocks : pm::red_blocks(L.nodes());
+  double* partials = quad ? mg->partials_q : mg->partials;
   {
     PM_TIMED("mg.residual64", kRes64Bytes * L.nodes(), s);
     if (quad)
       residual64_s4_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, mg->hn, 1.0 / (L.h * L.h),
-                                                     mg->r32s, mg->partials);
+                                                     mg->r32s, partials);
     else if (split)
       residual64_s_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, mg->hn, 1.0 / (L.h * L.h),
                                                     mg->r32s, mg->partials);
@@ -1443,7 +1452,7 @@ pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split 
                                                   mg->r32, mg->partials);
     PM_CHECK_LAUNCH("residual64_kernel");
   }
-  pm::fold_partials_kernel<1><<<1, pm::kBlock, 0, s>>>(mg->partials, static_cast<int>(b),
+  pm::fold_partials_kernel<1><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(b),
                                                         mg->red + 2);
   PM_CHECK_LAUNCH("fold_partials_kernel<1>");
   return PM_OK;
@@ -1460,8 +1469,10 @@ pm_status enqueue_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
   if (kind == kCyclePlain) return vcycle(mg, 0, s);
   if (mg->split_active()) {
     if ((st = vcycle_split0(mg, phi, s))) return st;
-    pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(
-        mg->partials, static_cast<int>(split_update_blocks(mg, phi)), mg->red);
+    double* partials;
+    const unsigned blocks = split_update_blocks(mg, phi, &partials);
+    pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(blocks),
+                                                          mg->red);
     PM_CHECK_LAUNCH("fold_partials_kernel<2>");
     return PM_OK;
   }
@@ -1713,6 +1724,15 @@ pm_status pm_mg_create(const pm_mg_desc* desc, pm_mg** out) {
       if (ze != hipSuccess) {
         destroy(mg);
         return pm::hip_fail("hipMemset (multigrid split buffers)", ze);
+      }
+      if (n % 8 == 0) {
+        const int64_t quads = static_cast<int64_t>(n) * n * mg->hn / 4;
+        mg->quad_blocks = static_cast<unsigned>((quads + pm::kBlock - 1) / pm::kBlock);
+        if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->partials_q),
+                         sizeof(double) * pm::kRedMaxQ * mg->quad_blocks))) {
+          destroy(mg);
+          return st;
+        }
       }
     }
   }
