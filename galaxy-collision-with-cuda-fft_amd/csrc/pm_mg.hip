@@ -1003,7 +1003,7 @@ __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restr
     *reinterpret_cast<float4*>(r32s + plane + q.t0) = make_float4(a1[0], a1[1], a1[2], a1[3]);
   }
   pm::block_sum<1>(v, s_red);
-  if (threadIdx.x == 0) partials[blockIdx.x * pm::kRedMaxQ] = v[0];
+  if (threadIdx.x == 0) partials[blockIdx.x] = v[0];  // one column (pm::fold_columns_kernel)
 }
 
 // black_update_kernel, 4 positions of both colours (8 natural nodes) per thread.
@@ -1065,9 +1065,9 @@ __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restr
     }
   }
   pm::block_sum<2>(v, s_red);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x * pm::kRedMaxQ] = v[0];
-    partials[blockIdx.x * pm::kRedMaxQ + 1] = v[1];
+  if (threadIdx.x == 0) {  // two columns of gridDim.x partials (pm::fold_columns_kernel)
+    partials[blockIdx.x] = v[0];
+    partials[gridDim.x + blockIdx.x] = v[1];
   }
 }
 
@@ -1452,8 +1452,12 @@ pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split 
                                                   mg->r32, mg->partials);
     PM_CHECK_LAUNCH("residual64_kernel");
   }
-  pm::fold_partials_kernel<1><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(b),
-                                                        mg->red + 2);
+  if (quad)
+    pm::fold_columns_kernel<1><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(b),
+                                                         mg->red + 2);
+  else
+    pm::fold_partials_kernel<1><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(b),
+                                                          mg->red + 2);
   PM_CHECK_LAUNCH("fold_partials_kernel<1>");
   return PM_OK;
 }
@@ -1471,8 +1475,12 @@ pm_status enqueue_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
     if ((st = vcycle_split0(mg, phi, s))) return st;
     double* partials;
     const unsigned blocks = split_update_blocks(mg, phi, &partials);
-    pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(blocks),
-                                                          mg->red);
+    if (partials == mg->partials_q)  // black_update4_kernel's columns
+      pm::fold_columns_kernel<2><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(blocks),
+                                                           mg->red);
+    else
+      pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(blocks),
+                                                            mg->red);
     PM_CHECK_LAUNCH("fold_partials_kernel<2>");
     return PM_OK;
   }

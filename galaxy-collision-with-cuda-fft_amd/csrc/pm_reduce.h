@@ -68,6 +68,27 @@ __global__ void __launch_bounds__(256) fold_partials_kernel(const double* __rest
   }
 }
 
+// fold_partials_kernel for partials stored quantity-major, partials[q * blocks + b] (one
+// contiguous column per quantity): the same per-thread order and tree, so the same result.
+template <int Q>
+__global__ void __launch_bounds__(256) fold_columns_kernel(const double* __restrict__ partials,
+                                                           int blocks,
+                                                           double* __restrict__ out) {
+  __shared__ double s_red[4 * Q];
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = v[q] + partials[static_cast<int64_t>(q) * blocks + b];
+  }
+  block_sum<Q>(v, s_red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) out[q] = v[q];
+  }
+}
+
 // Scratch for reductions issued without a handle: kRedBlocks*kRedMaxQ partial doubles and
 // kRedMaxQ result doubles on the device, plus a pinned host mirror of the result.
 struct RedScratch {
@@ -86,8 +107,7 @@ pm_status launch_sumsq_f64(const double* x, int64_t count, double* partials, dou
 // Ten raw moments of rho -> sums_dev[0..9]; if face_params is non-null also writes the
 // eleven float32 face parameters [G, M, cx, cy, cz, sxx, syy, szz, sxy, sxz, syz]
 // computed from them exactly as reference multigrid.py:364-389 does on the host. With f,
-// the same pass writes f = factor * rho and, with f_sumsq_dev, the sum of f^2 there
-// (bitwise launch_sumsq_f32 of f).
+// the same pass writes f = factor * rho and, with f_sumsq_dev, the sum of f^2 there.
 pm_status launch_moments(const float* rho, int n, float dx, double grav, double* partials,
                          double* sums_dev, float* face_params, hipStream_t stream,
                          float* f = nullptr, float factor = 0.0f, double* f_sumsq_dev = nullptr);
