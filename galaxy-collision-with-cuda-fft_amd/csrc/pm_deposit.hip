@@ -174,19 +174,6 @@ __global__ void __launch_bounds__(256) radix_scan_kernel(uint32_t* __restrict__ 
   if (threadIdx.x == 0) digit_tot[blockIdx.x] = carry;
 }
 
-// digit_tot -> exclusive prefix (digit_start), one workgroup; thread t owns digits 2t, 2t+1.
-__global__ void __launch_bounds__(256) radix_digit_start_kernel(uint32_t* __restrict__ digit_tot,
-                                                                int buckets) {
-  static_assert(2 * 256 >= (1 << kMaxDigitBits), "two digits per thread");
-  __shared__ uint32_t s_wave[4];
-  const int d0 = 2 * threadIdx.x;
-  const uint32_t v0 = d0 < buckets ? digit_tot[d0] : 0u;
-  const uint32_t v1 = d0 + 1 < buckets ? digit_tot[d0 + 1] : 0u;
-  uint32_t total;
-  const uint32_t ex = block_exclusive_scan(v0 + v1, total, s_wave);
-  if (d0 < buckets) digit_tot[d0] = ex;
-  if (d0 + 1 < buckets) digit_tot[d0 + 1] = ex + v0;
-}
 
 // Stable scatter of one tile. Lane l of wave w, round r holds tile position
 // w*64*kSortItems + r*64 + l, so (wave, round, lane) order is input order. A null vals_in
@@ -194,7 +181,7 @@ __global__ void __launch_bounds__(256) radix_digit_start_kernel(uint32_t* __rest
 __global__ void __launch_bounds__(256) radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_start, int64_t np,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_tot, int64_t np,
     int tiles, int shift, int bits) {
   __shared__ uint32_t wave_cnt[kSortWaves][1 << kMaxDigitBits];
   __shared__ uint32_t tile_base[1 << kMaxDigitBits];
@@ -205,8 +192,19 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(
   const int64_t tile = pm::xcd_block();  // as in radix_hist_kernel
   for (int d = threadIdx.x; d < kSortWaves * (1 << kMaxDigitBits); d += blockDim.x)
     (&wave_cnt[0][0])[d] = 0;
-  for (int d = threadIdx.x; d < buckets; d += blockDim.x)
-    tile_base[d] = digit_start[d] + hist[static_cast<int64_t>(d) * tiles + tile];
+  {
+    // digit starts: exclusive scan of the digit totals (2 digits per thread)
+    static_assert(2 * 256 >= (1 << kMaxDigitBits), "two digits per thread");
+    __shared__ uint32_t s_wave[4];
+    const int d0 = 2 * threadIdx.x;
+    const uint32_t t0 = d0 < buckets ? digit_tot[d0] : 0u;
+    const uint32_t t1 = d0 + 1 < buckets ? digit_tot[d0 + 1] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(t0 + t1, total, s_wave);
+    if (d0 < buckets) tile_base[d0] = ex + hist[static_cast<int64_t>(d0) * tiles + tile];
+    if (d0 + 1 < buckets)
+      tile_base[d0 + 1] = ex + t0 + hist[static_cast<int64_t>(d0 + 1) * tiles + tile];
+  }
   __syncthreads();
 
   const uint64_t lanes_below = (1ull << lane) - 1ull;
@@ -727,16 +725,17 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
   }
 }
 
-// map = -1 for every extended cell, bricks = 0.
+// map = -1 for every extended cell, bricks = 0, no big cells listed yet.
 __global__ void __launch_bounds__(256) reset_map_kernel(int32_t* __restrict__ map, int64_t cells,
                                                         uint8_t* __restrict__ bricks,
-                                                        int64_t nbricks) {
+                                                        int64_t nbricks,
+                                                        uint32_t* __restrict__ big_count) {
   const int64_t i = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (i < cells) map[i] = -1;
   if (i < nbricks) bricks[i] = 0;
+  if (i == 0) *big_count = 0u;
 }
 
-__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
 
 }  // namespace
 
@@ -929,7 +928,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   {
     PM_TIMED("deposit.fill_map", 4.0 * cells, stream);
     reset_map_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(
-        plan->map, cells, plan->bricks, nbricks_of(n));
+        plan->map, cells, plan->bricks, nbricks_of(n), plan->big_count);
     PM_CHECK_LAUNCH("reset_map_kernel");
   }
   if (np == 0) {
@@ -974,11 +973,6 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
       PM_CHECK_LAUNCH("radix_scan_kernel");
     }
     {
-      PM_TIMED("deposit.radix_digit", 8.0 * buckets, stream);
-      radix_digit_start_kernel<<<1, 256, 0, stream>>>(plan->digit, buckets);
-      PM_CHECK_LAUNCH("radix_digit_start_kernel");
-    }
-    {
       // read key (+ index after the first pass), write key + index
       PM_TIMED("deposit.radix_scatter", (s == 0 ? 12.0 : 16.0) * np, stream);
       radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src],
@@ -1006,11 +1000,6 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
         skey, sidx, plan->rec, np, plan->sentinel, chunks, plan->sums, plan->cap, plan->map,
         plan->head, plan->tail, plan->tail_slot, plan->head_end, n, plan->bricks);
     PM_CHECK_LAUNCH("piece_kernel");
-  }
-  {
-    PM_TIMED("deposit.zero_count", 4.0, stream);
-    zero_u32_kernel<<<1, 1, 0, stream>>>(plan->big_count);
-    PM_CHECK_LAUNCH("zero_u32_kernel");
   }
   {
     PM_TIMED("deposit.join", 8.0 * chunks, stream);

@@ -88,8 +88,8 @@ __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ 
 
 // Centre of mass and second moments about it (parallel axis), as reference
 // multigrid.py:378-389 does on the host in float64, rounded to float32 face parameters.
-__global__ void moments_finalize_kernel(const double* __restrict__ s, double grav,
-                                        float* __restrict__ params) {
+__device__ __forceinline__ void moments_finalize(const double* s, double grav,
+                                                 float* __restrict__ params) {
   const double total = s[0];
   double cx = 0, cy = 0, cz = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
   double mass = 0.0;
@@ -116,6 +116,31 @@ __global__ void moments_finalize_kernel(const double* __restrict__ s, double gra
   params[8] = static_cast<float>(sxy);
   params[9] = static_cast<float>(sxz);
   params[10] = static_cast<float>(syz);
+}
+
+// The moments' stage-2 fold (fold_partials_kernel's order and tree, Q = 10 or 11 with the
+// sum of f^2 in slot 10) with the face parameters computed from its result on thread 0.
+template <int Q>
+__global__ void __launch_bounds__(256) fold_moments_kernel(const double* __restrict__ partials,
+                                                           int blocks, double grav,
+                                                           double* __restrict__ sums,
+                                                           double* __restrict__ f_sumsq,
+                                                           float* __restrict__ params) {
+  __shared__ double s_red[4 * Q];
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = v[q] + partials[b * pm::kRedMaxQ + q];
+  }
+  pm::block_sum<Q>(v, s_red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) sums[q] = v[q];
+    if (Q > 10 && f_sumsq) f_sumsq[0] = v[Q - 1];
+    if (params) moments_finalize(v, grav, params);
+  }
 }
 
 // [M, KE, Px, Py, Pz, Lx, Ly, Lz, Smx, Smy, Smz] (reference diagnostics_device.py:41-74).
@@ -335,17 +360,13 @@ pm_status launch_moments(const float* rho, int n, float dx, double grav, double*
                                                     nullptr, 0.0f);
   }
   PM_CHECK_LAUNCH("moments_kernel");
-  fold_partials_kernel<10><<<1, kBlock, 0, stream>>>(partials, static_cast<int>(b), sums_dev);
-  PM_CHECK_LAUNCH("fold_partials_kernel<10>");
-  if (f && f_sumsq_dev) {
-    fold_partials_kernel<1><<<1, kBlock, 0, stream>>>(partials + 10, static_cast<int>(b),
-                                                       f_sumsq_dev);
-    PM_CHECK_LAUNCH("fold_partials_kernel<1>");
-  }
-  if (face_params) {
-    moments_finalize_kernel<<<1, 1, 0, stream>>>(sums_dev, grav, face_params);
-    PM_CHECK_LAUNCH("moments_finalize_kernel");
-  }
+  if (f)
+    fold_moments_kernel<11><<<1, kBlock, 0, stream>>>(partials, static_cast<int>(b), grav,
+                                                      sums_dev, f_sumsq_dev, face_params);
+  else
+    fold_moments_kernel<10><<<1, kBlock, 0, stream>>>(partials, static_cast<int>(b), grav,
+                                                      sums_dev, nullptr, face_params);
+  PM_CHECK_LAUNCH("fold_moments_kernel");
   return PM_OK;
 }
 
