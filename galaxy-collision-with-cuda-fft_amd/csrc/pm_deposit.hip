@@ -725,6 +725,30 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
   }
 }
 
+// Sparse reset, one workgroup per node brick: the map holds entries only for the cells
+// occupied at the last deposit, and each of those has a corner node in a brick marked then,
+// so clearing the cells (ex, ey, ez) in [8b, 8b+8]^3 of every marked brick b restores
+// map = -1 everywhere; the brick's mark is cleared for this deposit.
+__global__ void __launch_bounds__(256) reset_marked_kernel(int32_t* __restrict__ map, int n,
+                                                           uint8_t* __restrict__ bricks,
+                                                           uint32_t* __restrict__ big_count) {
+  const int nb = (n + kBrick - 1) / kBrick;
+  const int64_t b = pm::xcd_block();
+  if (b == 0 && threadIdx.x == 0) *big_count = 0u;
+  if (bricks[b] == 0) return;
+  __syncthreads();  // every thread has read the mark before it is cleared
+  if (threadIdx.x == 0) bricks[b] = 0;
+  const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
+            bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+  const int64_t m = n + 1;
+  constexpr int kSpan = kBrick + 1;  // cells ex in [8bi, 8bi+8]
+  for (int q = threadIdx.x; q < kSpan * kSpan * kSpan; q += blockDim.x) {
+    const int ex = bi * kBrick + q / (kSpan * kSpan), ey = bj * kBrick + (q / kSpan) % kSpan,
+              ez = bk * kBrick + q % kSpan;
+    if (ex <= n && ey <= n && ez <= n) map[(ex * m + ey) * m + ez] = -1;
+  }
+}
+
 // map = -1 for every extended cell, bricks = 0, no big cells listed yet.
 __global__ void __launch_bounds__(256) reset_map_kernel(int32_t* __restrict__ map, int64_t cells,
                                                         uint8_t* __restrict__ bricks,
@@ -758,6 +782,7 @@ struct pm_deposit_plan {
   int32_t* mid_pieces = nullptr;
   uint8_t* bricks = nullptr;    // ceil(n/8)^3 node-brick marks
   bool bricks_valid = false;    // marks of the last deposit (sorted mode only)
+  bool map_clean = false;       // map is -1 outside the cells around the marked bricks
   int32_t* map = nullptr;
   BigCell* big = nullptr;
   uint32_t* big_count = nullptr;
@@ -927,9 +952,15 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
 
   {
     PM_TIMED("deposit.fill_map", 4.0 * cells, stream);
-    reset_map_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(
-        plan->map, cells, plan->bricks, nbricks_of(n), plan->big_count);
+    if (plan->map_clean) {
+      reset_marked_kernel<<<static_cast<unsigned>(nbricks_of(n)), pm::kBlock, 0, stream>>>(
+          plan->map, n, plan->bricks, plan->big_count);
+    } else {
+      reset_map_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(
+          plan->map, cells, plan->bricks, nbricks_of(n), plan->big_count);
+    }
     PM_CHECK_LAUNCH("reset_map_kernel");
+    plan->map_clean = true;  // holds again once this deposit's marks are written
   }
   if (np == 0) {
     if (timing) {
