@@ -1005,12 +1005,10 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     }
     {
       // read key (+ index after the first pass), write key + index
-      PM_TIMED("deposit.radix_scatter", (s == 0 ? 12.0 : 16.0) * np, stream);
-      radix_scatter_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src],
-                                                       s == 0 ? nullptr : plan->vals[src],
-                                                       plan->keys[1 - src], plan->vals[1 - src],
-                                                       plan->hist, plan->digit, np, tiles, shift,
-                                                       bits);
+      PM_LAUNCH_TIMED("deposit.radix_scatter", (s == 0 ? 12.0 : 16.0) * np,
+                      radix_scatter_kernel, tiles, 256, 0, stream, plan->keys[src],
+                      s == 0 ? nullptr : plan->vals[src], plan->keys[1 - src],
+                      plan->vals[1 - src], plan->hist, plan->digit, np, tiles, shift, bits);
       PM_CHECK_LAUNCH("radix_scatter_kernel");
     }
     src = 1 - src;

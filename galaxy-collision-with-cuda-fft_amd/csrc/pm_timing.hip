@@ -17,7 +17,7 @@ struct Record {
   hipEvent_t stop = nullptr;
 };
 
-std::atomic<bool> g_enabled{false};
+std::atomic<int> g_mode{0};  // 0 off, 1 every timer, 2 dispatch-recorded timers only
 std::mutex g_mu;
 std::vector<Record> g_records;      // pending event pairs
 std::vector<hipEvent_t> g_free;     // recycled events
@@ -36,7 +36,9 @@ hipEvent_t take_event() {
     return e;
   }
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  // timing-only events: no system-scope fence (and its cache writeback and invalidate,
+  // which would also slow the kernel that follows) when they are recorded
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
@@ -63,7 +65,9 @@ pm_status drain() {
 
 namespace pm {
 
-bool timing_enabled() { return g_enabled.load(std::memory_order_relaxed); }
+bool timing_enabled() { return g_mode.load(std::memory_order_relaxed) == 1; }
+
+bool launch_timing_enabled() { return g_mode.load(std::memory_order_relaxed) != 0; }
 
 KernelTimer::KernelTimer(const char* name, double bytes, hipStream_t stream) {
   if (!timing_enabled()) return;
@@ -89,13 +93,29 @@ KernelTimer::~KernelTimer() {
   if (g_records.size() > 200000) (void)drain();
 }
 
+LaunchTimer::LaunchTimer(const char* name, double bytes) {
+  if (!launch_timing_enabled()) return;
+  std::lock_guard<std::mutex> lock(g_mu);
+  Record r;
+  r.name = name;
+  r.bytes = bytes;
+  r.start = take_event();
+  r.stop = take_event();
+  if (!r.start || !r.stop) return;
+  start_ = r.start;
+  stop_ = r.stop;
+  g_records.push_back(r);
+  if (g_records.size() > 200000) (void)drain();
+}
+
 }  // namespace pm
 
 extern "C" {
 
 pm_status pm_timing_enable(int32_t on) {
   std::lock_guard<std::mutex> lock(g_mu);
-  g_enabled.store(on != 0);
+  PM_REQUIRE(on >= 0 && on <= 2, "pm_timing_enable: mode must be 0, 1 or 2");
+  g_mode.store(on);
   return PM_OK;
 }
 

@@ -113,8 +113,9 @@ class KernelTime(ctypes.Structure):
     ]
 
 
-def timing_enable(on: bool) -> None:
-    call("pm_timing_enable", 1 if on else 0)
+def timing_enable(on: bool | int) -> None:
+    """False/0 off, True/1 every instrumented launch, 2 dispatch-recorded launches only."""
+    call("pm_timing_enable", int(on))
 
 
 def timing_reset() -> None:

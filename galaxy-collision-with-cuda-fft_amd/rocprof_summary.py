@@ -3,7 +3,11 @@
     python rocprof_summary.py trace  <bench_results.db>                 > summary.txt
     python rocprof_summary.py pmc    <fetch.db> <write.db> [--json out] > traffic.txt
 
-``trace``: per kernel and grid size, dispatch count and average/min/max duration.
+``trace``: per kernel and grid size, dispatch count and average/min/max duration. With
+``--kernel NAME --last N`` it adds, for the last N dispatches of that kernel (the timed
+steps), the mean duration, the mean gap since the previous dispatch ended, and their sum: the
+span of a HIP event pair that the kernel's own launch records (hipExtLaunchKernelGGL), whose
+start marker completes when the previous kernel does.
 ``pmc``: per kernel and grid size, average FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
 scaled by the same counters of a plain device copy of known size found in the same run
 (copy_f32_kernel moving a 256 MiB buffer, bench.py --copy-probe): factor = expected bytes /
@@ -30,7 +34,23 @@ def short(name: str) -> str:
     return n.strip()
 
 
-def trace(db_path: str) -> str:
+def _timed_dispatches(db, kernel: str, last: int) -> str:
+    rows = db.execute("select name, start, end from kernels order by start").fetchall()
+    durs, gaps = [], []
+    for (_, _, e0), (n1, s1, e1) in zip(rows, rows[1:]):
+        if short(n1) == kernel:
+            durs.append(e1 - s1)
+            gaps.append(max(s1 - e0, 0))
+    durs, gaps = durs[-last:], gaps[-last:]
+    if not durs:
+        return f"# {kernel}: no dispatches"
+    d = sum(durs) / len(durs) / 1e3
+    g = sum(gaps) / len(gaps) / 1e3
+    return (f"# {kernel}, last {len(durs)} dispatches: duration {d:.2f} us, gap since the "
+            f"previous kernel {g:.2f} us, duration + gap {d + g:.2f} us")
+
+
+def trace(db_path: str, kernel: str | None = None, last: int = 0) -> str:
     db = sqlite3.connect(db_path)
     rows = db.execute("select name, grid_x, grid_y, grid_z, duration from kernels").fetchall()
     agg = defaultdict(list)
@@ -43,6 +63,8 @@ def trace(db_path: str) -> str:
         out.append(f"{name[:44]:44s} {grid:10d} {len(d):6d} {sum(d) / len(d) / 1e3:9.2f} "
                    f"{min(d) / 1e3:9.2f} {max(d) / 1e3:9.2f} {sum(d) / 1e6:9.3f} "
                    f"{100.0 * sum(d) / total:6.2f}")
+    if kernel:
+        out.insert(0, _timed_dispatches(db, kernel, last or 1 << 30))
     return "\n".join(out)
 
 
@@ -97,12 +119,15 @@ def main() -> int:
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("trace")
     t.add_argument("db")
+    t.add_argument("--kernel", default=None)
+    t.add_argument("--last", type=int, default=0)
     p = sub.add_parser("pmc")
     p.add_argument("fetch_db")
     p.add_argument("write_db")
     p.add_argument("--json", default=None)
     a = ap.parse_args()
-    print(trace(a.db) if a.cmd == "trace" else pmc(a.fetch_db, a.write_db, a.json))
+    print(trace(a.db, a.kernel, a.last) if a.cmd == "trace"
+          else pmc(a.fetch_db, a.write_db, a.json))
     return 0
 
 

@@ -53,3 +53,6 @@ def test_error_path_without_device_work():
     with pytest.raises(nat.PMChainError, match="grid_size"):
         out = ctypes.c_void_p()
         nat.call("pm_deposit_plan_create", 0, 10, ctypes.byref(out))
+    with pytest.raises(nat.PMChainError, match="mode must be"):
+        nat.call("pm_timing_enable", 3)
+    nat.call("pm_timing_enable", 0)

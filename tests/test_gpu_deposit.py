@@ -126,6 +126,35 @@ def test_deterministic_repeat(device):
     assert_bitwise(a, b)
 
 
+def test_dispatch_recorded_timing(device):
+    """Timing mode 2: only the radix scatter's dispatch-recorded pairs are kept (one per sort
+    pass, with the pass's key/index bytes), and the deposit is bitwise the untimed one."""
+    announce("deposit dispatch-recorded timing")
+    from galaxy_collision_amd import _native as nat
+
+    rng = np.random.default_rng(12)
+    npart = 300_000
+    pos = rng.normal(32, 4, size=(npart, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2.0, npart).astype(F32)
+    ref, _, _ = _run(device, pos, mass, 64, 1.0)
+    nat.timing_reset()
+    nat.timing_enable(2)
+    try:
+        got, _, _ = _run(device, pos, mass, 64, 1.0)
+        torch.cuda.synchronize()
+    finally:
+        nat.timing_enable(False)
+    rep = nat.timing_report()
+    nat.timing_reset()
+    assert_bitwise(got, ref)
+    assert set(rep) == {"deposit.radix_scatter"}, sorted(rep)
+    r = rep["deposit.radix_scatter"]
+    passes = r["launches"]
+    assert passes >= 1 and r["total_ms"] > 0.0
+    # 12 bytes per particle in the first pass (key read, key and index written), 16 after
+    assert r["bytes"] == pytest.approx((12.0 + 16.0 * (passes - 1)) * npart)
+
+
 # ---------------------------------------------------------------- fixed-point mode
 
 def _run_fixed(device, pos, mass, n, dx):
