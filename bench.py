@@ -40,12 +40,13 @@ REFERENCE_STEPS_PER_S = 24.8  # reference docs/performance.md:17 (RTX 3070, 256^
 
 WORKLOADS = {
     "collision10m": dict(kind="two_galaxy", n_per=5_000_000, grid=256, box=40.0, dt=0.01,
-                         solver="multigrid_mp", warm="extrapolate", tol=2e-6, sweeps=(1, 1)),
+                         solver="multigrid_mp", warm="extrapolate", tol=2e-6, sweeps=(1, 1),
+                         reorder_every=4),
     "collision2m": dict(kind="two_galaxy", n_per=1_048_576, grid=128, box=40.0, dt=0.01,
                         solver="multigrid"),
     "collision100m": dict(kind="two_galaxy", n_per=50_000_000, grid=256, box=40.0, dt=0.01,
                           solver="multigrid_mp", warm="extrapolate", masses=(1.0, 0.5),
-                          tol=2e-6, sweeps=(1, 1)),
+                          tol=2e-6, sweeps=(1, 1), reorder_every=4),
     "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
 }
 
@@ -185,6 +186,9 @@ def main() -> int:
     ap.add_argument("--warm", default=None, choices=["previous", "extrapolate"],
                     help="multigrid warm start across steps (default: the workload's)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--reorder-every", type=int, default=None,
+                    help="steps between moves of the particles into cell order (0: never; "
+                         "default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of timed CPU steps")
     ap.add_argument("--cpu-threads", type=int, default=None)
@@ -247,7 +251,8 @@ def main() -> int:
     if solver_name in ("multigrid", "multigrid_mp") and warm != "previous":
         kw["warm"] = warm
     solver = make_solver(solver_name, w["grid"], **kw)
-    chain = PMChain(parts, w["grid"], dx, solver)
+    reorder = args.reorder_every if args.reorder_every is not None else w.get("reorder_every", 0)
+    chain = PMChain(parts, w["grid"], dx, solver, reorder_every=reorder)
     chain.prime()
     chain.run(args.warmup, w["dt"])
     torch.cuda.synchronize()
@@ -372,7 +377,8 @@ def main() -> int:
             "precision": "f32 particles/V-cycles; f64 residual, coarsest solve, reductions"
             if solver_name == "multigrid_mp" else "f32",
             "parallelism": "single GPU",
-            "particle_order": "sorted by initial cell",
+            "particle_order": "sorted by initial cell" + (
+                f"; moved into cell order every {reorder} steps" if reorder else ""),
         },
         "particle_steps_per_s": value * icr.n,
         "mean_vcycles_per_step": float(np.mean(cycles)) if cycles else None,

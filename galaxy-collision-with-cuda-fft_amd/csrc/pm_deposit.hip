@@ -38,24 +38,22 @@ struct BigCell {
 };
 
 // ---------------------------------------------------------------------------------------
-// 1. keys and per-particle records
+// 1. keys (skipped when the caller supplies them, e.g. from the drift pass)
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) key_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    const float* __restrict__ mass, int64_t np, int n, float dx, float inv_vol,
-    uint32_t sentinel, uint32_t* __restrict__ keys, float4* __restrict__ rec,
+    int64_t np, int n, float dx, uint32_t sentinel, uint32_t* __restrict__ keys,
     uint32_t* __restrict__ keys_out) {
   const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  // all four loads first (the && below would otherwise order them)
-  const float x = px[p], y = py[p], z = pz[p], m = mass[p];
+  // all three loads first (the && below would otherwise order them)
+  const float x = px[p], y = py[p], z = pz[p];
   int ix = 0, iy = 0, iz = 0;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  float fx, fy, fz;
   const bool in = pm::cic_axis(x, dx, n, ix, fx) && pm::cic_axis(y, dx, n, iy, fy) &&
                   pm::cic_axis(z, dx, n, iz, fz);
   const uint32_t key = in ? pm::cell_key(ix, iy, iz, n) : sentinel;
   keys[p] = key;
-  rec[p] = make_float4(fx, fy, fz, m * inv_vol);
   if (keys_out) keys_out[p] = key;
 }
 
@@ -298,6 +296,7 @@ __device__ __forceinline__ void store_cell(float* __restrict__ sums, int64_t slo
   dst[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
 
+// Corner terms of one particle: r = (fx, fy, fz, mass / dx^3).
 __device__ __forceinline__ void corner_terms(const float4 r, float t[8]) {
   const float wx0 = 1.0f - r.x, wy0 = 1.0f - r.y, wz0 = 1.0f - r.z;
 #pragma unroll
@@ -309,16 +308,22 @@ __device__ __forceinline__ void corner_terms(const float4 r, float t[8]) {
   }
 }
 
-// Piece sums with one lane per sorted position: a chunk of kChunk = 16 positions is one
-// 16-lane row of a wave, so keys, indices and the gathered records load coalesced and
-// nothing is staged in LDS. A run of equal keys inside a chunk is summed strictly in order
-// by a 15-step serial scan along the row (step s: lane s adds its terms to lane s-1's
-// running sum), which is the sequential piece sum of the contract above, bit for bit. The
-// lane where a run ends writes it: complete cells straight to sums/map, a chunk's
-// continuation piece to head[chunk], its open last piece to tail[chunk]; the chunk's last
-// position writes tail_slot (sorted position of the open cell's first particle, or -1) and
-// head_end (1 when the continued cell ends in this chunk).
-static_assert(kChunk == 16, "a chunk is one 16-lane row");
+// Piece sums in two phases per workgroup of kPieceChunks chunks (kPiecePos positions).
+// Phase 1, one lane per sorted position (coalesced keys and indices, gathers in near-input
+// order): the particle's 8 corner terms go to LDS, chunk by chunk. Phase 2, eight lanes per
+// chunk, lane c summing corner c: the chunk's runs of equal keys are summed strictly in
+// order (acc = t0; acc = acc + t1; ...), the sequential piece sum of the contract above, bit
+// for bit. Where a run ends its lanes write it: complete cells straight to sums/map, a
+// chunk's continuation piece to head[chunk], its open last piece to tail[chunk]; lane 0 of
+// the chunk writes tail_slot (sorted position of the open cell's first particle, or -1) and
+// head_end (1 when the continued cell ends in this chunk), and the 8 lanes of a cell's first
+// run mark the bricks of its 8 corner nodes.
+constexpr int kPieceChunks = 32;                     // 8 lanes per chunk, 256 lanes
+constexpr int kPiecePos = kPieceChunks * kChunk;     // 512 positions per workgroup
+constexpr int kTermStride = 8 * kChunk + 8;          // floats per chunk in LDS: lanes
+                                                     // (chunk q, corner c) hit bank 8q + c
+static_assert(kPieceChunks * 8 == 256, "eight lanes per chunk in a 256-lane workgroup");
+static_assert(kPiecePos % 256 == 0, "phase 1 covers whole positions per lane");
 
 // Node bricks of kBrick^3 nodes: bricks[b] != 0 when some occupied cell has a corner node in
 // brick b. Cell (ex, ey, ez) (extended indices, key (ex*m + ey)*m + ez, m = n+1) holds
@@ -326,82 +331,130 @@ static_assert(kChunk == 16, "a chunk is one 16-lane row");
 // their nodes are 0).
 constexpr int kBrick = pm::kNodeBrick;
 
-__device__ __forceinline__ void mark_bricks(uint32_t key, int n, uint8_t* __restrict__ bricks) {
+__device__ __forceinline__ void mark_corner_brick(uint32_t key, int c, int n,
+                                                  uint8_t* __restrict__ bricks) {
   const uint32_t m = static_cast<uint32_t>(n) + 1u;
   const int ez = static_cast<int>(key % m);
   const int ey = static_cast<int>((key / m) % m);
   const int ex = static_cast<int>(key / (m * m));
   const int nb = (n + kBrick - 1) / kBrick;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int i = ex - 1 + (c >> 2), j = ey - 1 + ((c >> 1) & 1), k = ez - 1 + (c & 1);
-    if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n)
-      bricks[((i / kBrick) * nb + j / kBrick) * nb + k / kBrick] = 1;
-  }
+  const int i = ex - 1 + (c >> 2), j = ey - 1 + ((c >> 1) & 1), k = ez - 1 + (c & 1);
+  if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n)
+    bricks[((i / kBrick) * nb + j / kBrick) * nb + k / kBrick] = 1;
 }
 
 __global__ void __launch_bounds__(256) piece_kernel(
     const uint32_t* __restrict__ skey, const int32_t* __restrict__ sidx,
-    const float4* __restrict__ rec, int64_t np, uint32_t sentinel, int64_t chunks,
-    float* __restrict__ sums, int64_t cap, int32_t* __restrict__ map, float* __restrict__ head,
-    float* __restrict__ tail, int64_t* __restrict__ tail_slot, uint8_t* __restrict__ head_end,
-    int n, uint8_t* __restrict__ bricks) {
-  const int64_t g = pm::xcd_block() * blockDim.x + threadIdx.x;  // sorted position
-  const int lane = threadIdx.x & 63;
-  const int i = lane & (kChunk - 1);
-  const bool valid = g < np;
-  const uint32_t key = valid ? skey[g] : 0xFFFFFFFFu;
-  const uint32_t prevk = (valid && g > 0) ? skey[g - 1] : 0xFFFFFFFFu;
-  const uint32_t nextk = g + 1 < np ? skey[g + 1] : 0xFFFFFFFFu;
-  const bool live = valid && key < sentinel;
-  float t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (live) corner_terms(rec[sidx[g]], t);
-  // this position continues the run of the previous position of its chunk
-  const bool cont = i > 0 && live && prevk == key;
-  float acc[8];
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float* __restrict__ mass, float dx, float inv_vol, int64_t np, uint32_t sentinel,
+    int64_t chunks, float* __restrict__ sums, int32_t* __restrict__ map,
+    float* __restrict__ head, float* __restrict__ tail, int64_t* __restrict__ tail_slot,
+    uint8_t* __restrict__ head_end, int n, uint8_t* __restrict__ bricks) {
+  __shared__ float4 terms4[kPieceChunks * kTermStride / 4];
+  __shared__ uint32_t keys_l[kPiecePos + 2];  // keys of positions base-1 .. base+kPiecePos
+  float* terms = reinterpret_cast<float*>(terms4);
+  const int64_t blk = pm::xcd_block();
+  const int64_t base = blk * kPiecePos;
+  const int tid = threadIdx.x;
+
+  // phase 1: corner terms, one lane per position (positions >= np get key ~0u, terms 0).
+  // All loads of a lane's positions are issued before any is used: keys and indices, then
+  // the particle gathers (any index is valid, so they need not wait for the key test).
+  constexpr int kPer = kPiecePos / 256;
+  uint32_t key[kPer];
+  int32_t q[kPer];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) acc[c] = t[c];
-#pragma unroll
-  for (int st = 1; st < kChunk; ++st) {
-    const bool take = i == st && cont;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      // DPP row_shr:1: lane i of each 16-lane row reads lane i-1 of the same row
-      const float up = __int_as_float(
-          __builtin_amdgcn_update_dpp(0, __float_as_int(acc[c]), 0x111, 0xF, 0xF, true));
-      if (take) acc[c] = up + t[c];
-    }
+  for (int r = 0; r < kPer; ++r) {
+    const int64_t g = base + r * 256 + tid;
+    key[r] = g < np ? skey[g] : 0xFFFFFFFFu;
+    q[r] = g < np ? sidx[g] : 0;
   }
-  // first lane of this position's run (within the chunk)
-  const uint64_t run_starts = __ballot(live && !cont);
-  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const uint64_t below = run_starts & upto;
-  const int ls = below ? 63 - __builtin_clzll(below) : lane;
-  const uint32_t prev_at_start = __shfl(prevk, ls, 64);
-  const int64_t g_start = g - (lane - ls);
-  const bool end_here = live && (i == kChunk - 1 || g + 1 >= np || nextk != key);
-  const bool starts = g_start == 0 || prev_at_start != key;
-  const bool ends = g + 1 >= np || nextk != key;
-  const int64_t ch = g / kChunk;
-  if (end_here) {
+  float x[kPer], y[kPer], z[kPer], w[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    x[r] = px[q[r]];
+    y[r] = py[q[r]];
+    z[r] = pz[q[r]];
+    w[r] = mass[q[r]];
+  }
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int l = r * 256 + tid;
+    float t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (key[r] < sentinel) {
+      // the particle's offsets in its cell, recomputed from its coordinates exactly as its
+      // key was (a live key means all three axes are in range)
+      int ix, iy, iz;
+      float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      (void)pm::cic_axis(x[r], dx, n, ix, fx);
+      (void)pm::cic_axis(y[r], dx, n, iy, fy);
+      (void)pm::cic_axis(z[r], dx, n, iz, fz);
+      corner_terms(make_float4(fx, fy, fz, w[r] * inv_vol), t);
+    }
+    keys_l[l + 1] = key[r];
+    float4* dst = terms4 + ((l / kChunk) * kTermStride + (l % kChunk) * 8) / 4;
+    dst[0] = make_float4(t[0], t[1], t[2], t[3]);
+    dst[1] = make_float4(t[4], t[5], t[6], t[7]);
+  }
+  if (tid == 0) keys_l[0] = base > 0 ? skey[base - 1] : 0xFFFFFFFFu;
+  if (tid == 1) keys_l[kPiecePos + 1] = base + kPiecePos < np ? skey[base + kPiecePos] : 0xFFFFFFFFu;
+  __syncthreads();
+
+  // phase 2: lane c of chunk q sums corner c over the chunk's runs, in order
+  const int lq = tid >> 3;
+  const int c = tid & 7;
+  const int64_t ch = blk * kPieceChunks + lq;
+  if (ch >= chunks) return;  // no barrier below
+  const int64_t g0 = ch * kChunk;
+  const uint32_t* kk = keys_l + 1 + lq * kChunk;  // kk[-1]: key before the chunk
+  const float* tt = terms + lq * kTermStride + c;
+  const uint32_t prevk = kk[-1];
+  float acc = 0.0f;
+  uint32_t rk = 0u;
+  int rs = 0;
+  bool open = false;
+  bool closing = false;  // the run continued from the previous chunk ends here
+  // ends the open run (rs, rk) with sum acc
+  auto flush = [&](bool ends) {
+    const bool starts = rs > 0 || prevk != rk;
     if (starts && ends) {
-      store_cell(sums, g_start, acc);
-      map[key] = static_cast<int32_t>(g_start);
+      sums[(g0 + rs) * 8 + c] = acc;
+      if (c == 0) map[rk] = static_cast<int32_t>(g0 + rs);
     } else if (starts) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) tail[ch * 8 + c] = acc[c];
+      tail[ch * 8 + c] = acc;
     } else {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) head[ch * 8 + c] = acc[c];
+      head[ch * 8 + c] = acc;
     }
+    if (starts) mark_corner_brick(rk, c, n, bricks);
+    if (!starts && ends) closing = true;
+  };
+  uint32_t kr[kChunk];
+  float tr[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    kr[j] = kk[j];
+    tr[j] = tt[j * 8];
   }
-  // the first particle of a cell marks the bricks of the cell's corner nodes
-  if (live && lane == ls && starts) mark_bricks(key, n, bricks);
-  // head_end[ch] = 1 when the run continued from the previous chunk ends in this one
-  const uint64_t closing = __ballot(end_here && !starts && ends);
-  if (valid && (i == kChunk - 1 || g + 1 == np)) {
-    tail_slot[ch] = (end_here && starts && !ends) ? g_start : -1;
-    head_end[ch] = ((closing >> (lane & ~(kChunk - 1))) & 0xFFFFull) != 0 ? 1 : 0;
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    const uint32_t key = kr[j];
+    const float t = tr[j];
+    const bool live = key < sentinel;
+    const bool same = open && live && key == rk;
+    if (open && !same) flush(true);
+    acc = same ? acc + t : t;
+    if (!same) {
+      rk = key;
+      rs = j;
+    }
+    open = live;
+  }
+  // a run still open reaches the chunk's last position
+  const bool last_ends = kk[kChunk] != rk;
+  if (open) flush(last_ends);
+  if (c == 0) {
+    tail_slot[ch] = (open && (rs > 0 || prevk != rk) && !last_ends) ? g0 + rs : -1;
+    head_end[ch] = closing ? 1 : 0;
   }
 }
 
@@ -771,7 +824,6 @@ struct pm_deposit_plan {
   int digit_bits[4] = {0, 0, 0, 0};
   uint32_t* keys[2] = {nullptr, nullptr};
   int32_t* vals[2] = {nullptr, nullptr};
-  float4* rec = nullptr;
   uint32_t* hist = nullptr;
   uint32_t* digit = nullptr;
   float* sums = nullptr;
@@ -782,6 +834,7 @@ struct pm_deposit_plan {
   int32_t* mid_pieces = nullptr;
   uint8_t* bricks = nullptr;    // ceil(n/8)^3 node-brick marks
   bool bricks_valid = false;    // marks of the last deposit (sorted mode only)
+  const int32_t* order = nullptr;  // particle order of the last sorted-mode deposit
   bool map_clean = false;       // map is -1 outside the cells around the marked bricks
   int32_t* map = nullptr;
   BigCell* big = nullptr;
@@ -798,7 +851,7 @@ namespace {
 
 void free_plan(pm_deposit_plan* p) {
   if (!p) return;
-  void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->rec, p->hist, p->digit,
+  void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->hist, p->digit,
                   p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->mid_pieces, p->bricks, p->map, p->big,
                   p->big_count, p->acc64, p->maxbits};
   for (void* b : bufs)
@@ -847,7 +900,6 @@ pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) 
       (st = alloc(&p->keys[1], cap, "hipMalloc keys")) ||
       (st = alloc(&p->vals[0], cap, "hipMalloc vals")) ||
       (st = alloc(&p->vals[1], cap, "hipMalloc vals")) ||
-      (st = alloc(&p->rec, cap, "hipMalloc rec")) ||
       (st = alloc(&p->hist, tiles << kMaxDigitBits, "hipMalloc hist")) ||
       (st = alloc(&p->digit, int64_t(1) << kMaxDigitBits, "hipMalloc digit")) ||
       (st = alloc(&p->sums, 8 * cap, "hipMalloc sums")) ||
@@ -897,6 +949,12 @@ pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode) {
   return PM_OK;
 }
 
+pm_status pm_deposit_plan_order(pm_deposit_plan* plan, const int32_t** order) {
+  PM_REQUIRE(plan && order, "pm_deposit_plan_order: null pointer");
+  *order = plan->order;
+  return PM_OK;
+}
+
 pm_status pm_deposit_plan_node_bricks(pm_deposit_plan* plan, const uint8_t** bricks,
                                       int32_t* brick_edge) {
   PM_REQUIRE(plan && bricks, "pm_deposit_plan_node_bricks: null pointer");
@@ -922,9 +980,16 @@ pm_status pm_deposit_last_timing(pm_deposit_plan* plan, float out_ms[4]) {
   return PM_OK;
 }
 
-pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py,
-                         const float* pz, const float* mass, int64_t np, float dx, float* rho,
-                         uint32_t* keys_out, int32_t* perm_out, hipStream_t stream) {
+}  // extern "C"
+
+namespace {
+
+// The deposit; keys_in (optional) holds the particles' keys already, so the key pass is
+// skipped and the first sort pass reads them where they lie.
+pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, const float* pz,
+                      const float* mass, int64_t np, float dx, float* rho,
+                      const uint32_t* keys_in, uint32_t* keys_out, int32_t* perm_out,
+                      hipStream_t stream) {
   PM_REQUIRE(plan, "pm_deposit_cic: null plan");
   PM_REQUIRE(rho, "pm_deposit_cic: null rho");
   PM_REQUIRE(np >= 0, "pm_deposit_cic: negative particle count");
@@ -936,6 +1001,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   }
   PM_REQUIRE(np == 0 || (px && py && pz && mass), "pm_deposit_cic: null particle array");
   plan->bricks_valid = plan->mode == PM_DEPOSIT_SORTED;
+  plan->order = nullptr;
   if (plan->mode == PM_DEPOSIT_FIXED) {
     PM_REQUIRE(!perm_out, "pm_deposit_cic: the particle order needs PM_DEPOSIT_SORTED mode");
     return pm::depfix::deposit(px, py, pz, mass, np, plan->n, dx, rho, keys_out, plan->acc64,
@@ -977,12 +1043,14 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
 
   // 1. keys
   const float inv_vol = 1.0f / ((dx * dx) * dx);
-  {
-    PM_TIMED("deposit.key", 36.0 * np, stream);
+  if (!keys_in) {
+    PM_TIMED("deposit.key", 16.0 * np, stream);
     key_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
-        px, py, pz, mass, np, n, dx, inv_vol, plan->sentinel, plan->keys[0], plan->rec,
-        keys_out);
+        px, py, pz, np, n, dx, plan->sentinel, plan->keys[0], keys_out);
     PM_CHECK_LAUNCH("key_kernel");
+  } else if (keys_out) {
+    PM_TRY_HIP(hipMemcpyAsync(keys_out, keys_in, sizeof(uint32_t) * np,
+                              hipMemcpyDeviceToDevice, stream));
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[1], stream));
 
@@ -992,10 +1060,10 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   for (int s = 0; s < plan->passes; ++s) {
     const int bits = plan->digit_bits[s];
     const int buckets = 1 << bits;
+    const uint32_t* kin = (s == 0 && keys_in) ? keys_in : plan->keys[src];
     {
       PM_TIMED("deposit.radix_hist", 4.0 * np, stream);
-      radix_hist_kernel<<<tiles, 256, 0, stream>>>(plan->keys[src], np, tiles, shift, bits,
-                                                    plan->hist);
+      radix_hist_kernel<<<tiles, 256, 0, stream>>>(kin, np, tiles, shift, bits, plan->hist);
       PM_CHECK_LAUNCH("radix_hist_kernel");
     }
     {
@@ -1006,7 +1074,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
     {
       // read key (+ index after the first pass), write key + index
       PM_LAUNCH_TIMED("deposit.radix_scatter", (s == 0 ? 12.0 : 16.0) * np,
-                      radix_scatter_kernel, tiles, 256, 0, stream, plan->keys[src],
+                      radix_scatter_kernel, tiles, 256, 0, stream, kin,
                       s == 0 ? nullptr : plan->vals[src], plan->keys[1 - src],
                       plan->vals[1 - src], plan->hist, plan->digit, np, tiles, shift, bits);
       PM_CHECK_LAUNCH("radix_scatter_kernel");
@@ -1016,6 +1084,7 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   }
   const uint32_t* skey = plan->keys[src];
   const int32_t* sidx = plan->vals[src];
+  plan->order = sidx;
   if (perm_out)
     PM_TRY_HIP(hipMemcpyAsync(perm_out, sidx, sizeof(int32_t) * np, hipMemcpyDeviceToDevice,
                               stream));
@@ -1025,9 +1094,10 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   const int64_t chunks = (np + kChunk - 1) / kChunk;
   {
     PM_TIMED("deposit.piece", 24.0 * np, stream);
-    piece_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
-        skey, sidx, plan->rec, np, plan->sentinel, chunks, plan->sums, plan->cap, plan->map,
-        plan->head, plan->tail, plan->tail_slot, plan->head_end, n, plan->bricks);
+    const unsigned piece_blocks = static_cast<unsigned>((np + kPiecePos - 1) / kPiecePos);
+    piece_kernel<<<piece_blocks, 256, 0, stream>>>(
+        skey, sidx, px, py, pz, mass, dx, inv_vol, np, plan->sentinel, chunks, plan->sums,
+        plan->map, plan->head, plan->tail, plan->tail_slot, plan->head_end, n, plan->bricks);
     PM_CHECK_LAUNCH("piece_kernel");
   }
   {
@@ -1060,6 +1130,24 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[4], stream));
   return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* px, const float* py,
+                         const float* pz, const float* mass, int64_t np, float dx, float* rho,
+                         uint32_t* keys_out, int32_t* perm_out, hipStream_t stream) {
+  return deposit_cic(plan, px, py, pz, mass, np, dx, rho, nullptr, keys_out, perm_out, stream);
+}
+
+pm_status pm_deposit_cic_keyed(pm_deposit_plan* plan, const float* px, const float* py,
+                               const float* pz, const float* mass, int64_t np, float dx,
+                               float* rho, const uint32_t* keys, int32_t* perm_out,
+                               hipStream_t stream) {
+  PM_REQUIRE(np == 0 || keys, "pm_deposit_cic_keyed: null keys");
+  return deposit_cic(plan, px, py, pz, mass, np, dx, rho, keys, nullptr, perm_out, stream);
 }
 
 }  // extern "C"

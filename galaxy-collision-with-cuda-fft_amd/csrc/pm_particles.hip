@@ -196,11 +196,14 @@ __global__ void __launch_bounds__(256) gather_kick4_kernel(
 // Step t's closing gather+kick(h_close) joined with step t+1's opening kick(h_open) and
 // drift(dt): the same float operations in the same order as gather_kick4 followed by
 // kick_drift, without the round trip of a and v through memory.
+// With KEYS it also writes each particle's extended cell key for the drifted position, as
+// the deposit's key pass would compute it from the stored coordinates.
+template <bool KEYS>
 __global__ void __launch_bounds__(256) gather_kick_drift4_kernel(
     float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
     const float4* __restrict__ g4, float* __restrict__ vx, float* __restrict__ vy,
     float* __restrict__ vz, int64_t np, int n, float dx, float h_close, float h_open,
-    float dt) {
+    float dt, uint32_t* __restrict__ keys) {
   const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const float x = px[p], y = py[p], z = pz[p];
@@ -215,9 +218,72 @@ __global__ void __launch_bounds__(256) gather_kick_drift4_kernel(
   vx[p] = w0;
   vy[p] = w1;
   vz[p] = w2;
-  px[p] = x + w0 * dt;
-  py[p] = y + w1 * dt;
-  pz[p] = z + w2 * dt;
+  const float x1 = x + w0 * dt, y1 = y + w1 * dt, z1 = z + w2 * dt;
+  px[p] = x1;
+  py[p] = y1;
+  pz[p] = z1;
+  if (KEYS) {
+    int ix = 0, iy = 0, iz = 0;
+    float fx, fy, fz;
+    const bool in = pm::cic_axis(x1, dx, n, ix, fx) && pm::cic_axis(y1, dx, n, iy, fy) &&
+                    pm::cic_axis(z1, dx, n, iz, fz);
+    const uint32_t m = static_cast<uint32_t>(n) + 1u;
+    keys[p] = in ? pm::cell_key(ix, iy, iz, n) : m * m * m;
+  }
+}
+
+// gather_kick_drift4_kernel<true> that also moves particle order[p] of `src` to slot p of
+// `dst`: the same float operations per particle, only the slots change.
+struct ParticleSlots {
+  float* x;
+  float* y;
+  float* z;
+  float* vx;
+  float* vy;
+  float* vz;
+  float* m;
+  int32_t* gid;
+  int32_t* pid;
+};
+
+template <bool PID>
+__global__ void __launch_bounds__(256) gather_kick_drift4_reorder_kernel(
+    const int32_t* __restrict__ order, ParticleSlots src, ParticleSlots dst,
+    const float4* __restrict__ g4, int64_t np, int n, float dx, float h_close, float h_open,
+    float dt, uint32_t* __restrict__ keys) {
+  const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  int64_t q = order[p];
+  if (q < 0 || q >= np) q = p;  // not a permutation entry: keep the slot's own particle
+  const float x = src.x[q], y = src.y[q], z = src.z[q];
+  const float u0 = src.vx[q], u1 = src.vy[q], u2 = src.vz[q];
+  const float m = src.m[q];
+  const int32_t gid = src.gid[q];
+  const int32_t pid = PID ? src.pid[q] : 0;
+  float a0, a1, a2;
+  cic_gather4(x, y, z, g4, n, dx, a0, a1, a2);
+  const float v0 = u0 + a0 * h_close;
+  const float v1 = u1 + a1 * h_close;
+  const float v2 = u2 + a2 * h_close;
+  const float w0 = v0 + a0 * h_open;
+  const float w1 = v1 + a1 * h_open;
+  const float w2 = v2 + a2 * h_open;
+  dst.vx[p] = w0;
+  dst.vy[p] = w1;
+  dst.vz[p] = w2;
+  const float x1 = x + w0 * dt, y1 = y + w1 * dt, z1 = z + w2 * dt;
+  dst.x[p] = x1;
+  dst.y[p] = y1;
+  dst.z[p] = z1;
+  dst.m[p] = m;
+  dst.gid[p] = gid;
+  if (PID) dst.pid[p] = pid;
+  int ix = 0, iy = 0, iz = 0;
+  float fx, fy, fz;
+  const bool in = pm::cic_axis(x1, dx, n, ix, fx) && pm::cic_axis(y1, dx, n, iy, fy) &&
+                  pm::cic_axis(z1, dx, n, iz, fz);
+  const uint32_t mm = static_cast<uint32_t>(n) + 1u;
+  keys[p] = in ? pm::cell_key(ix, iy, iz, n) : mm * mm * mm;
 }
 
 __global__ void __launch_bounds__(256) gather_kernel(
@@ -441,7 +507,8 @@ pm_status pm_gather_kick4(const float* px, const float* py, const float* pz, con
 
 pm_status pm_gather_kick_drift4(float* px, float* py, float* pz, const float* g4, float* vx,
                                 float* vy, float* vz, int64_t np, int32_t n, float dx,
-                                float h_close, float h_open, float dt, hipStream_t stream) {
+                                float h_close, float h_open, float dt, uint32_t* keys_out,
+                                hipStream_t stream) {
   PM_REQUIRE(np >= 0, "pm_gather_kick_drift4: negative particle count");
   if (np == 0) return PM_OK;
   PM_REQUIRE(px && py && pz && g4 && vx && vy && vz, "pm_gather_kick_drift4: null pointer");
@@ -450,11 +517,75 @@ pm_status pm_gather_kick_drift4(float* px, float* py, float* pz, const float* g4
   PM_REQUIRE(n >= 1 && n <= 1024, "pm_gather_kick_drift4: grid_size must be in [1, 1024]");
   PM_REQUIRE(dx > 0.0f, "pm_gather_kick_drift4: dx must be positive");
   {
-    PM_TIMED("gather_kick_drift", 48.0 * np + 12.0 * n * n * n, stream);
-    gather_kick_drift4_kernel<<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
-        px, py, pz, reinterpret_cast<const float4*>(g4), vx, vy, vz, np, n, dx, h_close,
-        h_open, dt);
+    PM_TIMED("gather_kick_drift", (keys_out ? 52.0 : 48.0) * np + 12.0 * n * n * n, stream);
+    if (keys_out) {
+      gather_kick_drift4_kernel<true><<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+          px, py, pz, reinterpret_cast<const float4*>(g4), vx, vy, vz, np, n, dx, h_close,
+          h_open, dt, keys_out);
+    } else {
+      gather_kick_drift4_kernel<false><<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+          px, py, pz, reinterpret_cast<const float4*>(g4), vx, vy, vz, np, n, dx, h_close,
+          h_open, dt, nullptr);
+    }
     PM_CHECK_LAUNCH("gather_kick_drift4_kernel");
+  }
+  return PM_OK;
+}
+
+pm_status pm_gather_kick_drift4_reorder(const int32_t* order, const pm_particle_arrays* src,
+                                        const pm_particle_arrays* dst, const float* g4,
+                                        int64_t np, int32_t n, float dx, float h_close,
+                                        float h_open, float dt, uint32_t* keys_out,
+                                        hipStream_t stream) {
+  PM_REQUIRE(np >= 0, "pm_gather_kick_drift4_reorder: negative particle count");
+  if (np == 0) return PM_OK;
+  PM_REQUIRE(order && src && dst && g4 && keys_out, "pm_gather_kick_drift4_reorder: null pointer");
+  PM_REQUIRE((reinterpret_cast<uintptr_t>(g4) & 15u) == 0,
+             "pm_gather_kick_drift4_reorder: g4 must be 16-byte aligned");
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_gather_kick_drift4_reorder: grid_size must be in [1, 1024]");
+  PM_REQUIRE(dx > 0.0f, "pm_gather_kick_drift4_reorder: dx must be positive");
+  const void* sp[9] = {src->pos_x, src->pos_y, src->pos_z, src->vel_x, src->vel_y,
+                       src->vel_z, src->mass, src->gid, src->pid};
+  const void* dp[9] = {dst->pos_x, dst->pos_y, dst->pos_z, dst->vel_x, dst->vel_y,
+                       dst->vel_z, dst->mass, dst->gid, dst->pid};
+  const bool pid = src->pid != nullptr;
+  PM_REQUIRE(pid == (dst->pid != nullptr),
+             "pm_gather_kick_drift4_reorder: pid must be given in both src and dst or in neither");
+  const int count = pid ? 9 : 8;
+  for (int k = 0; k < count; ++k)
+    PM_REQUIRE(sp[k] && dp[k], "pm_gather_kick_drift4_reorder: null particle array");
+  // out of place: no destination array may overlap a source array (or another destination)
+  const uint64_t bytes = static_cast<uint64_t>(np) * 4u;
+  auto overlap = [bytes](const void* a, const void* b) {
+    const uint64_t x = reinterpret_cast<uint64_t>(a), y = reinterpret_cast<uint64_t>(b);
+    return x < y + bytes && y < x + bytes;
+  };
+  for (int k = 0; k < count; ++k) {
+    for (int j = 0; j < count; ++j) {
+      PM_REQUIRE(!overlap(dp[k], sp[j]),
+                 "pm_gather_kick_drift4_reorder: a destination array overlaps a source array");
+      PM_REQUIRE(j == k || !overlap(dp[k], dp[j]),
+                 "pm_gather_kick_drift4_reorder: destination arrays overlap");
+    }
+    PM_REQUIRE(!overlap(dp[k], keys_out) && !overlap(dp[k], order),
+               "pm_gather_kick_drift4_reorder: a destination array overlaps keys_out or order");
+  }
+  ParticleSlots s{src->pos_x, src->pos_y, src->pos_z, src->vel_x, src->vel_y, src->vel_z,
+                  src->mass, src->gid, src->pid};
+  ParticleSlots d{dst->pos_x, dst->pos_y, dst->pos_z, dst->vel_x, dst->vel_y, dst->vel_z,
+                  dst->mass, dst->gid, dst->pid};
+  {
+    PM_TIMED("gather_kick_drift_reorder", (pid ? 80.0 : 72.0) * np + 12.0 * n * n * n, stream);
+    if (pid) {
+      gather_kick_drift4_reorder_kernel<true><<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+          order, s, d, reinterpret_cast<const float4*>(g4), np, n, dx, h_close, h_open, dt,
+          keys_out);
+    } else {
+      gather_kick_drift4_reorder_kernel<false><<<pm::blocks_for(np), pm::kBlock, 0, stream>>>(
+          order, s, d, reinterpret_cast<const float4*>(g4), np, n, dx, h_close, h_open, dt,
+          keys_out);
+    }
+    PM_CHECK_LAUNCH("gather_kick_drift4_reorder_kernel");
   }
   return PM_OK;
 }

@@ -104,6 +104,13 @@ class WsnetLayer(ctypes.Structure):
     ]
 
 
+class ParticleArrays(ctypes.Structure):
+    """pm_particle_arrays: device pointers of one particle set's arrays."""
+
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "mass", "gid", "pid")]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 48),
@@ -137,6 +144,7 @@ _SIGNATURES = {
     "pm_deposit_plan_create": [_i32, _i64, ctypes.POINTER(_vp)],
     "pm_deposit_plan_destroy": [_vp],
     "pm_deposit_cic": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
+    "pm_deposit_cic_keyed": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "pm_deposit_plan_set_timing": [_vp, _i32],
     "pm_deposit_plan_set_mode": [_vp, _i32],
     "pm_deposit_last_timing": [_vp, ctypes.POINTER(_f)],
@@ -150,10 +158,14 @@ _SIGNATURES = {
     "pm_grad4": [_vp, _vp, _i32, _f, _vp],
     "pm_grad4_bricks": [_vp, _vp, _i32, _f, _vp, _vp],
     "pm_deposit_plan_node_bricks": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i32)],
+    "pm_deposit_plan_order": [_vp, ctypes.POINTER(_vp)],
+    "pm_gather_kick_drift4_reorder": [_vp, ctypes.POINTER(ParticleArrays),
+                                      ctypes.POINTER(ParticleArrays), _vp, _i64, _i32, _f, _f,
+                                      _f, _f, _vp, _vp],
     "pm_gather_kick4": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _f,
                         _vp],
     "pm_gather_kick_drift4": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _f, _f, _f, _f,
-                              _vp],
+                              _vp, _vp],
     "pm_direct_accel": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _vp],
     "pm_mg_create": [ctypes.POINTER(MGDesc), ctypes.POINTER(_vp)],
     "pm_mg_destroy": [_vp],

@@ -109,7 +109,12 @@ def as_tensor(f) -> torch.Tensor:
 
 
 class ParticleState:
-    """SoA particle fields on the device (reference data.py:84-113)."""
+    """SoA particle fields on the device (reference data.py:84-113).
+
+    ``pid`` holds each particle's index in load order. A chain that reorders the particles
+    (``PMChain(reorder_every=...)``) moves every field, ``pid`` included, into a second set
+    of arrays and swaps the sets, so ``pid`` keeps identifying particles.
+    """
 
     def __init__(self, n: int, device=None):
         if n <= 0:
@@ -121,10 +126,46 @@ class ParticleState:
         self.vel_x, self.vel_y, self.vel_z = mk(), mk(), mk()
         self.mass = mk()
         self.gid = mk(torch.int32)
+        self.pid = DeviceField(torch.arange(n, dtype=torch.int32, device=dev))
+        self._shadow: list[torch.Tensor] | None = None
 
     def fields(self):
         return (self.pos_x, self.pos_y, self.pos_z, self.vel_x, self.vel_y, self.vel_z,
-                self.mass, self.gid)
+                self.mass, self.gid, self.pid)
+
+    def arrays(self, shadow: bool = False):
+        """pm_particle_arrays of the fields (or of the second set, allocated on first use)."""
+        from galaxy_collision_amd._native import ParticleArrays
+
+        if shadow:
+            if self._shadow is None:
+                self._shadow = [torch.empty_like(f.tensor) for f in self.fields()]
+            ptrs = [t.data_ptr() for t in self._shadow]
+        else:
+            ptrs = [f.tensor.data_ptr() for f in self.fields()]
+        return ParticleArrays(*ptrs)
+
+    def swap_shadow(self) -> None:
+        """Make the second set of arrays the fields (after it was filled)."""
+        if self._shadow is None:
+            raise RuntimeError("no second set of particle arrays to swap in")
+        for k, f in enumerate(self.fields()):
+            f.tensor, self._shadow[k] = self._shadow[k], f.tensor
+
+    def slots_of(self, indices) -> torch.Tensor:
+        """Current slots (device int64) of the particles with the given load indices."""
+        pid = self.pid.tensor
+        slot = torch.empty(pid.numel(), dtype=torch.int64, device=pid.device)
+        slot[pid.long()] = torch.arange(pid.numel(), dtype=torch.int64, device=pid.device)
+        return slot[torch.as_tensor(np.asarray(indices), dtype=torch.int64, device=pid.device)]
+
+    def in_load_order(self) -> np.ndarray | None:
+        """Host permutation that puts host copies of the fields back into load order, or None
+        when the particles are in load order."""
+        pid = self.pid.to_numpy()
+        if np.array_equal(pid, np.arange(pid.shape[0], dtype=pid.dtype)):
+            return None
+        return np.argsort(pid, kind="stable")
 
     def load(self, pos: np.ndarray, vel: np.ndarray | None = None, mass: np.ndarray | None = None,
              gid: np.ndarray | None = None) -> None:
@@ -140,6 +181,8 @@ class ParticleState:
             self.mass.from_numpy(np.asarray(mass, dtype=np.float32))
         if gid is not None:
             self.gid.from_numpy(np.asarray(gid, dtype=np.int32))
+        t = self.pid.tensor
+        t.copy_(torch.arange(t.numel(), dtype=torch.int32, device=t.device))
 
     def positions(self) -> np.ndarray:
         return np.stack([f.to_numpy() for f in (self.pos_x, self.pos_y, self.pos_z)], axis=1)

@@ -57,6 +57,23 @@ class DepositPlan:
                  nat.ptr(perm_out, torch.int32) if perm_out is not None else None,
                  nat.stream())
 
+    def deposit_keyed(self, pos_x, pos_y, pos_z, mass, n: int, dx: float, rho,
+                      keys: torch.Tensor, perm_out: torch.Tensor | None = None):
+        """``deposit`` for particles whose cell keys ``keys`` (as ``keys_out`` would hold
+        them) are already known, e.g. written by the fused drift pass: no key pass."""
+        nat.call("pm_deposit_cic_keyed", self.handle, nat.ptr(as_tensor(pos_x)),
+                 nat.ptr(as_tensor(pos_y)), nat.ptr(as_tensor(pos_z)), nat.ptr(as_tensor(mass)),
+                 int(n), float(dx), nat.ptr(as_tensor(rho)), nat.ptr(keys, torch.int32),
+                 nat.ptr(perm_out, torch.int32) if perm_out is not None else None,
+                 nat.stream())
+
+    def order(self) -> int | None:
+        """Device address of the last deposit's particle order (sorted position -> particle
+        index), or None after a fixed-mode or empty deposit; valid until the next deposit."""
+        ptr = ctypes.c_void_p()
+        nat.call("pm_deposit_plan_order", self.handle, ctypes.byref(ptr))
+        return ptr.value
+
     def node_bricks(self) -> int | None:
         """Device address of the last deposit's node-brick marks (8^3 node bricks holding a
         CIC corner of some deposited particle), or None after a fixed-mode deposit."""

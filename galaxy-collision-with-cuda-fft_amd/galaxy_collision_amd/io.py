@@ -40,12 +40,17 @@ class Snapshot:
 
 
 def snapshot_from_states(step, time, parts, grid=None, diagnostics=None, dx=1.0) -> Snapshot:
-    """Copy a device ParticleState (and optionally GridState) into a Snapshot."""
+    """Copy a device ParticleState (and optionally GridState) into a Snapshot, particles in
+    load order (a chain that reorders particles moves them; ``parts.pid`` maps them back)."""
     pos = np.stack([parts.pos_x.to_numpy(), parts.pos_y.to_numpy(), parts.pos_z.to_numpy()], 1)
     vel = np.stack([parts.vel_x.to_numpy(), parts.vel_y.to_numpy(), parts.vel_z.to_numpy()], 1)
+    mass, gid = parts.mass.to_numpy(), parts.gid.to_numpy()
+    back = parts.in_load_order() if hasattr(parts, "in_load_order") else None
+    if back is not None:
+        pos, vel, mass, gid = pos[back], vel[back], mass[back], gid[back]
     return Snapshot(
-        step=int(step), time=float(time), pos=pos, vel=vel, mass=parts.mass.to_numpy(),
-        gid=parts.gid.to_numpy(), dx=float(dx),
+        step=int(step), time=float(time), pos=pos, vel=vel, mass=mass,
+        gid=gid, dx=float(dx),
         rho=grid.rho.to_numpy() if grid is not None else None,
         phi=grid.phi.to_numpy() if grid is not None else None,
         diagnostics={k: np.asarray(v) for k, v in (diagnostics or {}).items()})

@@ -46,6 +46,7 @@ class SimConfig:
     output_cadence: int = 0     # steps between snapshots (0: none)
     output_dir: str = "outputs"
     seed: int = 0
+    reorder_every: int = 4      # steps between moves of the particles into cell order (0: never)
 
     def __post_init__(self) -> None:
         self.validate()
@@ -65,6 +66,8 @@ class SimConfig:
             raise ValueError(f"steps must be non-negative, got {self.steps}")
         if self.grid_size < 3 or self.dx <= 0:
             raise ValueError("grid_size must be >= 3 and dx positive")
+        if self.reorder_every < 0:
+            raise ValueError(f"reorder_every must be non-negative, got {self.reorder_every}")
         if self.output_cadence < 0:
             raise ValueError(f"output_cadence must be non-negative, got {self.output_cadence}")
 
@@ -132,10 +135,9 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
     grid = GridState(gsize)
     solver = make_solver(config.solver, gsize, dx=dx, grav_constant=config.grav,
                          **(solver_kwargs or {}))
-    chain = PMChain(parts, gsize, dx, solver, grid=grid)
+    chain = PMChain(parts, gsize, dx, solver, grid=grid, reorder_every=config.reorder_every)
     diag = DeviceDiagnostics(gsize, dx=dx)
     tracer_idx = None if tracer_indices is None else [int(i) for i in np.atleast_1d(tracer_indices)]
-    mass_np = icr.mass.astype(np.float64) if direct_pe_softening is not None else None
     frames, frame_times = [], []
     extent = (0.0, gsize * dx, 0.0, gsize * dx)
     frame_img = (torch.zeros((frame_bins, frame_bins), device=torch.device("cuda"))
@@ -149,12 +151,16 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
                "half_mass_radius": d["half_mass_radius"], "virial": virial_ratio(ke, pe)}
         if direct_pe_softening is not None:
             pos = parts.positions().astype(np.float64)
+            mass_np = parts.mass.to_numpy().astype(np.float64)  # same (current) slot order
             pe_d = potential_energy_direct(mass_np, pos, direct_pe_softening, config.grav)
             rec["potential_direct"] = pe_d
             rec["energy_direct"] = ke + pe_d
         if tracer_idx is not None:
-            rec["tracer_pos"] = np.array([[parts.pos_x[i], parts.pos_y[i], parts.pos_z[i]]
-                                          for i in tracer_idx], dtype=np.float64)
+            # tracers are load indices; the chain may have moved the particles
+            slots = parts.slots_of(tracer_idx)
+            rec["tracer_pos"] = torch.stack(
+                [f.tensor[slots] for f in (parts.pos_x, parts.pos_y, parts.pos_z)],
+                1).cpu().numpy().astype(np.float64)
         return rec
 
     diag_keys = ("energy", "kinetic", "potential", "momentum", "ang_momentum",

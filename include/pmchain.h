@@ -72,9 +72,24 @@ pm_status pm_deposit_cic(pm_deposit_plan* plan, const float* pos_x, const float*
                          const float* pos_z, const float* mass, int64_t n_particles, float dx,
                          float* rho, uint32_t* keys_out, int32_t* perm_out, hipStream_t stream);
 
+/* pm_deposit_cic for particles whose cell keys the caller already holds (`keys`, as
+ * pm_deposit_cic's keys_out gives them for these positions, e.g. written by
+ * pm_gather_kick_drift4 when it moved them): the key pass is skipped and `keys` is only
+ * read. Same result, bit for bit. Replaces the same deposit call as pm_deposit_cic
+ * (reference deposit.py:47-82, sim.py:318-325); no counterpart of its own. */
+pm_status pm_deposit_cic_keyed(pm_deposit_plan* plan, const float* pos_x, const float* pos_y,
+                               const float* pos_z, const float* mass, int64_t n_particles,
+                               float dx, float* rho, const uint32_t* keys, int32_t* perm_out,
+                               hipStream_t stream);
+
 /* Timing hook: per-phase device times (ms) of the last pm_deposit_cic on this plan when
  * pm_deposit_plan_set_timing(plan, 1) was set before it; phases = key, sort, reduce, assemble. */
 pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable);
+
+/* Particle order of the last pm_deposit_cic / pm_deposit_cic_keyed with this plan: order[i]
+ * is the particle at sorted position i (the perm_out array), or NULL after a fixed-mode or
+ * empty deposit. Device memory owned by the plan, valid until its next deposit. */
+pm_status pm_deposit_plan_order(pm_deposit_plan* plan, const int32_t** order);
 
 /* Node bricks of the last deposit with this plan: bricks[b] != 0 when some deposited
  * particle has a CIC corner node in the brick_edge^3 node brick b = (bi*nb + bj)*nb + bk,
@@ -137,11 +152,42 @@ pm_status pm_gather_kick4(const float* pos_x, const float* pos_y, const float* p
  * kick(h_open) and drift(dt) of step t+1, in one pass. Bitwise the same as pm_gather_kick4
  * followed by pm_kick_drift (reference integrator.py:100-131 `kdk_step`, its closing kick
  * followed by the next call's opening kick and drift); acc is not written, so the caller
- * ends a run of steps with pm_gather_kick4. */
+ * ends a run of steps with pm_gather_kick4. keys_out (optional, n_particles uint32): the
+ * drifted particles' cell keys on this grid, as pm_deposit_cic's keys_out would give them,
+ * for the next step's pm_deposit_cic_keyed. */
 pm_status pm_gather_kick_drift4(float* pos_x, float* pos_y, float* pos_z, const float* g4,
                                 float* vel_x, float* vel_y, float* vel_z, int64_t n_particles,
                                 int32_t grid_size, float dx, float h_close, float h_open,
-                                float dt, hipStream_t stream);
+                                float dt, uint32_t* keys_out, hipStream_t stream);
+
+/* The per-particle arrays of one particle set (structure of arrays, n_particles each). pid
+ * (optional) is carried along untouched, e.g. a particle's index in the initial conditions. */
+typedef struct {
+  float* pos_x;
+  float* pos_y;
+  float* pos_z;
+  float* vel_x;
+  float* vel_y;
+  float* vel_z;
+  float* mass;
+  int32_t* gid;
+  int32_t* pid;
+} pm_particle_arrays;
+
+/* pm_gather_kick_drift4 (keys_out required) that also moves particle order[i] of `src` to
+ * slot i of `dst`, with every array (out of place: no destination may overlap a source,
+ * keys_out or order; pid must be given in both or neither). With order = the last deposit's
+ * particle order (pm_deposit_plan_order), particles of one cell become neighbours in memory
+ * again, which keeps the next deposit's and gather's loads local over long runs. Each
+ * particle's new position, velocity and key are bitwise those of pm_gather_kick_drift4;
+ * entries of order outside [0, n_particles) keep the slot's own particle. Replaces the same
+ * reference calls as pm_gather_kick_drift4 (integrator.py:100-131); the reordering itself
+ * has no counterpart (the reference keeps its particles in load order). */
+pm_status pm_gather_kick_drift4_reorder(const int32_t* order, const pm_particle_arrays* src,
+                                        const pm_particle_arrays* dst, const float* g4,
+                                        int64_t n_particles, int32_t grid_size, float dx,
+                                        float h_close, float h_open, float dt,
+                                        uint32_t* keys_out, hipStream_t stream);
 
 /* Fused gather then kick(h): bitwise the same as pm_gather followed by pm_kick
  * (the closing half of reference integrator.py:100-131 `kdk_step`). */

@@ -53,6 +53,11 @@ def test_error_path_without_device_work():
     with pytest.raises(nat.PMChainError, match="grid_size"):
         out = ctypes.c_void_p()
         nat.call("pm_deposit_plan_create", 0, 10, ctypes.byref(out))
+    # the reordering drift is out of place: a destination overlapping a source is refused
+    arr = nat.ParticleArrays(*[0x10000000 + k * 0x100000 for k in range(9)])
+    with pytest.raises(nat.PMChainError, match="overlaps a source"):
+        nat.call("pm_gather_kick_drift4_reorder", 0x20000000, ctypes.byref(arr),
+                 ctypes.byref(arr), 0x30000000, 1000, 16, 1.0, 0.1, 0.1, 0.1, 0x40000000, None)
     with pytest.raises(nat.PMChainError, match="mode must be"):
         nat.call("pm_timing_enable", 3)
     nat.call("pm_timing_enable", 0)

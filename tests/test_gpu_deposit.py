@@ -258,3 +258,57 @@ def test_plan_reuse_across_disjoint_particle_sets(device):
         ref, _, _ = O.deposit_cic(pos, mass, n, dx)
         assert_bitwise(rho.cpu().numpy(), ref, f"set {q}")
     plan.close()
+
+
+def test_drift_keys_and_keyed_deposit(device):
+    """The fused drift pass writes each moved particle's cell key exactly as the deposit's
+    key pass computes it (out-of-box particles included), and the deposit that takes those
+    keys gives the same density and particle order bit for bit."""
+    announce("drift keys + keyed deposit")
+    from galaxy_collision_amd import _native as nat
+    from galaxy_collision_amd.data import ParticleState
+    from galaxy_collision_amd.deposit import deposit_plan
+
+    rng = np.random.default_rng(21)
+    n, dx, npart = 48, 0.75, 200_003
+    box = n * dx
+    pos = rng.normal(box / 2, box / 6, size=(npart, 3)).astype(F32)
+    pos[:1000] = rng.uniform(-2 * dx, box + 2 * dx, size=(1000, 3))  # edges and outside
+    pos[1000:1100] = np.float32(box - 0.5 * dx)  # one crowded cell at the far edge
+    vel = rng.normal(0, 1, size=(npart, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2.0, npart).astype(F32)
+    parts = ParticleState(npart)
+    parts.load(pos, vel, mass)
+    g4 = torch.randn((n, n, n, 4), device=device)
+    keys = torch.empty(npart, dtype=torch.int32, device=device)
+    p = lambda f: nat.ptr(f.tensor)  # noqa: E731
+    nat.call("pm_gather_kick_drift4", p(parts.pos_x), p(parts.pos_y), p(parts.pos_z),
+             nat.ptr(g4), p(parts.vel_x), p(parts.vel_y), p(parts.vel_z), npart, n, dx,
+             0.01, 0.01, 0.05, nat.ptr(keys, torch.int32), nat.stream())
+    plan = deposit_plan(n, npart)
+    plan.set_mode("sorted")
+    rho_a = torch.full((n, n, n), 7.0, device=device)
+    rho_b = torch.full((n, n, n), 7.0, device=device)
+    keys_ref = torch.empty(npart, dtype=torch.int32, device=device)
+    perm_a = torch.empty(npart, dtype=torch.int32, device=device)
+    perm_b = torch.empty(npart, dtype=torch.int32, device=device)
+    plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, npart, dx, rho_a, keys_ref,
+                 perm_a)
+    keys_before = keys.clone()
+    plan.deposit_keyed(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, npart, dx, rho_b,
+                       keys, perm_b)
+    assert_bitwise(_np_u32(keys), _np_u32(keys_ref))
+    assert_bitwise(_np_u32(keys), _np_u32(keys_before))  # only read
+    assert_bitwise(rho_b.cpu().numpy(), rho_a.cpu().numpy())
+    assert_bitwise(perm_b.cpu().numpy(), perm_a.cpu().numpy())
+    sentinel = (n + 1) ** 3
+    k = _np_u32(keys)
+    assert (k == sentinel).any() and (k < sentinel).any()
+    # and against the oracle on the moved positions
+    ref_rho, ref_keys = O.deposit_cic(parts.positions(), mass, n, dx)[:2]
+    assert_bitwise(k, ref_keys)
+    assert_bitwise(rho_b.cpu().numpy(), ref_rho)
+
+
+def _np_u32(t):
+    return t.cpu().numpy().view(np.uint32)

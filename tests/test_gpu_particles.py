@@ -7,6 +7,8 @@ integrator.py:27-131).
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -307,3 +309,92 @@ def test_grad4_on_deposit_bricks(device, n, npart):
     plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, npart, dx, rho)
     assert plan.node_bricks() is None
     plan.close()
+
+
+def test_reordering_drift_moves_particles_bitwise(device):
+    """pm_gather_kick_drift4_reorder: slot i of the destination holds particle order[i]
+    after exactly pm_gather_kick_drift4's arithmetic (position, velocity, key), with its
+    mass, gid and pid moved along; an order entry outside [0, n) keeps the slot's own
+    particle."""
+    announce("reordering drift")
+    from galaxy_collision_amd import _native as nat
+    from galaxy_collision_amd.data import ParticleState
+
+    rng = np.random.default_rng(8)
+    n, dx, npart = 40, 0.5, 150_001
+    box = n * dx
+    pos = rng.uniform(-dx, box + dx, size=(npart, 3)).astype(F32)
+    vel = rng.normal(0, 1, size=(npart, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2.0, npart).astype(F32)
+    gid = rng.integers(0, 2, npart).astype(np.int32)
+    g4 = torch.randn((n, n, n, 4), device=device)
+    h, dt = 0.01, 0.02
+
+    a = ParticleState(npart)
+    a.load(pos, vel, mass, gid)
+    keys_a = torch.empty(npart, dtype=torch.int32, device=device)
+    p = lambda f: nat.ptr(f.tensor)  # noqa: E731
+    nat.call("pm_gather_kick_drift4", p(a.pos_x), p(a.pos_y), p(a.pos_z), nat.ptr(g4),
+             p(a.vel_x), p(a.vel_y), p(a.vel_z), npart, n, dx, h, h, dt,
+             nat.ptr(keys_a, torch.int32), nat.stream())
+
+    order = rng.permutation(npart).astype(np.int32)
+    order[5] = -3          # not a permutation entry: slot 5 keeps particle 5
+    order[6] = npart + 9   # likewise
+    b = ParticleState(npart)
+    b.load(pos, vel, mass, gid)
+    ot = torch.from_numpy(order).to(device)
+    keys_b = torch.empty(npart, dtype=torch.int32, device=device)
+    nat.call("pm_gather_kick_drift4_reorder", nat.ptr(ot, torch.int32),
+             ctypes.byref(b.arrays()), ctypes.byref(b.arrays(shadow=True)), nat.ptr(g4),
+             npart, n, dx, h, h, dt, nat.ptr(keys_b, torch.int32), nat.stream())
+    b.swap_shadow()
+    src = order.astype(np.int64)
+    src[5], src[6] = 5, 6
+    assert_bitwise(b.positions(), a.positions()[src], "positions")
+    assert_bitwise(b.velocities(), a.velocities()[src], "velocities")
+    assert_bitwise(b.mass.to_numpy(), mass[src], "mass")
+    assert_bitwise(b.gid.to_numpy(), gid[src], "gid")
+    assert_bitwise(b.pid.to_numpy(), src.astype(np.int32), "pid")
+    assert_bitwise(keys_b.cpu().numpy(), keys_a.cpu().numpy()[src], "keys")
+
+
+def test_chain_reorder_every(device):
+    """PMChain(reorder_every=K): particles end in cell order with pid a permutation; the run
+    is deterministic; the last deposit is the oracle's for the final slot order (bitwise);
+    and the state, mapped back by pid, follows the unordered run to rounding level."""
+    announce("chain with reordering")
+    from galaxy_collision_amd.chain import PMChain
+    from galaxy_collision_amd.data import ParticleState
+    from galaxy_collision_amd.ic import plummer_ic
+    from galaxy_collision_amd.solver import make_solver
+
+    n, dx, dt = 32, 0.5, 0.02
+    icr = plummer_ic(20_000, total_mass=1.0, scale_a=1.5, centre=(8.0, 8.0, 8.0), seed=9)
+
+    def run(k):
+        parts = ParticleState(icr.n)
+        parts.load(icr.pos, icr.vel, icr.mass, icr.gid)
+        chain = PMChain(parts, n, dx, make_solver("multigrid", n, dx=dx, grav_constant=1.0),
+                        reorder_every=k)
+        chain.prime()
+        chain.run(4, dt)
+        chain.run(3, dt)
+        torch.cuda.synchronize()
+        return parts, chain
+
+    a, ca = run(2)
+    b, _ = run(2)
+    plain, _ = run(0)
+    pid = a.pid.to_numpy()
+    assert not np.array_equal(pid, np.arange(icr.n)), "no reorder happened"
+    assert np.array_equal(np.sort(pid), np.arange(icr.n))
+    assert_bitwise(a.positions(), b.positions(), "deterministic positions")
+    assert_bitwise(a.velocities(), b.velocities(), "deterministic velocities")
+    rho_ref = O.deposit_cic(a.positions(), a.mass.to_numpy(), n, dx)[0]
+    assert_bitwise(ca.grid.rho.to_numpy(), rho_ref, "rho vs oracle in the reordered slots")
+    back = np.argsort(pid)
+    assert_bitwise(a.mass.to_numpy()[back], icr.mass.astype(F32), "mass by pid")
+    assert_bitwise(a.gid.to_numpy()[back], icr.gid.astype(np.int32), "gid by pid")
+    np.testing.assert_allclose(a.positions()[back], plain.positions(), rtol=0, atol=1e-4)
+    np.testing.assert_allclose(a.velocities()[back], plain.velocities(), rtol=0, atol=1e-3)
