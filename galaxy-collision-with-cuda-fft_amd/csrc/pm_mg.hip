@@ -587,9 +587,29 @@ __global__ void __launch_bounds__(256) residual64_kernel(const float* __restrict
 }
 
 // phi += e (e is 0 on the faces); partial sums of e^2 and phi^2 (float64).
+//
+// Carried guess (the extrapolated warm start without its own pass): the next solve's guess
+// 2*phi_t - phi_(t-1) lives in `guess`, and the correction updates it too.
+//   kCarryNone:  phi = phi + e
+//   kCarryFirst: phi = guess + e (the guess this solve started from); guess = 2*phi - old phi
+//   kCarryMore:  phi = phi + e; guess = guess + 2*e (a further correction of the same solve)
+enum { kCarryNone = 0, kCarryFirst = 1, kCarryMore = 2 };
+
+template <int CARRY>
+__device__ __forceinline__ float carry_update(float old, float& g, float d) {
+  if (CARRY == kCarryFirst) {
+    const float p = g + d;
+    g = 2.0f * p - old;
+    return p;
+  }
+  if (CARRY == kCarryMore) g = g + 2.0f * d;
+  return old + d;
+}
+
+template <int CARRY>
 __global__ void __launch_bounds__(256) update_kernel(float* __restrict__ phi,
                                                      const float* __restrict__ e,
-                                                     int64_t count,
+                                                     float* __restrict__ guess, int64_t count,
                                                      double* __restrict__ partials) {
   __shared__ double s_red[4 * 2];
   double v[2] = {0.0, 0.0};
@@ -597,8 +617,10 @@ __global__ void __launch_bounds__(256) update_kernel(float* __restrict__ phi,
   pm::block_range(count, r_begin, r_end);
   for (int64_t c = r_begin + threadIdx.x; c < r_end; c += blockDim.x) {
     const float d = e[c];
-    const float p = phi[c] + d;
+    float g = CARRY != kCarryNone ? guess[c] : 0.0f;
+    const float p = carry_update<CARRY>(phi[c], g, d);
     phi[c] = p;
+    if (CARRY != kCarryNone) guess[c] = g;
     v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
     v[1] = v[1] + static_cast<double>(p) * static_cast<double>(p);
   }
@@ -706,9 +728,11 @@ __global__ void __launch_bounds__(256) prolong_s_kernel(const float* __restrict_
 
 // The last colour-1 pass fused with update_kernel: phi += e over every node (e = 0 on the
 // faces), partial sums of e^2 and phi^2.
+template <int CARRY>
 __global__ void __launch_bounds__(256) black_update_kernel(const float* __restrict__ e,
                                                            const float* __restrict__ r,
-                                                           float* __restrict__ phi, int n,
+                                                           float* __restrict__ phi,
+                                                           float* __restrict__ guess, int n,
                                                            int hn, float h2,
                                                            double* __restrict__ partials) {
   __shared__ double s_red[4 * 2];
@@ -723,8 +747,10 @@ __global__ void __launch_bounds__(256) black_update_kernel(const float* __restri
     const int k0 = 2 * q.kk + ((q.i + q.j) & 1);
     if (k0 < n) {
       const float d = e[t];
-      const float p = phi[row + k0] + d;
+      float g = CARRY != kCarryNone ? guess[row + k0] : 0.0f;
+      const float p = carry_update<CARRY>(phi[row + k0], g, d);
       phi[row + k0] = p;
+      if (CARRY != kCarryNone) guess[row + k0] = g;
       v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
       v[1] = v[1] + static_cast<double>(p) * static_cast<double>(p);
     }
@@ -733,8 +759,10 @@ __global__ void __launch_bounds__(256) black_update_kernel(const float* __restri
       float d = 0.0f;
       if (q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2 && k1 >= 1 && k1 <= n - 2)
         d = (split_nbr_sum(e, t, q.kk, k1, nh, hn) - h2 * r[plane + t]) / 6.0f;
-      const float p = phi[row + k1] + d;
+      float g = CARRY != kCarryNone ? guess[row + k1] : 0.0f;
+      const float p = carry_update<CARRY>(phi[row + k1], g, d);
       phi[row + k1] = p;
+      if (CARRY != kCarryNone) guess[row + k1] = g;
       v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
       v[1] = v[1] + static_cast<double>(p) * static_cast<double>(p);
     }
@@ -1007,9 +1035,11 @@ __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restr
 }
 
 // black_update_kernel, 4 positions of both colours (8 natural nodes) per thread.
+template <int CARRY>
 __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restrict__ e,
                                                             const float* __restrict__ r,
-                                                            float* __restrict__ phi, int n,
+                                                            float* __restrict__ phi,
+                                                            float* __restrict__ guess, int n,
                                                             int hn, float h2,
                                                             double* __restrict__ partials) {
   __shared__ double s_red[4 * 2];
@@ -1036,33 +1066,45 @@ __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restr
         if (k >= 1 && k <= n - 2) d1[k4] = (s[k4] - h2 * at4(rv, k4)) / 6.0f;
       }
     }
-    float* dst = phi + (static_cast<int64_t>(q.i) * n + q.j) * n + 2 * q.kk0;
-    float ph[8];
+    const int64_t row0 = (static_cast<int64_t>(q.i) * n + q.j) * n + 2 * q.kk0;
+    float* dst = phi + row0;
+    float* gdst = CARRY != kCarryNone ? guess + row0 : nullptr;
+    float ph[8], gv[8];
     const bool full = 2 * q.kk0 + 8 <= n;
-    if (full) {
-      const float4 a = *reinterpret_cast<const float4*>(dst);
-      const float4 b = *reinterpret_cast<const float4*>(dst + 4);
-      ph[0] = a.x; ph[1] = a.y; ph[2] = a.z; ph[3] = a.w;
-      ph[4] = b.x; ph[5] = b.y; ph[6] = b.z; ph[7] = b.w;
-    } else {
-      for (int k = 0; k < 8; ++k) ph[k] = 2 * q.kk0 + k < n ? dst[k] : 0.0f;
-    }
+    auto load8 = [&](float* out, const float* src) {
+      if (full) {
+        const float4 a = *reinterpret_cast<const float4*>(src);
+        const float4 b = *reinterpret_cast<const float4*>(src + 4);
+        out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+        out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
+      } else {
+        for (int k = 0; k < 8; ++k) out[k] = 2 * q.kk0 + k < n ? src[k] : 0.0f;
+      }
+    };
+    auto store8 = [&](float* out, const float* val) {
+      if (full) {
+        *reinterpret_cast<float4*>(out) = make_float4(val[0], val[1], val[2], val[3]);
+        *reinterpret_cast<float4*>(out + 4) = make_float4(val[4], val[5], val[6], val[7]);
+      } else {
+        for (int k = 0; k < 8 && 2 * q.kk0 + k < n; ++k) out[k] = val[k];
+      }
+    };
+    load8(ph, dst);
+    if (CARRY != kCarryNone) load8(gv, gdst);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       if (2 * q.kk0 + k >= n) continue;
       const int k4 = k >> 1;
       const float d = ((k & 1) == p0) ? at4(e0, k4) : d1[k4];
-      const float pv = ph[k] + d;
+      float g = CARRY != kCarryNone ? gv[k] : 0.0f;
+      const float pv = carry_update<CARRY>(ph[k], g, d);
       ph[k] = pv;
+      if (CARRY != kCarryNone) gv[k] = g;
       v[0] = v[0] + static_cast<double>(d) * static_cast<double>(d);
       v[1] = v[1] + static_cast<double>(pv) * static_cast<double>(pv);
     }
-    if (full) {
-      *reinterpret_cast<float4*>(dst) = make_float4(ph[0], ph[1], ph[2], ph[3]);
-      *reinterpret_cast<float4*>(dst + 4) = make_float4(ph[4], ph[5], ph[6], ph[7]);
-    } else {
-      for (int k = 0; k < 8 && 2 * q.kk0 + k < n; ++k) dst[k] = ph[k];
-    }
+    store8(dst, ph);
+    if (CARRY != kCarryNone) store8(gdst, gv);
   }
   pm::block_sum<2>(v, s_red);
   if (threadIdx.x == 0) {  // two columns of gridDim.x partials (pm::fold_columns_kernel)
@@ -1140,6 +1182,11 @@ struct pm_mg {
   float* hist = nullptr;      // phi of the previous warm solve (the time history)
   float* cand = nullptr;      // extrapolated guess of the current solve
   bool hist_valid = false;
+  // carried guess (EXTRAPOLATE without pairs, MIXED): the next solve's guess
+  // 2*phi_t - phi_(t-1), written by the updates of the solve before (no extrapolation pass)
+  float* guess = nullptr;
+  bool guess_valid = false;
+  const float* guess_phi = nullptr;  // the phi buffer the carried guess belongs to
   int warm_pool = 0;          // network pool factor the buffers below are sized for
   float* pooled = nullptr;    // [R^3][3] pooled input channels
   uint16_t* net_in = nullptr; // [R^3][8] bf16 network input
@@ -1333,7 +1380,7 @@ unsigned split_update_blocks(const pm_mg* mg, const float* phi, double** partial
 
 // The finest level of the MIXED correction in split layout: e0s = V(r32s) from a zero
 // start (pre/post sweeps >= 1), then phi += e0s with the norms in partials.
-pm_status vcycle_split0(pm_mg* mg, float* phi, hipStream_t s) {
+pm_status vcycle_split0(pm_mg* mg, float* phi, int carry, hipStream_t s) {
   const Level& L = mg->lv[0];
   const Level& C = mg->lv[1];
   const int n = L.n, hn = mg->hn;
@@ -1392,13 +1439,27 @@ pm_status vcycle_split0(pm_mg* mg, float* phi, hipStream_t s) {
     if (q + 1 < mg->d.post_sweeps && (st = pass(1, false))) return st;
   }
   {
-    PM_TIMED("mg.black_update", 12.0 * nodes, s);
+    PM_TIMED("mg.black_update", (carry != kCarryNone ? 20.0 : 12.0) * nodes, s);
     double* partials;
     const unsigned blocks = split_update_blocks(mg, phi, &partials);
-    if (quad && phi_aligned)
-      black_update4_kernel<<<blocks, pm::kBlock, 0, s>>>(e, r, phi, n, hn, L.h2, partials);
+    float* g = mg->guess;
+    const bool vec = quad && phi_aligned && (carry == kCarryNone ||
+                                             (reinterpret_cast<uintptr_t>(g) & 15u) == 0);
+#define PM_BLACK_UPDATE(K, C)                                                        \
+  K<C><<<blocks, pm::kBlock, 0, s>>>(e, r, phi, g, n, hn, L.h2, partials)
+    if (vec && carry == kCarryFirst)
+      PM_BLACK_UPDATE(black_update4_kernel, kCarryFirst);
+    else if (vec && carry == kCarryMore)
+      PM_BLACK_UPDATE(black_update4_kernel, kCarryMore);
+    else if (vec)
+      PM_BLACK_UPDATE(black_update4_kernel, kCarryNone);
+    else if (carry == kCarryFirst)
+      PM_BLACK_UPDATE(black_update_kernel, kCarryFirst);
+    else if (carry == kCarryMore)
+      PM_BLACK_UPDATE(black_update_kernel, kCarryMore);
     else
-      black_update_kernel<<<blocks, pm::kBlock, 0, s>>>(e, r, phi, n, hn, L.h2, partials);
+      PM_BLACK_UPDATE(black_update_kernel, kCarryNone);
+#undef PM_BLACK_UPDATE
     PM_CHECK_LAUNCH("black_update_kernel");
   }
   return PM_OK;
@@ -1464,15 +1525,19 @@ pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split 
 
 // The launches of one V-cycle step. kCycleCorrect (MIXED): e0 = 0, e0 = V(r32), phi += e0,
 // red[0] = ||e0||^2, red[1] = ||phi||^2. kCyclePlain (PLAIN): phi = V(phi) on f0.
-enum { kCycleCorrect = 0, kCyclePlain = 1 };
-constexpr size_t kMaxGraphs = 4;
+// kCycleCarryFirst / kCycleCarryMore: kCycleCorrect with the carried guess (update_kernel).
+enum { kCycleCorrect = 0, kCyclePlain = 1, kCycleCarryFirst = 2, kCycleCarryMore = 3 };
+constexpr size_t kMaxGraphs = 6;
 
 pm_status enqueue_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
   Level& L0 = mg->lv[0];
   pm_status st;
   if (kind == kCyclePlain) return vcycle(mg, 0, s);
+  const int carry = kind == kCycleCarryFirst  ? kCarryFirst
+                    : kind == kCycleCarryMore ? kCarryMore
+                                              : kCarryNone;
   if (mg->split_active()) {
-    if ((st = vcycle_split0(mg, phi, s))) return st;
+    if ((st = vcycle_split0(mg, phi, carry, s))) return st;
     double* partials;
     const unsigned blocks = split_update_blocks(mg, phi, &partials);
     if (partials == mg->partials_q)  // black_update4_kernel's columns
@@ -1488,8 +1553,16 @@ pm_status enqueue_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
   if ((st = vcycle(mg, 0, s))) return st;
   const unsigned b = pm::red_blocks(L0.nodes());
   {
-    PM_TIMED("mg.update", kUpdBytes * L0.nodes(), s);
-    update_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->e0, L0.nodes(), mg->partials);
+    PM_TIMED("mg.update", (carry != kCarryNone ? kUpdBytes + 8.0 : kUpdBytes) * L0.nodes(), s);
+    if (carry == kCarryFirst)
+      update_kernel<kCarryFirst><<<b, pm::kBlock, 0, s>>>(phi, mg->e0, mg->guess, L0.nodes(),
+                                                          mg->partials);
+    else if (carry == kCarryMore)
+      update_kernel<kCarryMore><<<b, pm::kBlock, 0, s>>>(phi, mg->e0, mg->guess, L0.nodes(),
+                                                         mg->partials);
+    else
+      update_kernel<kCarryNone><<<b, pm::kBlock, 0, s>>>(phi, mg->e0, nullptr, L0.nodes(),
+                                                         mg->partials);
     PM_CHECK_LAUNCH("update_kernel");
   }
   pm::fold_partials_kernel<2><<<1, pm::kBlock, 0, s>>>(mg->partials, static_cast<int>(b),
@@ -1537,9 +1610,10 @@ pm_status run_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
 
 // One defect-correction iteration: e = V(r32); phi += e. Returns (synchronizing) red[0..3]:
 // ||e||^2, ||phi||^2, the squared defect norm this correction started from, ||f||^2.
-pm_status mixed_correct(pm_mg* mg, float* phi, double v[4], hipStream_t s) {
+pm_status mixed_correct(pm_mg* mg, float* phi, double v[4], hipStream_t s,
+                        int kind = kCycleCorrect) {
   pm_status st;
-  if ((st = run_cycle(mg, kCycleCorrect, phi, s))) return st;
+  if ((st = run_cycle(mg, kind, phi, s))) return st;
   return read_scalars(mg, mg->red, 4, v, s);
 }
 
@@ -1572,9 +1646,10 @@ pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStr
 
 // The V-cycle loop of a solve whose setup has run (PLAIN: reference stop rule; MIXED:
 // defect correction with the error-estimate stop). Fills stats. residual_ready: MIXED only,
-// r32 already holds the defect of phi and red[2] its squared norm.
+// r32 already holds the defect of phi and red[2] its squared norm. carry: MIXED only, the
+// solve starts from mg->guess and its corrections carry the next guess (kCycleCarry*).
 pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s,
-                        bool residual_ready = false) {
+                        bool residual_ready = false, bool carry = false) {
   Level& L0 = mg->lv[0];
   const bool mixed = mg->d.mode == PM_MG_MIXED;
   pm_status st;
@@ -1604,7 +1679,7 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
     L0.rhs = split ? mg->r32s : mg->r32;
     mg->last_phi = phi;
     if (!residual_ready) {
-      if ((st = mixed_residual(mg, phi, s, split))) return st;
+      if ((st = mixed_residual(mg, carry ? mg->guess : phi, s, split))) return st;
     } else if (split) {
       PM_TIMED("mg.to_split", 8.0 * L0.nodes(), s);
       to_split_kernel<<<pm::blocks_for(L0.nodes()), pm::kBlock, 0, s>>>(mg->r32, mg->r32s,
@@ -1615,7 +1690,8 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
     double v[4] = {0.0, 0.0, -1.0, 0.0};
     bool read = false;
     for (int c = 0; c < mg->d.n_cycles; ++c) {
-      if ((st = mixed_correct(mg, phi, v, s))) return st;
+      const int kind = !carry ? kCycleCorrect : c == 0 ? kCycleCarryFirst : kCycleCarryMore;
+      if ((st = mixed_correct(mg, phi, v, s, kind))) return st;
       read = true;
       const double e_norm = std::sqrt(v[0]);
       const double phi_norm = std::sqrt(v[1]);
@@ -1630,6 +1706,10 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
       }
       if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s, split))) return st;
     }
+    // no correction ran: phi is the guess (as an in-place guess would leave it)
+    if (!read && carry)
+      PM_TRY_HIP(hipMemcpyAsync(phi, mg->guess, sizeof(float) * L0.nodes(),
+                                hipMemcpyDeviceToDevice, s));
     // the last correction's read holds the defect it started from and ||f||^2
     if (!read && (st = read_scalars(mg, mg->red, 4, v, s))) return st;
     resid = std::sqrt(v[2]);
@@ -1769,6 +1849,7 @@ pm_status pm_mg_solve(pm_mg* mg, const float* rho, float* phi, int32_t warm,
   PM_REQUIRE(mg && rho && phi, "pm_mg_solve: null pointer");
   PM_REQUIRE(rho != phi, "pm_mg_solve: rho and phi must not alias");
   mg->hist_valid = false;  // a solve outside the warm-start sequence ends its history
+  mg->guess_valid = false;
   pm_status st;
   if ((st = solve_setup(mg, rho, phi, warm != 0, s))) return st;
   return solve_iterate(mg, phi, stats, s);
@@ -1777,6 +1858,7 @@ pm_status pm_mg_solve(pm_mg* mg, const float* rho, float* phi, int32_t warm,
 pm_status pm_mg_warm_reset(pm_mg* mg) {
   PM_REQUIRE(mg, "pm_mg_warm_reset: null handle");
   mg->hist_valid = false;
+  mg->guess_valid = false;
   return PM_OK;
 }
 
@@ -1833,6 +1915,27 @@ pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* 
     mg->warm_pool = w->pool;
   }
 
+  // Extrapolation without training pairs on the MIXED solver: the guess 2*phi_t - phi_(t-1)
+  // was written by the previous solve's last update (bitwise the extrapolation pass's
+  // guess after a one-cycle solve), so this solve reads it where it lies.
+  if (w->mode == PM_WARM_EXTRAPOLATE && !pairs && mg->d.mode == PM_MG_MIXED) {
+    if (!mg->guess && (st = dalloc(mg, reinterpret_cast<void**>(&mg->guess),
+                                   sizeof(float) * nodes)))
+      return st;
+    const bool have = mg->guess_valid && mg->guess_phi == phi;
+    if (!have)  // first solve of a sequence: phi itself is the guess
+      PM_TRY_HIP(hipMemcpyAsync(mg->guess, phi, sizeof(float) * nodes, hipMemcpyDeviceToDevice,
+                                s));
+    mg->hist_valid = false;  // the explicit time history is not kept alongside
+    mg->guess_valid = false;
+    if ((st = solve_setup(mg, rho, mg->guess, true, s))) return st;  // faces into the guess
+    w->extrapolated = have ? 1 : 0;
+    if ((st = solve_iterate(mg, phi, stats, s, false, true))) return st;
+    mg->guess_valid = mg->d.n_cycles > 0;
+    mg->guess_phi = phi;
+    return PM_OK;
+  }
+  mg->guess_valid = false;
   if ((st = solve_setup(mg, rho, phi, true, s))) return st;
   const bool extrap = w->mode != PM_WARM_PREVIOUS && mg->hist_valid;
   if (!extrap) {
@@ -1873,7 +1976,8 @@ pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* 
       const unsigned b = pm::red_blocks(nodes);
       {
         PM_TIMED("mg.update", kUpdBytes * nodes, s);
-        update_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->e0, nodes, mg->partials);
+        update_kernel<kCarryNone><<<b, pm::kBlock, 0, s>>>(phi, mg->e0, nullptr, nodes,
+                                                           mg->partials);
         PM_CHECK_LAUNCH("update_kernel");
       }
       std::swap(mg->r32, lv0.res);  // the defect of the corrected phi

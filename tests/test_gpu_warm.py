@@ -252,3 +252,33 @@ def test_split_fine_level_odd_and_even_grids(device, n):
         out.append((phi.clone(), c0, s.last_cycles))
     assert torch.equal(out[0][0], out[1][0])
     assert out[0][1:] == out[1][1:]
+
+
+def test_carried_guess_equals_extrapolation_pass(device, seq_b):
+    """The extrapolated warm start without its own pass: each MIXED solve's last update
+    writes the next guess 2*phi_t - phi_(t-1), and the next solve starts from it. Over a
+    sequence of one-cycle solves phi is bitwise that of the explicit extrapolation pass
+    (run here by the learned mode with a judge that rejects every correction), in both
+    fine-level layouts."""
+    announce("carried guess vs extrapolation pass")
+    from galaxy_collision_amd.solver.multigrid import MixedPrecisionMultigridSolver
+    from galaxy_collision_amd.warmstart import WarmStartNet
+
+    rhos, dx = seq_b
+    net = WarmStartNet(N_GRID // 4, seed=3)
+    for split in (True, False):
+        kw = dict(dx=dx, grav_constant=1.0, n_cycles=1, split_fine=split)
+        carried = MixedPrecisionMultigridSolver(N_GRID, warm="extrapolate", **kw)
+        explicit = MixedPrecisionMultigridSolver(N_GRID, warm="learned", net=net, pool=4,
+                                                 fallback_ratio=1e-30, **kw)
+        pa = torch.zeros((N_GRID,) * 3, device=device)
+        pb = torch.zeros((N_GRID,) * 3, device=device)
+        for t, rho in enumerate(rhos[:8]):
+            carried.solve(rho, pa, warm_start=t > 0)
+            explicit.solve(rho, pb, warm_start=t > 0)
+            torch.cuda.synchronize()
+            assert carried.last_cycles == explicit.last_cycles == 1
+            if t >= 2:
+                assert carried.last_warm["extrapolated"] and explicit.last_warm["extrapolated"]
+                assert not explicit.last_warm["used_net"]
+            assert torch.equal(pa, pb), (split, t)
