@@ -46,7 +46,7 @@ WORKLOADS = {
                         solver="multigrid"),
     "collision100m": dict(kind="two_galaxy", n_per=50_000_000, grid=256, box=40.0, dt=0.01,
                           solver="multigrid_mp", warm="extrapolate", masses=(1.0, 0.5),
-                          tol=2e-6, sweeps=(1, 1), reorder_every=4),
+                          tol=2e-6, sweeps=(1, 1), reorder_every=8),
     "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
 }
 

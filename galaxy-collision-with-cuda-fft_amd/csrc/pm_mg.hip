@@ -558,7 +558,8 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
 // r = f - lap(phi) evaluated in float64 from the float32 iterate (interior; 0 on the
 // boundary); r32 = (float)r; partial sums of r^2 (float64).
 __global__ void __launch_bounds__(256) residual64_kernel(const float* __restrict__ phi,
-                                                         const float* __restrict__ f, int n,
+                                                         const float* __restrict__ rho,
+                                                         float factor, int n,
                                                          double inv_h2,
                                                          float* __restrict__ r32,
                                                          double* __restrict__ partials) {
@@ -577,7 +578,7 @@ __global__ void __launch_bounds__(256) residual64_kernel(const float* __restrict
                          static_cast<double>(phi[c + n]) + static_cast<double>(phi[c - n]) +
                          static_cast<double>(phi[c + 1]) + static_cast<double>(phi[c - 1]);
       const double lap = (sum - 6.0 * static_cast<double>(phi[c])) * inv_h2;
-      r = static_cast<double>(f[c]) - lap;
+      r = static_cast<double>(factor * rho[c]) - lap;
     }
     r32[c] = static_cast<float>(r);
     v[0] = v[0] + r * r;
@@ -776,7 +777,8 @@ __global__ void __launch_bounds__(256) black_update_kernel(const float* __restri
 
 // residual64_kernel writing r32 in split layout (same sums, same partials).
 __global__ void __launch_bounds__(256) residual64_s_kernel(const float* __restrict__ phi,
-                                                           const float* __restrict__ f, int n,
+                                                           const float* __restrict__ rho,
+                                                           float factor, int n,
                                                            int hn, double inv_h2,
                                                            float* __restrict__ r32s,
                                                            double* __restrict__ partials) {
@@ -796,7 +798,7 @@ __global__ void __launch_bounds__(256) residual64_s_kernel(const float* __restri
                          static_cast<double>(phi[c + n]) + static_cast<double>(phi[c - n]) +
                          static_cast<double>(phi[c + 1]) + static_cast<double>(phi[c - 1]);
       const double lap = (sum - 6.0 * static_cast<double>(phi[c])) * inv_h2;
-      r = static_cast<double>(f[c]) - lap;
+      r = static_cast<double>(factor * rho[c]) - lap;
     }
     r32s[static_cast<int64_t>((i + j + k) & 1) * plane +
          (static_cast<int64_t>(i) * n + j) * hn + (k >> 1)] = static_cast<float>(r);
@@ -974,7 +976,8 @@ __global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict
 // residual64_s_kernel, 8 consecutive natural nodes per thread (n % 8 == 0): the r values
 // are the same; the squared norm is summed in another order.
 __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restrict__ phi,
-                                                            const float* __restrict__ f, int n,
+                                                            const float* __restrict__ rho,
+                                                            float factor, int n,
                                                             int hn, double inv_h2,
                                                             float* __restrict__ r32s,
                                                             double* __restrict__ partials) {
@@ -1006,7 +1009,9 @@ __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restr
       load8(im, phi, c0 - n2);
       load8(jp, phi, c0 + n);
       load8(jm, phi, c0 - n);
-      load8(fv, f, c0);
+      load8(fv, rho, c0);
+#pragma unroll
+      for (int x = 0; x < 8; ++x) fv[x] = factor * fv[x];  // f = 4*pi*G*rho, as the setup
 #pragma unroll
       for (int x = 0; x < 8; ++x) {
         const int k = 2 * q.kk0 + x;
@@ -1167,7 +1172,9 @@ constexpr double kUpdBytes = 12.0;       // read phi + e, write phi
 struct pm_mg {
   pm_mg_desc d{};
   std::vector<Level> lv;
-  float* f0 = nullptr;        // 4*pi*G*rho, float32 (both modes)
+  float* f0 = nullptr;        // 4*pi*G*rho, float32 (PLAIN, and the network's input)
+  const float* cur_rho = nullptr;  // rho of the last setup (the float64 defect scales it)
+  float factor = 0.0f;             // 4*pi*G as float32
   float* e0 = nullptr;        // MIXED: finest correction
   float* r32 = nullptr;       // MIXED: finest float32 defect
   double* partials = nullptr;
@@ -1496,20 +1503,24 @@ pm_status plain_residual_norm(pm_mg* mg, const float* phi, double* out, hipStrea
 // its sum of squares lands in red[2] (no sync).
 pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split = false) {
   const Level& L = mg->lv[0];
-  // 8 nodes per thread on split grids with n % 8 == 0 and 16-byte aligned phi and f
-  const bool quad = split && mg->quad_blocks && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0;
+  // 8 nodes per thread on split grids with n % 8 == 0 and 16-byte aligned phi and rho
+  const bool quad = split && mg->quad_blocks && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0 &&
+                    (reinterpret_cast<uintptr_t>(mg->cur_rho) & 15u) == 0;
   const unsigned b = quad ? mg->quad_blocks : pm::red_blocks(L.nodes());
   double* partials = quad ? mg->partials_q : mg->partials;
   {
     PM_TIMED("mg.residual64", kRes64Bytes * L.nodes(), s);
     if (quad)
-      residual64_s4_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, mg->hn, 1.0 / (L.h * L.h),
+      residual64_s4_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->cur_rho, mg->factor, L.n, mg->hn,
+                                                     1.0 / (L.h * L.h),
                                                      mg->r32s, partials);
     else if (split)
-      residual64_s_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, mg->hn, 1.0 / (L.h * L.h),
+      residual64_s_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->cur_rho, mg->factor, L.n, mg->hn,
+                                                    1.0 / (L.h * L.h),
                                                     mg->r32s, mg->partials);
     else
-      residual64_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->f0, L.n, 1.0 / (L.h * L.h),
+      residual64_kernel<<<b, pm::kBlock, 0, s>>>(phi, mg->cur_rho, mg->factor, L.n,
+                                                  1.0 / (L.h * L.h),
                                                   mg->r32, mg->partials);
     PM_CHECK_LAUNCH("residual64_kernel");
   }
@@ -1618,8 +1629,10 @@ pm_status mixed_correct(pm_mg* mg, float* phi, double v[4], hipStream_t s,
 }
 
 // Moments, f = 4*pi*G*rho, the initial guess (zero unless warm) with multipole faces, and
-// ||f|| (into red[3]). Enqueued only.
-pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStream_t s) {
+// ||f|| (into red[3]). Enqueued only. f is stored in f0 for the PLAIN cycle and the
+// network's input (store_f); the MIXED defect scales rho itself with the same float product.
+pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStream_t s,
+                      bool store_f) {
   Level& L0 = mg->lv[0];
   const int n = L0.n;
   const int64_t nodes = L0.nodes();
@@ -1629,9 +1642,12 @@ pm_status solve_setup(pm_mg* mg, const float* rho, float* phi, bool warm, hipStr
   // same pass over rho
   {
     const float factor = static_cast<float>(4.0 * kPi * mg->d.grav);
-    PM_TIMED("mg.moments", 8.0 * nodes, s);
+    mg->cur_rho = rho;
+    mg->factor = factor;
+    PM_TIMED("mg.moments", (store_f ? 8.0 : 4.0) * nodes, s);
     if ((st = pm::launch_moments(rho, n, mg->d.dx, mg->d.grav, mg->partials, mg->moments,
-                                 mg->face_params, s, mg->f0, factor, mg->red + 3)))
+                                 mg->face_params, s, store_f ? mg->f0 : nullptr, factor,
+                                 mg->red + 3)))
       return st;
   }
   if (!warm && (st = fill(phi, nodes, 0.0f, s))) return st;
@@ -1851,7 +1867,7 @@ pm_status pm_mg_solve(pm_mg* mg, const float* rho, float* phi, int32_t warm,
   mg->hist_valid = false;  // a solve outside the warm-start sequence ends its history
   mg->guess_valid = false;
   pm_status st;
-  if ((st = solve_setup(mg, rho, phi, warm != 0, s))) return st;
+  if ((st = solve_setup(mg, rho, phi, warm != 0, s, mg->d.mode == PM_MG_PLAIN))) return st;
   return solve_iterate(mg, phi, stats, s);
 }
 
@@ -1928,7 +1944,7 @@ pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* 
                                 s));
     mg->hist_valid = false;  // the explicit time history is not kept alongside
     mg->guess_valid = false;
-    if ((st = solve_setup(mg, rho, mg->guess, true, s))) return st;  // faces into the guess
+    if ((st = solve_setup(mg, rho, mg->guess, true, s, false))) return st;  // faces into it
     w->extrapolated = have ? 1 : 0;
     if ((st = solve_iterate(mg, phi, stats, s, false, true))) return st;
     mg->guess_valid = mg->d.n_cycles > 0;
@@ -1936,7 +1952,8 @@ pm_status pm_mg_solve_warm(pm_mg* mg, const float* rho, float* phi, pm_mg_warm* 
     return PM_OK;
   }
   mg->guess_valid = false;
-  if ((st = solve_setup(mg, rho, phi, true, s))) return st;
+  if ((st = solve_setup(mg, rho, phi, true, s, mg->d.mode == PM_MG_PLAIN || want_net_grid)))
+    return st;
   const bool extrap = w->mode != PM_WARM_PREVIOUS && mg->hist_valid;
   if (!extrap) {
     // first solve of a sequence (or PREVIOUS mode): phi itself is the guess; remember it

@@ -106,8 +106,8 @@ pm_status launch_sumsq_f64(const double* x, int64_t count, double* partials, dou
                            hipStream_t stream);
 // Ten raw moments of rho -> sums_dev[0..9]; if face_params is non-null also writes the
 // eleven float32 face parameters [G, M, cx, cy, cz, sxx, syy, szz, sxy, sxz, syz]
-// computed from them exactly as reference multigrid.py:364-389 does on the host. With f,
-// the same pass writes f = factor * rho and, with f_sumsq_dev, the sum of f^2 there.
+// computed from them exactly as reference multigrid.py:364-389 does on the host. With
+// f_sumsq_dev the same pass sums f^2 for f = factor * rho into it, and with f also writes f.
 pm_status launch_moments(const float* rho, int n, float dx, double grav, double* partials,
                          double* sums_dev, float* face_params, hipStream_t stream,
                          float* f = nullptr, float factor = 0.0f, double* f_sumsq_dev = nullptr);

@@ -38,9 +38,9 @@ __global__ void __launch_bounds__(256) sumsq_f64_kernel(const double* __restrict
 }
 
 // Ten raw moments (reference multigrid.py:133-178): m = rho*dx^3, (x,y,z) = (i,j,k)*dx.
-// With F, the same pass also writes f = factor * rho (multigrid.py:249-252) and sums f^2
-// into partial slot 10.
-template <bool F>
+// With F, the same pass also sums f^2 (f = factor * rho, multigrid.py:249-252) into partial
+// slot 10 and, with STORE, writes f.
+template <bool F, bool STORE>
 __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ rho, int n,
                                                       double dxd, double vol,
                                                       double* __restrict__ partials,
@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ 
     const float r = rho[idx];
     if (F) {
       const float fv = factor * r;
-      f[idx] = fv;
+      if (STORE) f[idx] = fv;
       const double t = static_cast<double>(fv);
       v[Q - 1] = v[Q - 1] + t * t;
     }
@@ -266,7 +266,7 @@ pm_status red_scratch(RedScratch** out) {
 
 // moments_kernel with 4 consecutive nodes (one row, n % 4 == 0) per iteration and 16-byte
 // loads and stores: a quarter of the dependent iterations per thread.
-template <bool F>
+template <bool F, bool STORE>
 __global__ void __launch_bounds__(256) moments4_kernel(const float* __restrict__ rho, int n,
                                                        double dxd, double vol,
                                                        double* __restrict__ partials,
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(256) moments4_kernel(const float* __restrict__
     const float r[4] = {r4.x, r4.y, r4.z, r4.w};
     if (F) {
       const float4 f4 = make_float4(factor * r[0], factor * r[1], factor * r[2], factor * r[3]);
-      *reinterpret_cast<float4*>(f + c) = f4;
+      if (STORE) *reinterpret_cast<float4*>(f + c) = f4;
       const float fv[4] = {f4.x, f4.y, f4.z, f4.w};
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -346,21 +346,25 @@ pm_status launch_moments(const float* rho, int n, float dx, double grav, double*
   const double dxd = static_cast<double>(dx);
   const bool vec = n % 4 == 0 && (reinterpret_cast<uintptr_t>(rho) & 15u) == 0 &&
                    (!f || (reinterpret_cast<uintptr_t>(f) & 15u) == 0);
-  if (vec && f) {
-    moments4_kernel<true><<<b, kBlock, 0, stream>>>(rho, n, dxd, dxd * dxd * dxd, partials, f,
-                                                    factor);
-  } else if (vec) {
-    moments4_kernel<false><<<b, kBlock, 0, stream>>>(rho, n, dxd, dxd * dxd * dxd, partials,
-                                                     nullptr, 0.0f);
-  } else if (f) {
-    moments_kernel<true><<<b, kBlock, 0, stream>>>(rho, n, dxd, dxd * dxd * dxd, partials, f,
-                                                   factor);
-  } else {
-    moments_kernel<false><<<b, kBlock, 0, stream>>>(rho, n, dxd, dxd * dxd * dxd, partials,
-                                                    nullptr, 0.0f);
-  }
+  const bool sq = f_sumsq_dev != nullptr;
+  const double vol = dxd * dxd * dxd;
+#define PM_MOMENTS(K, F_, S_) \
+  K<F_, S_><<<b, kBlock, 0, stream>>>(rho, n, dxd, vol, partials, f, factor)
+  if (vec && sq && f)
+    PM_MOMENTS(moments4_kernel, true, true);
+  else if (vec && sq)
+    PM_MOMENTS(moments4_kernel, true, false);
+  else if (vec)
+    PM_MOMENTS(moments4_kernel, false, false);
+  else if (sq && f)
+    PM_MOMENTS(moments_kernel, true, true);
+  else if (sq)
+    PM_MOMENTS(moments_kernel, true, false);
+  else
+    PM_MOMENTS(moments_kernel, false, false);
+#undef PM_MOMENTS
   PM_CHECK_LAUNCH("moments_kernel");
-  if (f)
+  if (sq)
     fold_moments_kernel<11><<<1, kBlock, 0, stream>>>(partials, static_cast<int>(b), grav,
                                                       sums_dev, f_sumsq_dev, face_params);
   else
