@@ -482,7 +482,7 @@ __device__ __forceinline__ int64_t cell_end(const uint32_t* __restrict__ skey, i
 // Cells whose pieces span chunks: small ones (<= kSmallPieces pieces) finish here with an
 // unrolled aligned pairwise tree; up to kWavePieces pieces are marked in mid_pieces[chunk]
 // for mid_cell_kernel (no shared counter: 10^4 such cells would serialize on one
-// device-scope atomic); larger ones are listed for big_cell_kernel.
+// device-scope atomic); larger ones are listed for big_batch_kernel.
 __global__ void __launch_bounds__(256) join_kernel(
     const uint32_t* __restrict__ skey, int64_t np, int64_t chunks,
     const int64_t* __restrict__ tail_slot, const uint8_t* __restrict__ head_end,
@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(256) join_kernel(
     const int64_t end = cell_end(skey, slot, np, key);
     pieces = (end - 1) / kChunk - ch + 1;
   }
-  if (pieces > kWavePieces) {  // rare: listed for big_cell_kernel
+  if (pieces > kWavePieces) {  // rare: listed for big_batch_kernel
     const uint32_t at = atomicAdd(big_count, 1u);
     big[at] = BigCell{slot, ch, pieces};
     return;
@@ -555,7 +555,7 @@ __device__ __forceinline__ void load_piece(const float* __restrict__ head,
 }
 
 // Cells of kSmallPieces+1 .. kWavePieces pieces: one wave each, the first stage of
-// big_cell_kernel's batch (4 pieces per lane, aligned shuffle tree), whose result for a
+// big_batch_kernel's batch (4 pieces per lane, aligned shuffle tree), whose result for a
 // batch of at most 256 pieces is the wave-0 value, so the sums are bitwise the same.
 __global__ void __launch_bounds__(256) mid_cell_kernel(
     const uint32_t* __restrict__ skey, const int64_t* __restrict__ tail_slot,
@@ -621,23 +621,61 @@ __global__ void __launch_bounds__(256) mid_cell_kernel(
 // same way. Batch results are merged by a binary-counter cascade (push; merge while the two
 // top partials cover equal piece counts) and a final right-nested combine, which is the
 // same aligned tree one level up.
-__global__ void __launch_bounds__(256) big_cell_kernel(
-    const uint32_t* __restrict__ skey, const BigCell* __restrict__ big,
-    const uint32_t* __restrict__ big_count, const float* __restrict__ head,
-    const float* __restrict__ tail, float* __restrict__ sums, int64_t cap,
-    int32_t* __restrict__ map) {
-  constexpr int kMaxDepth = 40;
+// Cells of more than kWavePieces pieces, in two kernels so that a few huge cells (the cores
+// of a galaxy at 100M particles hold tens of thousands of pieces each) spread over the
+// whole device. big_batch_kernel sums each aligned batch of kBigBatch pieces of every such
+// cell (4 pieces per lane, shuffle levels, the wave combine: the levels of the aligned
+// pairwise tree inside the batch), one workgroup per batch, and stores the batch sum in the
+// batch's first piece slot (tail[chunk] for batch 0, head[chunk + kBigBatch*b] after, read
+// by no one else). big_merge_kernel then folds a cell's batch sums in order with the binary
+// counter of the tree's upper levels (equal sizes merge, sizes double) and the final
+// right-to-left fold: the same float operations in the same order as one workgroup walking
+// the batches.
+__device__ __forceinline__ float* batch_slot(float* __restrict__ head, float* __restrict__ tail,
+                                             int64_t chunk, int64_t b) {
+  return b == 0 ? tail + chunk * 8 : head + (chunk + b * kBigBatch) * 8;
+}
+
+// big_off[i] = batches of the listed cells before cell i; big_off[count] = all batches.
+__global__ void __launch_bounds__(256) big_scan_kernel(const BigCell* __restrict__ big,
+                                                       const uint32_t* __restrict__ big_count,
+                                                       int64_t* __restrict__ big_off) {
+  __shared__ uint32_t s_wave[4];
+  const uint32_t count = *big_count;
+  int64_t running = 0;
+  for (uint32_t base = 0; base < count; base += 256) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t nb =
+        i < count ? static_cast<uint32_t>((big[i].pieces + kBigBatch - 1) / kBigBatch) : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(nb, total, s_wave);
+    if (i < count) big_off[i] = running + ex;
+    running += total;
+  }
+  if (threadIdx.x == 0) big_off[count] = running;
+}
+
+__global__ void __launch_bounds__(256) big_batch_kernel(
+    const BigCell* __restrict__ big, const uint32_t* __restrict__ big_count,
+    const int64_t* __restrict__ big_off, float* __restrict__ head, float* __restrict__ tail) {
   __shared__ float s_wave[4][8];
-  __shared__ float stack[kMaxDepth][8];
-  __shared__ int64_t stack_size[kMaxDepth];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const uint32_t count = *big_count;
-  for (uint32_t bcell = blockIdx.x; bcell < count; bcell += gridDim.x) {
-    const BigCell cell = big[bcell];
-    int depth = 0;
-    for (int64_t base = 0; base < cell.pieces; base += kBigBatch) {
+  if (count == 0) return;
+  const int64_t total = big_off[count];
+  for (int64_t f = blockIdx.x; f < total; f += gridDim.x) {
+    // the cell of batch f: the last i with big_off[i] <= f (offsets increase: nb >= 1)
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (big_off[mid] <= f) lo = mid; else hi = mid - 1;
+    }
+    const BigCell cell = big[lo];
+    const int64_t b = f - big_off[lo];
+    {
+      const int64_t base = b * kBigBatch;
       const int64_t m = cell.pieces - base < kBigBatch ? cell.pieces - base : kBigBatch;
       // lane holds pieces base + 256*wave + 4*lane + {0..3}
       const int64_t q0 = 256 * wave + 4 * lane;  // relative to base
@@ -675,35 +713,55 @@ __global__ void __launch_bounds__(256) big_cell_kernel(
 #pragma unroll
         for (int c = 0; c < 8; ++c) s_wave[wave][c] = v[c];
       }
-      __syncthreads();
-      int d = depth;
-      int64_t sz = m;
-      float val = 0.0f;
+      __syncthreads();  // every piece of the batch has been loaded
       if (tid < 8) {
         const int c = tid;
         float w0 = s_wave[0][c], w2 = s_wave[2][c];
         if (256 < m) w0 = w0 + s_wave[1][c];
         if (768 < m) w2 = w2 + s_wave[3][c];
-        val = 512 < m ? w0 + w2 : w0;
+        batch_slot(head, tail, cell.chunk, b)[c] = 512 < m ? w0 + w2 : w0;
       }
-      while (d > 0 && stack_size[d - 1] == sz) {
-        if (tid < 8) val = stack[d - 1][tid] + val;
+      __syncthreads();  // s_wave is reused by the next batch
+    }
+  }
+}
+
+// Eight lanes per cell, lane c folding corner c.
+constexpr int kMergeDepth = 40;
+__global__ void __launch_bounds__(256) big_merge_kernel(
+    const uint32_t* __restrict__ skey, const BigCell* __restrict__ big,
+    const uint32_t* __restrict__ big_count, float* __restrict__ head,
+    float* __restrict__ tail, float* __restrict__ sums, int32_t* __restrict__ map) {
+  __shared__ float stack[kMergeDepth][256];
+  __shared__ int64_t stack_size[kMergeDepth][32];
+  const int tid = threadIdx.x;
+  const int c = tid & 7;
+  const int g = tid >> 3;  // cell group of this workgroup
+  const uint32_t count = *big_count;
+  for (uint32_t bcell = blockIdx.x * 32 + g; bcell < count; bcell += gridDim.x * 32) {
+    const BigCell cell = big[bcell];
+    const int64_t nb = (cell.pieces + kBigBatch - 1) / kBigBatch;
+    int depth = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+      const int64_t m = cell.pieces - b * kBigBatch < kBigBatch ? cell.pieces - b * kBigBatch
+                                                                : kBigBatch;
+      float val = batch_slot(head, tail, cell.chunk, b)[c];
+      int d = depth;
+      int64_t sz = m;
+      while (d > 0 && stack_size[d - 1][g] == sz) {
+        val = stack[d - 1][tid] + val;
         sz *= 2;
         --d;
       }
-      if (tid < 8) stack[d][tid] = val;
-      __syncthreads();  // every thread has read stack_size and s_wave
-      if (tid == 0) stack_size[d] = sz;
+      stack[d][tid] = val;
+      // the eight lanes of the group read stack_size above and write the same value here
+      stack_size[d][g] = sz;
       depth = d + 1;
-      __syncthreads();
     }
-    if (tid < 8) {
-      float acc = stack[depth - 1][tid];
-      for (int d = depth - 2; d >= 0; --d) acc = stack[d][tid] + acc;
-      sums[cell.slot * 8 + tid] = acc;
-    }
-    if (tid == 0) map[skey[cell.slot]] = static_cast<int32_t>(cell.slot);
-    __syncthreads();
+    float acc = stack[depth - 1][tid];
+    for (int d = depth - 2; d >= 0; --d) acc = stack[d][tid] + acc;
+    sums[cell.slot * 8 + c] = acc;
+    if (c == 0) map[skey[cell.slot]] = static_cast<int32_t>(cell.slot);
   }
 }
 
@@ -839,6 +897,7 @@ struct pm_deposit_plan {
   int32_t* map = nullptr;
   BigCell* big = nullptr;
   uint32_t* big_count = nullptr;
+  int64_t* big_off = nullptr;   // batch offsets of the listed big cells (chunks + 1)
   bool timing = false;
   hipEvent_t ev[5] = {};
   float last_ms[4] = {0, 0, 0, 0};
@@ -853,7 +912,7 @@ void free_plan(pm_deposit_plan* p) {
   if (!p) return;
   void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->hist, p->digit,
                   p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->mid_pieces, p->bricks, p->map, p->big,
-                  p->big_count, p->acc64, p->maxbits};
+                  p->big_count, p->big_off, p->acc64, p->maxbits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
@@ -911,7 +970,8 @@ pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) 
       (st = alloc(&p->bricks, nbricks_of(n), "hipMalloc bricks")) ||
       (st = alloc(&p->map, m * m * m, "hipMalloc map")) ||
       (st = alloc(&p->big, chunks, "hipMalloc big")) ||
-      (st = alloc(&p->big_count, 1, "hipMalloc big_count"))) {
+      (st = alloc(&p->big_count, 1, "hipMalloc big_count")) ||
+      (st = alloc(&p->big_off, chunks + 1, "hipMalloc big_off"))) {
     free_plan(p);
     return st;
   }
@@ -1115,9 +1175,14 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
   }
   {
     PM_TIMED("deposit.big_cell", 0.0, stream);
-    big_cell_kernel<<<256, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
-                                             plan->tail, plan->sums, plan->cap, plan->map);
-    PM_CHECK_LAUNCH("big_cell_kernel");
+    big_scan_kernel<<<1, 256, 0, stream>>>(plan->big, plan->big_count, plan->big_off);
+    PM_CHECK_LAUNCH("big_scan_kernel");
+    big_batch_kernel<<<512, 256, 0, stream>>>(plan->big, plan->big_count, plan->big_off,
+                                              plan->head, plan->tail);
+    PM_CHECK_LAUNCH("big_batch_kernel");
+    big_merge_kernel<<<64, 256, 0, stream>>>(skey, plan->big, plan->big_count, plan->head,
+                                             plan->tail, plan->sums, plan->map);
+    PM_CHECK_LAUNCH("big_merge_kernel");
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[3], stream));
 
