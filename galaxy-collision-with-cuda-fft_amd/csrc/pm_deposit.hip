@@ -768,95 +768,164 @@ __global__ void __launch_bounds__(256) big_merge_kernel(
 // ---------------------------------------------------------------------------------------
 // 4. node assembly
 // ---------------------------------------------------------------------------------------
-// One thread per 4 consecutive z-nodes (i, j, k0..k0+3): they share the 2x2x5 cells
-// (ex, ey, ez) in {i, i+1} x {j, j+1} x {k0 .. k0+4}, whose map entries are loaded once.
-// The 32 sums loads are issued together (absent cells read slot 0 and are not added), so
-// a wave waits for two rounds of memory latency per 4 nodes; a wave with no occupied cell
-// writes zeros without loading sums. Per node the order is that of step 5 above.
-constexpr int kAsmNodes = 4;
+// rho = 0 with 16-byte stores where n^3 allows (the nodes of unmarked bricks stay 0).
+__global__ void __launch_bounds__(256) zero_rho_kernel(float* __restrict__ rho, int64_t count,
+                                                       bool vec) {
+  const int64_t c0 = 4 * (pm::xcd_block() * blockDim.x + threadIdx.x);
+  if (c0 >= count) return;
+  if (vec && c0 + 3 < count) {
+    *reinterpret_cast<float4*>(rho + c0) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  for (int64_t c = c0; c < count && c < c0 + 4; ++c) rho[c] = 0.0f;
+}
+
+// The marked bricks, compacted into list[0..*count) (any order: each is processed on its
+// own). One workgroup of 1024 lanes; each lane tests 32 consecutive marks, read as two
+// 16-byte loads issued before any is used.
+constexpr int kListLanes = 1024;
+constexpr int kListSpan = 32;  // marks per lane and round
+__global__ void __launch_bounds__(kListLanes) list_bricks_kernel(const uint8_t* __restrict__ bricks,
+                                                                 int64_t nbricks,
+                                                                 int32_t* __restrict__ list,
+                                                                 uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_count;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  if (tid == 0) s_count = 0u;
+  __syncthreads();
+  for (int64_t base = 0; base < nbricks; base += int64_t(kListLanes) * kListSpan) {
+    const int64_t lo = base + int64_t(tid) * kListSpan;
+    uint8_t mk[kListSpan];
+    if (lo + kListSpan <= nbricks) {
+      const uint4 a = reinterpret_cast<const uint4*>(bricks + lo)[0];
+      const uint4 b = reinterpret_cast<const uint4*>(bricks + lo)[1];
+      const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int q = 0; q < kListSpan; ++q) mk[q] = (w[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+    } else {
+#pragma unroll
+      for (int q = 0; q < kListSpan; ++q) mk[q] = lo + q < nbricks ? bricks[lo + q] : 0;
+    }
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < kListSpan; ++q) c += mk[q] != 0;
+    // wave prefix of the counts, one LDS atomic per wave
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const uint32_t wave_total = __shfl(incl, 63, 64);
+    uint32_t wave_base = 0;
+    if (lane == 0 && wave_total) wave_base = atomicAdd(&s_count, wave_total);
+    uint32_t pos = __shfl(wave_base, 0, 64) + incl - c;
+#pragma unroll
+    for (int q = 0; q < kListSpan; ++q)
+      if (mk[q]) list[pos++] = static_cast<int32_t>(lo + q);
+  }
+  __syncthreads();
+  if (tid == 0) *count = s_count;
+}
+
+// After zero_rho_kernel, the marked bricks (list_bricks_kernel), a workgroup at a time: the
+// brick's 9^3 extended cells (those holding a corner of one of its nodes) are staged in
+// LDS: their map entries, coalesced, and the 32-byte corner records of the occupied ones,
+// each loaded once. Then each thread assembles 2 nodes from LDS. Per node the order is that
+// of step 5 above: 0 + corner c = 0..7 of cell (i+1-cx, j+1-cy, k+1-cz) (extended indices),
+// absent cells skipped. Unmarked bricks (some 90% at 256^3 / 10M particles) cost nothing
+// beyond the zero fill.
+constexpr unsigned kPersistentBlocks = 2048;  // workgroups looping over listed bricks
+constexpr int kCellSpan = kBrick + 1;
+constexpr int kBrickCells = kCellSpan * kCellSpan * kCellSpan;  // 729
+static_assert(kBrick == 8, "two nodes per lane of a 256-lane workgroup cover an 8^3 brick");
 
 __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ map,
-                                                       const float* __restrict__ sums,
-                                                       int64_t cap, int n,
-                                                       float* __restrict__ rho, bool vec4,
-                                                       const uint8_t* __restrict__ bricks) {
-  const int kq = (n + kAsmNodes - 1) / kAsmNodes;
-  const int64_t items = static_cast<int64_t>(n) * n * kq;
-  const int64_t t = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= items) return;
-  const int k0 = static_cast<int>(t % kq) * kAsmNodes;
-  const int64_t ij = t / kq;
-  const int j = static_cast<int>(ij % n);
-  const int i = static_cast<int>(ij / n);
-  const int64_t m = n + 1;
+                                                       const float* __restrict__ sums, int n,
+                                                       float* __restrict__ rho, bool vec,
+                                                       const int32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ count) {
+  __shared__ int32_t s_slot[kBrickCells];
+  __shared__ float4 s_sum[kBrickCells][2];
   const int nb = (n + kBrick - 1) / kBrick;
-  // k0 .. k0+3 lie in one brick (kBrick is a multiple of kAsmNodes)
-  const bool marked = bricks[((i / kBrick) * nb + j / kBrick) * nb + k0 / kBrick] != 0;
-  int32_t cell[2][2][kAsmNodes + 1];
-  bool any = false;
-#pragma unroll
-  for (int ax = 0; ax < 2; ++ax) {
-#pragma unroll
-    for (int ay = 0; ay < 2; ++ay) {
-      const int32_t* row = map + ((i + ax) * m + (j + ay)) * m;
-#pragma unroll
-      for (int z = 0; z <= kAsmNodes; ++z) {
-        const int ez = k0 + z;
-        cell[ax][ay][z] = (marked && ez <= n) ? row[ez] : -1;
-        any |= cell[ax][ay][z] >= 0;
+  const int64_t m = n + 1;
+  const int tid = threadIdx.x;
+  // lane: row (a, bb) of the brick and nodes 2h, 2h + 1 of the row
+  const int a = tid >> 5, bb = (tid >> 2) & 7, h = tid & 3;
+  const uint32_t cnt = *count;
+  for (uint32_t e = blockIdx.x; e < cnt; e += gridDim.x) {
+    const int64_t b = list[e];
+    const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
+              bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+    const int i0 = bi * kBrick, j0 = bj * kBrick, k0 = bk * kBrick;
+    for (int q = tid; q < kBrickCells; q += blockDim.x) {
+      const int ex = i0 + q / (kCellSpan * kCellSpan), ey = j0 + (q / kCellSpan) % kCellSpan,
+                ez = k0 + q % kCellSpan;
+      int32_t slot = -1;
+      if (ex <= n && ey <= n && ez <= n) slot = map[(ex * m + ey) * m + ez];
+      s_slot[q] = slot;
+      if (slot >= 0) {
+        const float4* src = reinterpret_cast<const float4*>(sums + static_cast<int64_t>(slot) * 8);
+        s_sum[q][0] = src[0];
+        s_sum[q][1] = src[1];
       }
     }
-  }
-  float out[kAsmNodes] = {0.0f, 0.0f, 0.0f, 0.0f};
-  if (__ballot(any)) {
+    __syncthreads();
+    const int i = i0 + a, j = j0 + bb, k = k0 + 2 * h;
+    if (i < n && j < n && k < n) {
+      float out[2];
 #pragma unroll
-    for (int q = 0; q < kAsmNodes; ++q) {
-      float v[8];
+      for (int o = 0; o < 2; ++o) {
+        float acc = 0.0f;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int32_t slot = cell[1 - (c >> 2)][1 - ((c >> 1) & 1)][q + 1 - (c & 1)];
-        v[c] = sums[static_cast<int64_t>(slot >= 0 ? slot : 0) * 8 + c];
+        for (int c = 0; c < 8; ++c) {
+          const int q = ((a + 1 - (c >> 2)) * kCellSpan + (bb + 1 - ((c >> 1) & 1))) * kCellSpan +
+                        (2 * h + o + 1 - (c & 1));
+          const int32_t slot = s_slot[q];
+          const float4 half = s_sum[q][c >> 2];
+          const float v = (c & 3) == 0 ? half.x : (c & 3) == 1 ? half.y : (c & 3) == 2 ? half.z
+                                                                                      : half.w;
+          acc = slot >= 0 ? acc + v : acc;
+        }
+        out[o] = acc;
       }
-      float acc = 0.0f;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int32_t slot = cell[1 - (c >> 2)][1 - ((c >> 1) & 1)][q + 1 - (c & 1)];
-        acc = slot >= 0 ? acc + v[c] : acc;
+      float* dst = rho + (static_cast<int64_t>(i) * n + j) * n + k;
+      if (vec && k + 1 < n) {  // n even and rho 8-byte aligned
+        *reinterpret_cast<float2*>(dst) = make_float2(out[0], out[1]);
+      } else {
+        dst[0] = out[0];
+        if (k + 1 < n) dst[1] = out[1];
       }
-      out[q] = acc;
     }
-  }
-  float* dst = rho + (static_cast<int64_t>(i) * n + j) * n + k0;
-  if (vec4) {  // n % 4 == 0 and rho 16-byte aligned
-    *reinterpret_cast<float4*>(dst) = make_float4(out[0], out[1], out[2], out[3]);
-  } else {
-#pragma unroll
-    for (int q = 0; q < kAsmNodes; ++q)
-      if (k0 + q < n) dst[q] = out[q];
+    __syncthreads();  // the next brick reuses the LDS
   }
 }
 
-// Sparse reset, one workgroup per node brick: the map holds entries only for the cells
-// occupied at the last deposit, and each of those has a corner node in a brick marked then,
-// so clearing the cells (ex, ey, ez) in [8b, 8b+8]^3 of every marked brick b restores
-// map = -1 everywhere; the brick's mark is cleared for this deposit.
-__global__ void __launch_bounds__(256) reset_marked_kernel(int32_t* __restrict__ map, int n,
+// Sparse reset over the bricks listed at the last deposit: the map holds entries only for
+// the cells occupied then, and each of those has a corner node in a brick marked (and
+// listed) then, so clearing the cells (ex, ey, ez) in [8b, 8b+8]^3 of every listed brick b
+// restores map = -1 everywhere; the marks are cleared for this deposit.
+__global__ void __launch_bounds__(256) reset_listed_kernel(int32_t* __restrict__ map, int n,
                                                            uint8_t* __restrict__ bricks,
+                                                           const int32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ count,
                                                            uint32_t* __restrict__ big_count) {
   const int nb = (n + kBrick - 1) / kBrick;
-  const int64_t b = pm::xcd_block();
-  if (b == 0 && threadIdx.x == 0) *big_count = 0u;
-  if (bricks[b] == 0) return;
-  __syncthreads();  // every thread has read the mark before it is cleared
-  if (threadIdx.x == 0) bricks[b] = 0;
-  const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
-            bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *big_count = 0u;
   const int64_t m = n + 1;
   constexpr int kSpan = kBrick + 1;  // cells ex in [8bi, 8bi+8]
-  for (int q = threadIdx.x; q < kSpan * kSpan * kSpan; q += blockDim.x) {
-    const int ex = bi * kBrick + q / (kSpan * kSpan), ey = bj * kBrick + (q / kSpan) % kSpan,
-              ez = bk * kBrick + q % kSpan;
-    if (ex <= n && ey <= n && ez <= n) map[(ex * m + ey) * m + ez] = -1;
+  const uint32_t cnt = *count;
+  for (uint32_t e = blockIdx.x; e < cnt; e += gridDim.x) {
+    const int64_t b = list[e];
+    if (threadIdx.x == 0) bricks[b] = 0;
+    const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
+              bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+    for (int q = threadIdx.x; q < kSpan * kSpan * kSpan; q += blockDim.x) {
+      const int ex = bi * kBrick + q / (kSpan * kSpan), ey = bj * kBrick + (q / kSpan) % kSpan,
+                ez = bk * kBrick + q % kSpan;
+      if (ex <= n && ey <= n && ez <= n) map[(ex * m + ey) * m + ez] = -1;
+    }
   }
 }
 
@@ -891,6 +960,8 @@ struct pm_deposit_plan {
   uint8_t* head_end = nullptr;  // chunks + 16 bytes (join reads 16-byte windows)
   int32_t* mid_pieces = nullptr;
   uint8_t* bricks = nullptr;    // ceil(n/8)^3 node-brick marks
+  int32_t* brick_list = nullptr;   // the marked bricks of the last deposit (sorted mode)
+  uint32_t* brick_count = nullptr;
   bool bricks_valid = false;    // marks of the last deposit (sorted mode only)
   const int32_t* order = nullptr;  // particle order of the last sorted-mode deposit
   bool map_clean = false;       // map is -1 outside the cells around the marked bricks
@@ -911,7 +982,7 @@ namespace {
 void free_plan(pm_deposit_plan* p) {
   if (!p) return;
   void* bufs[] = {p->keys[0], p->keys[1], p->vals[0], p->vals[1], p->hist, p->digit,
-                  p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->mid_pieces, p->bricks, p->map, p->big,
+                  p->sums, p->head, p->tail, p->tail_slot, p->head_end, p->mid_pieces, p->bricks, p->brick_list, p->brick_count, p->map, p->big,
                   p->big_count, p->big_off, p->acc64, p->maxbits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -968,6 +1039,8 @@ pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) 
       (st = alloc(&p->head_end, chunks + 16, "hipMalloc head_end")) ||
       (st = alloc(&p->mid_pieces, chunks, "hipMalloc mid_pieces")) ||
       (st = alloc(&p->bricks, nbricks_of(n), "hipMalloc bricks")) ||
+      (st = alloc(&p->brick_list, nbricks_of(n), "hipMalloc brick_list")) ||
+      (st = alloc(&p->brick_count, 1, "hipMalloc brick_count")) ||
       (st = alloc(&p->map, m * m * m, "hipMalloc map")) ||
       (st = alloc(&p->big, chunks, "hipMalloc big")) ||
       (st = alloc(&p->big_count, 1, "hipMalloc big_count")) ||
@@ -1071,16 +1144,15 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
   const int64_t nodes = static_cast<int64_t>(n) * n * n;
   const int64_t m = static_cast<int64_t>(n) + 1;
   const int64_t cells = m * m * m;
-  const int64_t asm_items = static_cast<int64_t>(n) * n * ((n + kAsmNodes - 1) / kAsmNodes);
-  const bool vec4 = (n % kAsmNodes) == 0 && (reinterpret_cast<uintptr_t>(rho) & 15u) == 0;
+  const bool vec2 = n % 2 == 0 && (reinterpret_cast<uintptr_t>(rho) & 7u) == 0;
   const bool timing = plan->timing;
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[0], stream));
 
   {
     PM_TIMED("deposit.fill_map", 4.0 * cells, stream);
     if (plan->map_clean) {
-      reset_marked_kernel<<<static_cast<unsigned>(nbricks_of(n)), pm::kBlock, 0, stream>>>(
-          plan->map, n, plan->bricks, plan->big_count);
+      reset_listed_kernel<<<kPersistentBlocks, pm::kBlock, 0, stream>>>(
+          plan->map, n, plan->bricks, plan->brick_list, plan->brick_count, plan->big_count);
     } else {
       reset_map_kernel<<<pm::blocks_for(cells), pm::kBlock, 0, stream>>>(
           plan->map, cells, plan->bricks, nbricks_of(n), plan->big_count);
@@ -1094,8 +1166,14 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
     }
     {
       PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
-      assemble_kernel<<<pm::blocks_for(asm_items), pm::kBlock, 0, stream>>>(
-          plan->map, plan->sums, plan->cap, n, rho, vec4, plan->bricks);
+      zero_rho_kernel<<<pm::blocks_for((nodes + 3) / 4), pm::kBlock, 0, stream>>>(
+          rho, nodes, (reinterpret_cast<uintptr_t>(rho) & 15u) == 0);
+      PM_CHECK_LAUNCH("zero_rho_kernel");
+      list_bricks_kernel<<<1, kListLanes, 0, stream>>>(plan->bricks, nbricks_of(n),
+                                                       plan->brick_list, plan->brick_count);
+      PM_CHECK_LAUNCH("list_bricks_kernel");
+      assemble_kernel<<<kPersistentBlocks, 256, 0, stream>>>(
+          plan->map, plan->sums, n, rho, vec2, plan->brick_list, plan->brick_count);
       PM_CHECK_LAUNCH("assemble_kernel");
     }
     return PM_OK;
@@ -1189,8 +1267,14 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
   // 4. nodes
   {
     PM_TIMED("deposit.assemble", 8.0 * nodes, stream);
-    assemble_kernel<<<pm::blocks_for(asm_items), pm::kBlock, 0, stream>>>(
-        plan->map, plan->sums, plan->cap, n, rho, vec4, plan->bricks);
+    zero_rho_kernel<<<pm::blocks_for((nodes + 3) / 4), pm::kBlock, 0, stream>>>(
+          rho, nodes, (reinterpret_cast<uintptr_t>(rho) & 15u) == 0);
+      PM_CHECK_LAUNCH("zero_rho_kernel");
+      list_bricks_kernel<<<1, kListLanes, 0, stream>>>(plan->bricks, nbricks_of(n),
+                                                       plan->brick_list, plan->brick_count);
+      PM_CHECK_LAUNCH("list_bricks_kernel");
+      assemble_kernel<<<kPersistentBlocks, 256, 0, stream>>>(
+          plan->map, plan->sums, n, rho, vec2, plan->brick_list, plan->brick_count);
     PM_CHECK_LAUNCH("assemble_kernel");
   }
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[4], stream));
