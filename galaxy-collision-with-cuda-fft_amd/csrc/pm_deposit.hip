@@ -1082,6 +1082,15 @@ pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode) {
   return PM_OK;
 }
 
+pm_status pm_deposit_plan_brick_list(pm_deposit_plan* plan, const int32_t** list,
+                                     const uint32_t** count, int64_t* max_count) {
+  PM_REQUIRE(plan && list && count, "pm_deposit_plan_brick_list: null pointer");
+  *list = plan->bricks_valid ? plan->brick_list : nullptr;
+  *count = plan->bricks_valid ? plan->brick_count : nullptr;
+  if (max_count) *max_count = nbricks_of(plan->n);
+  return PM_OK;
+}
+
 pm_status pm_deposit_plan_order(pm_deposit_plan* plan, const int32_t** order) {
   PM_REQUIRE(plan && order, "pm_deposit_plan_order: null pointer");
   *order = plan->order;

@@ -141,6 +141,35 @@ __global__ void __launch_bounds__(256) grad4_bricks_kernel(const float* __restri
   }
 }
 
+// grad4_bricks_kernel over a list of bricks (the deposit's marked bricks, compacted),
+// workgroups looping over the list: no workgroups are spent on unmarked bricks.
+__global__ void __launch_bounds__(256) grad4_listed_kernel(const float* __restrict__ phi,
+                                                           float4* __restrict__ g4, int n,
+                                                           float inv_dx, float inv_2dx,
+                                                           const int32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ count) {
+  const int nb = (n + kGradBrick - 1) / kGradBrick;
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const uint32_t cnt = *count;
+  for (uint32_t e = blockIdx.x; e < cnt; e += gridDim.x) {
+    const int64_t b = list[e];
+    const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
+              bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = threadIdx.x + 256 * h;
+      const int i = bi * kGradBrick + (q >> 6), j = bj * kGradBrick + ((q >> 3) & 7),
+                k = bk * kGradBrick + (q & 7);
+      if (i < n && j < n && k < n) {
+        const int64_t idx = i * n2 + static_cast<int64_t>(j) * n + k;
+        float vx, vy, vz;
+        grad_at_ijk(phi, idx, i, j, k, n, n2, inv_dx, inv_2dx, vx, vy, vz);
+        g4[idx] = make_float4(vx, vy, vz, 0.0f);
+      }
+    }
+  }
+}
+
 // CIC interpolation from the interleaved field (same weights, order and roundings as
 // cic_gather, so bitwise equal).
 __device__ __forceinline__ void cic_gather4(float x, float y, float z,
@@ -481,6 +510,28 @@ pm_status pm_grad4_bricks(const float* phi, float* g4, int32_t n, float dx,
     grad4_bricks_kernel<<<static_cast<unsigned>(nb * nb * nb), 256, 0, stream>>>(
         phi, reinterpret_cast<float4*>(g4), n, inv_dx, inv_2dx, bricks);
     PM_CHECK_LAUNCH("grad4_bricks_kernel");
+  }
+  return PM_OK;
+}
+
+pm_status pm_grad4_listed(const float* phi, float* g4, int32_t n, float dx, const int32_t* list,
+                          const uint32_t* count, int64_t max_count, hipStream_t stream) {
+  PM_REQUIRE(phi && g4 && list && count, "pm_grad4_listed: null pointer");
+  PM_REQUIRE((reinterpret_cast<uintptr_t>(g4) & 15u) == 0, "pm_grad4_listed: g4 must be 16-byte aligned");
+  PM_REQUIRE(n >= 2 && n <= 1024, "pm_grad4_listed: grid_size must be in [2, 1024]");
+  PM_REQUIRE(dx > 0.0f, "pm_grad4_listed: dx must be positive");
+  const int64_t nb = (n + kGradBrick - 1) / kGradBrick;
+  PM_REQUIRE(max_count >= 0 && max_count <= nb * nb * nb,
+             "pm_grad4_listed: max_count must be in [0, number of bricks]");
+  if (max_count == 0) return PM_OK;
+  const float inv_dx = 1.0f / dx;
+  const float inv_2dx = 1.0f / (2.0f * dx);
+  const unsigned blocks = static_cast<unsigned>(max_count < 2048 ? max_count : 2048);
+  {
+    PM_TIMED("grad4", 0.0, stream);
+    grad4_listed_kernel<<<blocks, 256, 0, stream>>>(phi, reinterpret_cast<float4*>(g4), n,
+                                                     inv_dx, inv_2dx, list, count);
+    PM_CHECK_LAUNCH("grad4_listed_kernel");
   }
   return PM_OK;
 }

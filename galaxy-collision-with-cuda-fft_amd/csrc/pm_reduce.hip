@@ -293,23 +293,31 @@ __global__ void __launch_bounds__(256) moments4_kernel(const float* __restrict__
         v[Q - 1] = v[Q - 1] + t * t;
       }
     }
+    // the 4 nodes share x and y: sum m, m*z, m*z^2 over them, then form the moments (a
+    // quarter of the float64 products of the per-node form)
     const double x = static_cast<double>(q3.i) * dxd;
     const double y = static_cast<double>(q3.j) * dxd;
+    double s0 = 0.0, sz = 0.0, szz = 0.0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const double m = static_cast<double>(r[e]) * vol;
       const double z = static_cast<double>(q3.k + e) * dxd;
-      v[0] = v[0] + m;
-      v[1] = v[1] + m * x;
-      v[2] = v[2] + m * y;
-      v[3] = v[3] + m * z;
-      v[4] = v[4] + m * x * x;
-      v[5] = v[5] + m * y * y;
-      v[6] = v[6] + m * z * z;
-      v[7] = v[7] + m * x * y;
-      v[8] = v[8] + m * x * z;
-      v[9] = v[9] + m * y * z;
+      const double mz = m * z;
+      s0 = s0 + m;
+      sz = sz + mz;
+      szz = szz + mz * z;
     }
+    const double sx = s0 * x, sy = s0 * y;
+    v[0] = v[0] + s0;
+    v[1] = v[1] + sx;
+    v[2] = v[2] + sy;
+    v[3] = v[3] + sz;
+    v[4] = v[4] + sx * x;
+    v[5] = v[5] + sy * y;
+    v[6] = v[6] + szz;
+    v[7] = v[7] + sx * y;
+    v[8] = v[8] + sz * x;
+    v[9] = v[9] + sz * y;
   }
   pm::block_sum<Q>(v, s_red);
   if (threadIdx.x == 0) {

@@ -159,6 +159,9 @@ _SIGNATURES = {
     "pm_grad4_bricks": [_vp, _vp, _i32, _f, _vp, _vp],
     "pm_deposit_plan_node_bricks": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i32)],
     "pm_deposit_plan_order": [_vp, ctypes.POINTER(_vp)],
+    "pm_deposit_plan_brick_list": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                   ctypes.POINTER(_i64)],
+    "pm_grad4_listed": [_vp, _vp, _i32, _f, _vp, _vp, _i64, _vp],
     "pm_gather_kick_drift4_reorder": [_vp, ctypes.POINTER(ParticleArrays),
                                       ctypes.POINTER(ParticleArrays), _vp, _i64, _i32, _f, _f,
                                       _f, _f, _vp, _vp],

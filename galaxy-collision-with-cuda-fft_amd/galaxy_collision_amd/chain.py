@@ -5,7 +5,7 @@
     kick(dt/2) + drift(dt)     (pm_kick_drift, one pass over the particles)
     deposit                    (sort-based deterministic CIC deposit)
     solve                      (the configured Poisson solver, warm-started)
-    g = -grad(phi)             (pm_grad4_bricks: interleaved (gx, gy, gz, 0) at the nodes of
+    g = -grad(phi)             (pm_grad4_listed: interleaved (gx, gy, gz, 0) at the nodes of
                                the 8^3 bricks the deposit marked, i.e. every node the
                                gather reads; pm_grad4 on every node after a fixed-mode
                                deposit)
@@ -130,10 +130,11 @@ class PMChain:
         for s in range(steps):
             self.deposit()
             self.solve()
-            bricks = self.plan.node_bricks()
-            if bricks:  # only the nodes the gather below reads
-                nat.call("pm_grad4_bricks", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx,
-                         ctypes.c_void_p(bricks), nat.stream())
+            listed = self.plan.brick_list()
+            if listed:  # only the nodes the gather below reads
+                lst, cnt, mx = listed
+                nat.call("pm_grad4_listed", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx,
+                         ctypes.c_void_p(lst), ctypes.c_void_p(cnt), mx, nat.stream())
             else:
                 nat.call("pm_grad4", self._p(g.phi), nat.ptr(self.g4), self.n, self.dx,
                          nat.stream())

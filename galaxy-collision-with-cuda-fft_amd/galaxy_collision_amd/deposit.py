@@ -67,6 +67,14 @@ class DepositPlan:
                  nat.ptr(perm_out, torch.int32) if perm_out is not None else None,
                  nat.stream())
 
+    def brick_list(self) -> tuple[int, int, int] | None:
+        """(list address, device count address, max count) of the last deposit's marked node
+        bricks, compacted; None after a fixed-mode deposit."""
+        lst, cnt, mx = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int64()
+        nat.call("pm_deposit_plan_brick_list", self.handle, ctypes.byref(lst), ctypes.byref(cnt),
+                 ctypes.byref(mx))
+        return (lst.value, cnt.value, mx.value) if lst.value else None
+
     def order(self) -> int | None:
         """Device address of the last deposit's particle order (sorted position -> particle
         index), or None after a fixed-mode or empty deposit; valid until the next deposit."""

@@ -86,6 +86,13 @@ pm_status pm_deposit_cic_keyed(pm_deposit_plan* plan, const float* pos_x, const 
  * pm_deposit_plan_set_timing(plan, 1) was set before it; phases = key, sort, reduce, assemble. */
 pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable);
 
+/* The node bricks of pm_deposit_plan_node_bricks that are marked, compacted: list[0..*count)
+ * (device memory; *count is a device value, at most *max_count = the number of bricks), in
+ * no particular order. Owned by the plan, valid until its next deposit; NULL after a
+ * fixed-mode deposit. */
+pm_status pm_deposit_plan_brick_list(pm_deposit_plan* plan, const int32_t** list,
+                                     const uint32_t** count, int64_t* max_count);
+
 /* Particle order of the last pm_deposit_cic / pm_deposit_cic_keyed with this plan: order[i]
  * is the particle at sorted position i (the perm_out array), or NULL after a fixed-mode or
  * empty deposit. Device memory owned by the plan, valid until its next deposit. */
@@ -143,6 +150,13 @@ pm_status pm_grad4(const float* phi, float* g4, int32_t grid_size, float dx, hip
  * Bitwise pm_grad4 on those nodes. */
 pm_status pm_grad4_bricks(const float* phi, float* g4, int32_t grid_size, float dx,
                           const uint8_t* bricks, hipStream_t stream);
+
+/* pm_grad4_bricks on the listed bricks only (pm_deposit_plan_brick_list: list, its device
+ * count and max_count, the largest count it can hold): the same values at the same nodes,
+ * without a workgroup per unmarked brick. No counterpart in the reference. */
+pm_status pm_grad4_listed(const float* phi, float* g4, int32_t grid_size, float dx,
+                          const int32_t* list, const uint32_t* count, int64_t max_count,
+                          hipStream_t stream);
 pm_status pm_gather_kick4(const float* pos_x, const float* pos_y, const float* pos_z,
                           const float* g4, float* acc_x, float* acc_y, float* acc_z,
                           float* vel_x, float* vel_y, float* vel_z, int64_t n_particles,

@@ -299,6 +299,21 @@ def test_grad4_on_deposit_bricks(device, n, npart):
     assert m.any() and not m.all()
     assert_bitwise(part_np[m], full_np[m])
     assert np.all(part_np[~m] == 12345.0)
+    # the compacted list holds exactly the marked bricks, and the listed gradient writes
+    # the same nodes with the same values
+    lst, cnt_ptr, mx = plan.brick_list()
+    assert mx == nb ** 3
+    cnt = np.zeros(1, dtype=np.uint32)
+    assert hip.hipMemcpy(ctypes.c_void_p(cnt.ctypes.data), ctypes.c_void_p(cnt_ptr),
+                         ctypes.c_size_t(4), 2) == 0
+    ids = np.zeros(max(int(cnt[0]), 1), dtype=np.int32)
+    assert hip.hipMemcpy(ctypes.c_void_p(ids.ctypes.data), ctypes.c_void_p(lst),
+                         ctypes.c_size_t(4 * int(cnt[0])), 2) == 0
+    assert np.array_equal(np.sort(ids[: int(cnt[0])]), np.flatnonzero(marks))
+    listed = torch.full((n, n, n, 4), 12345.0, device=device)
+    nat.call("pm_grad4_listed", nat.ptr(phi), nat.ptr(listed), n, dx, ctypes.c_void_p(lst),
+             ctypes.c_void_p(cnt_ptr), mx, nat.stream())
+    assert_bitwise(_np(listed), part_np)
     # every in-box CIC corner node of every particle lies in a marked brick
     base = np.floor(pos.astype(np.float64) / dx).astype(np.int64)
     for c in range(8):
@@ -307,7 +322,7 @@ def test_grad4_on_deposit_bricks(device, n, npart):
         assert np.all(m[nd[:, 0], nd[:, 1], nd[:, 2]]), c
     plan.set_mode("fixed")
     plan.deposit(parts.pos_x, parts.pos_y, parts.pos_z, parts.mass, npart, dx, rho)
-    assert plan.node_bricks() is None
+    assert plan.node_bricks() is None and plan.brick_list() is None
     plan.close()
 
 
