@@ -100,7 +100,8 @@ def test_gather_and_fused_gather_kick_bitwise(device, npart, n, dx):
         assert_bitwise(v2[:, a], O.kick(v1[:, a], ref[a], h))
 
 
-@pytest.mark.parametrize("mode,how", [("fixed", "step"), ("sorted", "step"), ("sorted", "run")])
+@pytest.mark.parametrize("mode,how", [("fixed", "step"), ("sorted", "step"), ("sorted", "run"),
+                                      ("fixed", "run")])
 def test_fused_chain_step_equals_unfused_sequence(device, mode, how):
     """PMChain.step (kick+drift fused, interleaved gradient, gather+kick fused) and
     PMChain.run (one step's closing kick, the next one's opening kick and drift in one pass)
@@ -401,6 +402,15 @@ def test_chain_reorder_every(device):
     a, ca = run(2)
     b, _ = run(2)
     plain, _ = run(0)
+    # a fixed-mode deposit gives no particle order: the chain then never reorders
+    fx = ParticleState(icr.n)
+    fx.load(icr.pos, icr.vel, icr.mass, icr.gid)
+    cf = PMChain(fx, n, dx, make_solver("multigrid", n, dx=dx, grav_constant=1.0),
+                 deposit_mode="fixed", reorder_every=1)
+    cf.prime()
+    cf.run(4, dt)
+    torch.cuda.synchronize()
+    assert np.array_equal(fx.pid.to_numpy(), np.arange(icr.n))
     pid = a.pid.to_numpy()
     assert not np.array_equal(pid, np.arange(icr.n)), "no reorder happened"
     assert np.array_equal(np.sort(pid), np.arange(icr.n))
