@@ -1220,6 +1220,8 @@ struct pm_mg {
   // quad_blocks workgroups with their own partial sums
   unsigned quad_blocks = 0;
   double* partials_q = nullptr;
+  // the float64 defect's partials when their fold is left to the cycle's own fold launch
+  double* partials_r = nullptr;
   bool split_active() const { return e0s != nullptr && use_split; }
 };
 
@@ -1501,13 +1503,16 @@ pm_status plain_residual_norm(pm_mg* mg, const float* phi, double* out, hipStrea
 
 // r = f - A phi (float64 arithmetic) -> r32 (natural), or r32s (split layout) when `split`;
 // its sum of squares lands in red[2] (no sync).
-pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split = false) {
+// defer: the 8-node path only (the caller checks quad_defect_ok): the partials go to
+// partials_r and the cycle's fold launch folds them into red[2] (kCycleCarry*Fold).
+pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split = false,
+                         bool defer = false) {
   const Level& L = mg->lv[0];
   // 8 nodes per thread on split grids with n % 8 == 0 and 16-byte aligned phi and rho
   const bool quad = split && mg->quad_blocks && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0 &&
                     (reinterpret_cast<uintptr_t>(mg->cur_rho) & 15u) == 0;
   const unsigned b = quad ? mg->quad_blocks : pm::red_blocks(L.nodes());
-  double* partials = quad ? mg->partials_q : mg->partials;
+  double* partials = quad ? (defer ? mg->partials_r : mg->partials_q) : mg->partials;
   {
     PM_TIMED("mg.residual64", kRes64Bytes * L.nodes(), s);
     if (quad)
@@ -1524,6 +1529,7 @@ pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split 
                                                   mg->r32, mg->partials);
     PM_CHECK_LAUNCH("residual64_kernel");
   }
+  if (quad && defer) return PM_OK;
   if (quad)
     pm::fold_columns_kernel<1><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(b),
                                                          mg->red + 2);
@@ -1537,21 +1543,36 @@ pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split 
 // The launches of one V-cycle step. kCycleCorrect (MIXED): e0 = 0, e0 = V(r32), phi += e0,
 // red[0] = ||e0||^2, red[1] = ||phi||^2. kCyclePlain (PLAIN): phi = V(phi) on f0.
 // kCycleCarryFirst / kCycleCarryMore: kCycleCorrect with the carried guess (update_kernel).
-enum { kCycleCorrect = 0, kCyclePlain = 1, kCycleCarryFirst = 2, kCycleCarryMore = 3 };
+// kCycleCarry*Fold: also fold the deferred defect partials (partials_r) into red[2], in the
+// launch that folds the update's norms.
+enum {
+  kCycleCorrect = 0,
+  kCyclePlain = 1,
+  kCycleCarryFirst = 2,
+  kCycleCarryMore = 3,
+  kCycleCarryFirstFold = 4,
+  kCycleCarryMoreFold = 5
+};
 constexpr size_t kMaxGraphs = 6;
 
 pm_status enqueue_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
   Level& L0 = mg->lv[0];
   pm_status st;
   if (kind == kCyclePlain) return vcycle(mg, 0, s);
-  const int carry = kind == kCycleCarryFirst  ? kCarryFirst
-                    : kind == kCycleCarryMore ? kCarryMore
-                                              : kCarryNone;
+  const bool fold_defect = kind == kCycleCarryFirstFold || kind == kCycleCarryMoreFold;
+  const int carry = (kind == kCycleCarryFirst || kind == kCycleCarryFirstFold) ? kCarryFirst
+                    : (kind == kCycleCarryMore || kind == kCycleCarryMoreFold) ? kCarryMore
+                                                                                : kCarryNone;
   if (mg->split_active()) {
     if ((st = vcycle_split0(mg, phi, carry, s))) return st;
     double* partials;
     const unsigned blocks = split_update_blocks(mg, phi, &partials);
-    if (partials == mg->partials_q)  // black_update4_kernel's columns
+    if (fold_defect) {
+      PM_REQUIRE(partials == mg->partials_q, "multigrid: deferred defect fold off the 8-node path");
+      pm::fold_columns_pair_kernel<2, 1><<<1, pm::kBlock, 0, s>>>(
+          partials, static_cast<int>(blocks), mg->red, mg->partials_r,
+          static_cast<int>(mg->quad_blocks), mg->red + 2);
+    } else if (partials == mg->partials_q)  // black_update4_kernel's columns
       pm::fold_columns_kernel<2><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(blocks),
                                                            mg->red);
     else
@@ -1694,8 +1715,12 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
     L0.phi = split ? mg->e0s : mg->e0;
     L0.rhs = split ? mg->r32s : mg->r32;
     mg->last_phi = phi;
+    // carried solves on the 8-node path fold the defect norm in the cycle's fold launch
+    auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    const bool defer = carry && split && mg->quad_blocks && aligned(phi) && aligned(mg->guess) &&
+                       aligned(mg->cur_rho);
     if (!residual_ready) {
-      if ((st = mixed_residual(mg, carry ? mg->guess : phi, s, split))) return st;
+      if ((st = mixed_residual(mg, carry ? mg->guess : phi, s, split, defer))) return st;
     } else if (split) {
       PM_TIMED("mg.to_split", 8.0 * L0.nodes(), s);
       to_split_kernel<<<pm::blocks_for(L0.nodes()), pm::kBlock, 0, s>>>(mg->r32, mg->r32s,
@@ -1706,7 +1731,9 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
     double v[4] = {0.0, 0.0, -1.0, 0.0};
     bool read = false;
     for (int c = 0; c < mg->d.n_cycles; ++c) {
-      const int kind = !carry ? kCycleCorrect : c == 0 ? kCycleCarryFirst : kCycleCarryMore;
+      const int kind = !carry ? kCycleCorrect
+                       : c == 0   ? (defer ? kCycleCarryFirstFold : kCycleCarryFirst)
+                                  : (defer ? kCycleCarryMoreFold : kCycleCarryMore);
       if ((st = mixed_correct(mg, phi, v, s, kind))) return st;
       read = true;
       const double e_norm = std::sqrt(v[0]);
@@ -1720,7 +1747,7 @@ pm_status solve_iterate(pm_mg* mg, float* phi, pm_mg_stats* stats, hipStream_t s
         cycles = c + 1;
         break;
       }
-      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s, split))) return st;
+      if (c + 1 < mg->d.n_cycles && (st = mixed_residual(mg, phi, s, split, defer))) return st;
     }
     // no correction ran: phi is the guess (as an in-place guess would leave it)
     if (!read && carry)
@@ -1833,7 +1860,9 @@ pm_status pm_mg_create(const pm_mg_desc* desc, pm_mg** out) {
         const int64_t quads = static_cast<int64_t>(n) * n * mg->hn / 4;
         mg->quad_blocks = static_cast<unsigned>((quads + pm::kBlock - 1) / pm::kBlock);
         if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->partials_q),
-                         sizeof(double) * pm::kRedMaxQ * mg->quad_blocks))) {
+                         sizeof(double) * pm::kRedMaxQ * mg->quad_blocks)) ||
+            (st = dalloc(mg, reinterpret_cast<void**>(&mg->partials_r),
+                         sizeof(double) * mg->quad_blocks))) {
           destroy(mg);
           return st;
         }

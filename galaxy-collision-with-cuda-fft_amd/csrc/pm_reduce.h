@@ -89,6 +89,37 @@ __global__ void __launch_bounds__(256) fold_columns_kernel(const double* __restr
   }
 }
 
+// Two fold_columns_kernel folds in one launch: Q1 columns of p1 into out1 and Q2 columns of
+// p2 into out2, each quantity summed exactly as its own fold_columns_kernel would.
+template <int Q1, int Q2>
+__global__ void __launch_bounds__(256) fold_columns_pair_kernel(const double* __restrict__ p1,
+                                                                int blocks1,
+                                                                double* __restrict__ out1,
+                                                                const double* __restrict__ p2,
+                                                                int blocks2,
+                                                                double* __restrict__ out2) {
+  __shared__ double s_red[4 * (Q1 + Q2)];
+  double v[Q1 + Q2];
+#pragma unroll
+  for (int q = 0; q < Q1 + Q2; ++q) v[q] = 0.0;
+  for (int b = threadIdx.x; b < blocks1; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < Q1; ++q) v[q] = v[q] + p1[static_cast<int64_t>(q) * blocks1 + b];
+  }
+  for (int b = threadIdx.x; b < blocks2; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < Q2; ++q)
+      v[Q1 + q] = v[Q1 + q] + p2[static_cast<int64_t>(q) * blocks2 + b];
+  }
+  block_sum<Q1 + Q2>(v, s_red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < Q1; ++q) out1[q] = v[q];
+#pragma unroll
+    for (int q = 0; q < Q2; ++q) out2[q] = v[Q1 + q];
+  }
+}
+
 // Scratch for reductions issued without a handle: kRedBlocks*kRedMaxQ partial doubles and
 // kRedMaxQ result doubles on the device, plus a pinned host mirror of the result.
 struct RedScratch {
