@@ -44,7 +44,9 @@ class PMChain:
     so that particles of one cell stay neighbours in memory: as they drift out of order the
     deposit's and the gather's loads scatter and slow down (at 256^3 / 10M particles the
     piece sums take 112 us after 10 steps, 145 after 25 and 265 after 320 unordered).
-    Particle slots then change (``parts.pid`` keeps each particle's load index), and since
+    Particle slots then change (``parts.pid`` keeps each particle's load index; the fields
+    keep their ``DeviceField`` objects but swap the tensors behind them, so read
+    ``field.tensor`` after a run rather than holding it across one), and since
     a cell's particles are summed in slot order the density changes at rounding level
     against an unordered run; every step stays deterministic. A reorder needs a following
     drift, so only multi-step ``run`` calls reorder (``step`` never does)."""
