@@ -188,6 +188,21 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(
   const int buckets = 1 << bits;
   const uint32_t mask = static_cast<uint32_t>(buckets - 1);
   const int64_t tile = pm::xcd_block();  // as in radix_hist_kernel
+  // the tile's keys (and values) are loaded first: their latency overlaps the digit-start
+  // scan below, which does not need them
+  const uint64_t lanes_below = (1ull << lane) - 1ull;
+  const int64_t wave_start =
+      tile * kTile + static_cast<int64_t>(wave) * 64 * kSortItems;
+  uint32_t k[kSortItems];
+  int32_t v[kSortItems];
+  uint32_t rank[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t p = wave_start + r * 64 + lane;
+    const bool valid = p < np;
+    k[r] = valid ? keys_in[p] : 0u;
+    v[r] = valid ? (vals_in ? vals_in[p] : static_cast<int32_t>(p)) : 0;
+  }
   for (int d = threadIdx.x; d < kSortWaves * (1 << kMaxDigitBits); d += blockDim.x)
     (&wave_cnt[0][0])[d] = 0;
   {
@@ -204,20 +219,6 @@ __global__ void __launch_bounds__(256) radix_scatter_kernel(
       tile_base[d0 + 1] = ex + t0 + hist[static_cast<int64_t>(d0 + 1) * tiles + tile];
   }
   __syncthreads();
-
-  const uint64_t lanes_below = (1ull << lane) - 1ull;
-  const int64_t wave_start =
-      tile * kTile + static_cast<int64_t>(wave) * 64 * kSortItems;
-  uint32_t k[kSortItems];
-  int32_t v[kSortItems];
-  uint32_t rank[kSortItems];
-#pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    const int64_t p = wave_start + r * 64 + lane;
-    const bool valid = p < np;
-    k[r] = valid ? keys_in[p] : 0u;
-    v[r] = valid ? (vals_in ? vals_in[p] : static_cast<int32_t>(p)) : 0;
-  }
   // Rounds whose valid lanes all hold one digit (common for cell-ordered input) need no
   // per-bit ballots, and a run of such rounds keeps that digit's wave count in a register
   // (c_digit, c_count: wave-uniform) instead of an LDS write-then-read per round; the
