@@ -1059,18 +1059,6 @@ __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restr
     const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
     const bool inner_row = q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2;
     const int p0 = (q.i + q.j) & 1, p1 = p0 ^ 1;
-    const float4 e0 = ld4(e, q.t0);
-    float d1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (inner_row) {
-      float s[4];
-      quad_nbr_sums(e, q.t0, nh, static_cast<uint32_t>(hn), p1, s);
-      const float4 rv = ld4(r, plane + q.t0);
-#pragma unroll
-      for (int k4 = 0; k4 < 4; ++k4) {
-        const int k = 2 * (q.kk0 + k4) + p1;
-        if (k >= 1 && k <= n - 2) d1[k4] = (s[k4] - h2 * at4(rv, k4)) / 6.0f;
-      }
-    }
     const int64_t row0 = (static_cast<int64_t>(q.i) * n + q.j) * n + 2 * q.kk0;
     float* dst = phi + row0;
     float* gdst = CARRY != kCarryNone ? guess + row0 : nullptr;
@@ -1094,8 +1082,23 @@ __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restr
         for (int k = 0; k < 8 && 2 * q.kk0 + k < n; ++k) out[k] = val[k];
       }
     };
+    // every load of the quad issues before any is used: phi (and the guess), e, and the
+    // neighbour sums, which a face row takes from row (1, 1) and does not use
     load8(ph, dst);
     if (CARRY != kCarryNone) load8(gv, gdst);
+    const float4 e0 = ld4(e, q.t0);
+    const uint32_t tn = inner_row ? q.t0 : nh + static_cast<uint32_t>(hn) + q.kk0;
+    float s[4];
+    quad_nbr_sums(e, tn, nh, static_cast<uint32_t>(hn), p1, s);
+    const float4 rv = ld4(r, plane + q.t0);
+    float d1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (inner_row) {
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4) {
+        const int k = 2 * (q.kk0 + k4) + p1;
+        if (k >= 1 && k <= n - 2) d1[k4] = (s[k4] - h2 * at4(rv, k4)) / 6.0f;
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       if (2 * q.kk0 + k >= n) continue;
