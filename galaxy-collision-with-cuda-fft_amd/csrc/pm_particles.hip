@@ -212,14 +212,16 @@ __global__ void __launch_bounds__(256) gather_kick4_kernel(
     float* __restrict__ vz, int64_t np, int n, float dx, float h) {
   const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
+  const float x = px[p], y = py[p], z = pz[p];
+  const float u0 = vx[p], u1 = vy[p], u2 = vz[p];  // issued with the position loads
   float a0, a1, a2;
-  cic_gather4(px[p], py[p], pz[p], g4, n, dx, a0, a1, a2);
+  cic_gather4(x, y, z, g4, n, dx, a0, a1, a2);
   ax[p] = a0;
   ay[p] = a1;
   az[p] = a2;
-  vx[p] = vx[p] + a0 * h;
-  vy[p] = vy[p] + a1 * h;
-  vz[p] = vz[p] + a2 * h;
+  vx[p] = u0 + a0 * h;
+  vy[p] = u1 + a1 * h;
+  vz[p] = u2 + a2 * h;
 }
 
 // Step t's closing gather+kick(h_close) joined with step t+1's opening kick(h_open) and
@@ -235,12 +237,14 @@ __global__ void __launch_bounds__(256) gather_kick_drift4_kernel(
     float dt, uint32_t* __restrict__ keys) {
   const int64_t p = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (p >= np) return;
+  // the velocity loads issue with the position loads, not after the gather
   const float x = px[p], y = py[p], z = pz[p];
+  const float u0 = vx[p], u1 = vy[p], u2 = vz[p];
   float a0, a1, a2;
   cic_gather4(x, y, z, g4, n, dx, a0, a1, a2);
-  const float v0 = vx[p] + a0 * h_close;
-  const float v1 = vy[p] + a1 * h_close;
-  const float v2 = vz[p] + a2 * h_close;
+  const float v0 = u0 + a0 * h_close;
+  const float v1 = u1 + a1 * h_close;
+  const float v2 = u2 + a2 * h_close;
   const float w0 = v0 + a0 * h_open;
   const float w1 = v1 + a1 * h_open;
   const float w2 = v2 + a2 * h_open;
