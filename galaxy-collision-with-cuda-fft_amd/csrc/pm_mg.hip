@@ -879,6 +879,49 @@ __global__ void __launch_bounds__(256) rbs4_kernel(float* __restrict__ e,
   *reinterpret_cast<float4*>(e + own + q.t0) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
+// The first sweep of a cycle's pre-smoothing in one pass: rbs4_kernel<true> on colour 0
+// (zero guess: e0 = (0 - h2*r0)/6, 0 on faces) then rbs4_kernel<false> on colour 1, whose
+// colour-0 neighbours are those same values, computed here from r0 instead of read back
+// (r is 0 on faces and pad slots, so they come out 0 as written). Bitwise the two passes.
+__device__ __forceinline__ float zero_guess_value(float h2, float r) {
+  return (0.0f - h2 * r) / 6.0f;
+}
+
+__global__ void __launch_bounds__(256) rbs4_first_kernel(float* __restrict__ e,
+                                                         const float* __restrict__ r, int n,
+                                                         int hn, float h2) {
+  const int hq = hn >> 2;
+  const uint32_t quads = static_cast<uint32_t>(n) * n * hq;
+  const int64_t u = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (u >= quads) return;
+  const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
+  if (q.i < 1 || q.i > n - 2 || q.j < 1 || q.j > n - 2) return;
+  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
+  const uint32_t nh = static_cast<uint32_t>(n) * hn;
+  const int p0 = (q.i + q.j) & 1, p1 = p0 ^ 1;
+  // colour-0 defect around the quad (as quad_nbr_sums reads e) and colour-1 defect
+  const float4 a = ld4(r, q.t0 + nh), b = ld4(r, q.t0 - nh), c = ld4(r, q.t0 + hn),
+               d = ld4(r, q.t0 - hn), m = ld4(r, q.t0);
+  const float edge = p1 ? r[q.t0 + 4] : r[q.t0 - 1];
+  const float4 r1 = ld4(r, plane + q.t0);
+  float out0[4], out1[4];
+#pragma unroll
+  for (int k4 = 0; k4 < 4; ++k4) {
+    const int k0 = 2 * (q.kk0 + k4) + p0;
+    out0[k4] = (k0 >= 1 && k0 <= n - 2) ? zero_guess_value(h2, at4(m, k4)) : 0.0f;
+    const float kp = p1 ? (k4 < 3 ? zero_guess_value(h2, at4(m, k4 + 1)) : zero_guess_value(h2, edge))
+                        : zero_guess_value(h2, at4(m, k4));
+    const float km = p1 ? zero_guess_value(h2, at4(m, k4))
+                        : (k4 > 0 ? zero_guess_value(h2, at4(m, k4 - 1)) : zero_guess_value(h2, edge));
+    const float sum = zero_guess_value(h2, at4(a, k4)) + zero_guess_value(h2, at4(b, k4)) +
+                      zero_guess_value(h2, at4(c, k4)) + zero_guess_value(h2, at4(d, k4)) + kp + km;
+    const int k1 = 2 * (q.kk0 + k4) + p1;
+    out1[k4] = (k1 >= 1 && k1 <= n - 2) ? (sum - h2 * at4(r1, k4)) / 6.0f : 0.0f;
+  }
+  *reinterpret_cast<float4*>(e + q.t0) = make_float4(out0[0], out0[1], out0[2], out0[3]);
+  *reinterpret_cast<float4*>(e + plane + q.t0) = make_float4(out1[0], out1[1], out1[2], out1[3]);
+}
+
 // residual_s_kernel, 4 positions of both colours (8 consecutive natural nodes) per thread.
 __global__ void __launch_bounds__(256) residual_s4_kernel(const float* __restrict__ e,
                                                           const float* __restrict__ r,
@@ -1419,6 +1462,12 @@ pm_status vcycle_split0(pm_mg* mg, float* phi, int carry, hipStream_t s) {
   };
   pm_status st;
   for (int q = 0; q < mg->d.pre_sweeps; ++q) {
+    if (q == 0 && quad) {  // zero-guess colour 0 and colour 1 in one pass
+      PM_TIMED(level_name("mg.rbs_first", n), 8.0 * nodes, s);
+      rbs4_first_kernel<<<pm::blocks_for(quads), pm::kBlock, 0, s>>>(e, r, n, hn, L.h2);
+      PM_CHECK_LAUNCH("rbs4_first_kernel");
+      continue;
+    }
     if ((st = pass(0, q == 0)) || (st = pass(1, false))) return st;
   }
   {
