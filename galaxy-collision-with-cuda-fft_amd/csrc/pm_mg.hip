@@ -936,19 +936,26 @@ __global__ void __launch_bounds__(256) residual_s4_kernel(const float* __restric
   float out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2) {
     const uint32_t nh = static_cast<uint32_t>(n) * hn;
+    // both colours' loads issue before either colour is evaluated
+    float s[2][4];
+    float4 ev[2], rv[2];
 #pragma unroll
     for (int color = 0; color < 2; ++color) {
       const uint32_t own = static_cast<uint32_t>(color) * plane;
+      quad_nbr_sums(e + (plane - own), q.t0, nh, static_cast<uint32_t>(hn),
+                    (q.i + q.j + color) & 1, s[color]);
+      ev[color] = ld4(e, own + q.t0);
+      rv[color] = ld4(r, own + q.t0);
+    }
+#pragma unroll
+    for (int color = 0; color < 2; ++color) {
       const int p = (q.i + q.j + color) & 1;
-      float s[4];
-      quad_nbr_sums(e + (plane - own), q.t0, nh, static_cast<uint32_t>(hn), p, s);
-      const float4 ev = ld4(e, own + q.t0), rv = ld4(r, own + q.t0);
 #pragma unroll
       for (int k4 = 0; k4 < 4; ++k4) {
         const int k = 2 * (q.kk0 + k4) + p;
         if (k >= 1 && k <= n - 2) {
-          const float lap = (s[k4] - 6.0f * at4(ev, k4)) * inv_h2;
-          out[2 * k4 + p] = at4(rv, k4) - lap;
+          const float lap = (s[color][k4] - 6.0f * at4(ev[color], k4)) * inv_h2;
+          out[2 * k4 + p] = at4(rv[color], k4) - lap;
         }
       }
     }
