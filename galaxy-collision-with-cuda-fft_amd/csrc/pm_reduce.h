@@ -48,6 +48,31 @@ __device__ __forceinline__ void block_sum(double (&v)[Q], double* s_red) {
   __syncthreads();
 }
 
+// v[q] += p[b*sb + q*sq] for b = threadIdx.x, threadIdx.x + blockDim.x, ... < blocks, in that
+// order (the stage-2 loop of every fold), with the loads of kFoldUnroll iterations issued
+// before they are added.
+constexpr int kFoldUnroll = 4;
+template <int Q>
+__device__ __forceinline__ void fold_loop(const double* __restrict__ p, int blocks, int64_t sb,
+                                          int64_t sq, double (&v)[Q]) {
+  for (int b0 = threadIdx.x; b0 < blocks; b0 += kFoldUnroll * blockDim.x) {
+    double t[kFoldUnroll][Q];
+#pragma unroll
+    for (int u = 0; u < kFoldUnroll; ++u) {
+      const int b = b0 + u * static_cast<int>(blockDim.x);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) t[u][q] = b < blocks ? p[b * sb + q * sq] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kFoldUnroll; ++u) {
+      if (b0 + u * static_cast<int>(blockDim.x) < blocks) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q] = v[q] + t[u][q];
+      }
+    }
+  }
+}
+
 // Stage 2: fold `blocks` partials of Q quantities (partials[b*kRedMaxQ + q]) into out[q].
 template <int Q>
 __global__ void __launch_bounds__(256) fold_partials_kernel(const double* __restrict__ partials,
@@ -57,10 +82,7 @@ __global__ void __launch_bounds__(256) fold_partials_kernel(const double* __rest
   double v[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) v[q] = 0.0;
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = v[q] + partials[b * kRedMaxQ + q];
-  }
+  fold_loop<Q>(partials, blocks, kRedMaxQ, 1, v);
   block_sum<Q>(v, s_red);
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -78,10 +100,7 @@ __global__ void __launch_bounds__(256) fold_columns_kernel(const double* __restr
   double v[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) v[q] = 0.0;
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = v[q] + partials[static_cast<int64_t>(q) * blocks + b];
-  }
+  fold_loop<Q>(partials, blocks, 1, blocks, v);
   block_sum<Q>(v, s_red);
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -102,15 +121,17 @@ __global__ void __launch_bounds__(256) fold_columns_pair_kernel(const double* __
   double v[Q1 + Q2];
 #pragma unroll
   for (int q = 0; q < Q1 + Q2; ++q) v[q] = 0.0;
-  for (int b = threadIdx.x; b < blocks1; b += blockDim.x) {
+  double v1[Q1], v2[Q2];
 #pragma unroll
-    for (int q = 0; q < Q1; ++q) v[q] = v[q] + p1[static_cast<int64_t>(q) * blocks1 + b];
-  }
-  for (int b = threadIdx.x; b < blocks2; b += blockDim.x) {
+  for (int q = 0; q < Q1; ++q) v1[q] = 0.0;
 #pragma unroll
-    for (int q = 0; q < Q2; ++q)
-      v[Q1 + q] = v[Q1 + q] + p2[static_cast<int64_t>(q) * blocks2 + b];
-  }
+  for (int q = 0; q < Q2; ++q) v2[q] = 0.0;
+  fold_loop<Q1>(p1, blocks1, 1, blocks1, v1);
+  fold_loop<Q2>(p2, blocks2, 1, blocks2, v2);
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) v[q] = v1[q];
+#pragma unroll
+  for (int q = 0; q < Q2; ++q) v[Q1 + q] = v2[q];
   block_sum<Q1 + Q2>(v, s_red);
   if (threadIdx.x == 0) {
 #pragma unroll
