@@ -380,35 +380,58 @@ __global__ void __launch_bounds__(256) coarse_solve_kernel(float* __restrict__ p
 // Bottom solver: the V-cycle of every level of size <= kBottomMax in one workgroup, with
 // all of those levels resident in LDS. Each phase repeats the per-node arithmetic of the
 // global kernels above (RB pass, residual + restriction, prolongation) in the same order,
-// so the result is bitwise that of the multi-kernel V-cycle; the residual is recomputed
-// inside the restriction instead of being stored.
+// so the result is bitwise that of the multi-kernel V-cycle; the residual lives in an LDS
+// scratch of the finest bottom level's size between its evaluation and the restriction.
 // ---------------------------------------------------------------------------------------
 constexpr int kBottomMax = 17;
 constexpr int kBottomLevels = 8;
 constexpr int kBottomThreads = 1024;
+constexpr int kBottomLoads = (kBottomMax * kBottomMax * kBottomMax + kBottomThreads - 1) / kBottomThreads;
 constexpr int kCoarse64 = 5;  // coarsest levels up to this size are solved in float64 (MIXED)
 
 struct BottomDesc {
   int levels;
   int n[kBottomLevels];
   int off[kBottomLevels];  // LDS offset (floats) of phi; rhs follows at off + n^3
+  int res_off;             // LDS offset (floats) of the residual scratch ((n[0]-1)^3 floats)
   float h2[kBottomLevels];
   float inv_h2[kBottomLevels];
+  float inv_n[kBottomLevels];     // 1/n, 1/((n+1)/2), 1/(n-1) as float: node indices are
+  float inv_half[kBottomLevels];  // split by qdiv, a few VALU ops instead of an integer
+  float inv_m[kBottomLevels];     // division per index
+  float inv_cells[kBottomLevels];  // 1 / (((n-2)>>1) + 1): coarse cells per axis under level n
+  float inv_in[kBottomLevels];     // 1 / (n-2): interior nodes per axis
+  float inv_hi[kBottomLevels];     // 1 / ((n-1)>>1): most nodes of one colour per interior row
   double h2_coarsest;
   int pre, post, coarse_sweeps, coarse64_sweeps;
 };
 
+// x / d for 0 <= x < 2^15 and 1 <= d <= 64, given inv = 1.0f / d rounded to float: exact,
+// since (x + 0.5) / d lies at least 0.5 / d from an integer and the two roundings move it
+// by less than 2^-22 * x / d (checked exhaustively over that range)
+__device__ __forceinline__ int qdiv(int x, float inv) {
+  return static_cast<int>((static_cast<float>(x) + 0.5f) * inv);
+}
+static_assert(kBottomMax * kBottomMax * kBottomMax < (1 << 15), "qdiv range");
+
+// One colour of interior nodes: lanes cover the interior rows (i, j) and, per row, the
+// nodes k in 1..n-2 whose i + j + k has the colour's parity (k = 2 kk + 2 - p, p the
+// parity of i + j + color). Nodes of one colour do not read each other, so the order in
+// which lanes update them does not change the result.
 __device__ __forceinline__ void lds_rb_pass(float* phi, const float* rhs, int n, float h2,
-                                            int color) {
-  const int half = (n + 1) >> 1;
-  const int pairs = n * n * half;
+                                            int color, float inv_in, float inv_hi) {
+  const int m = n - 2;
+  const int hi = (n - 1) >> 1;  // most nodes of one colour in an interior row
+  const int pairs = m * m * hi;
   const int n2 = n * n;
   for (int t = threadIdx.x; t < pairs; t += blockDim.x) {
-    const int kk = t % half;
-    const int j = (t / half) % n;
-    const int i = t / (half * n);
-    const int k = 2 * kk + ((i + j + color) & 1);
-    if (i < 1 || i > n - 2 || j < 1 || j > n - 2 || k < 1 || k > n - 2) continue;
+    const int q = qdiv(t, inv_hi);  // t / hi
+    const int kk = t - q * hi;
+    const int ii = qdiv(q, inv_in);
+    const int i = ii + 1;
+    const int j = q - ii * m + 1;
+    const int k = 2 * kk + 2 - ((i + j + color) & 1);
+    if (k > n - 2) continue;
     const int c = i * n2 + j * n + k;
     const float sum = phi[c + n2] + phi[c - n2] + phi[c + n] + phi[c - n] + phi[c + 1] + phi[c - 1];
     phi[c] = (sum - h2 * rhs[c]) / 6.0f;
@@ -417,10 +440,10 @@ __device__ __forceinline__ void lds_rb_pass(float* phi, const float* rhs, int n,
 }
 
 __device__ __forceinline__ void lds_smooth(float* phi, const float* rhs, int n, float h2,
-                                           int sweeps) {
+                                           int sweeps, float inv_in, float inv_hi) {
   for (int q = 0; q < sweeps; ++q) {
-    lds_rb_pass(phi, rhs, n, h2, 0);
-    lds_rb_pass(phi, rhs, n, h2, 1);
+    lds_rb_pass(phi, rhs, n, h2, 0, inv_in, inv_hi);
+    lds_rb_pass(phi, rhs, n, h2, 1, inv_in, inv_hi);
   }
 }
 
@@ -440,16 +463,30 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
                                                                        const float* __restrict__ rhs_top,
                                                                        BottomDesc d) {
   extern __shared__ float lds[];
+  float* res = lds + d.res_off;  // residual scratch of the finest bottom level
   __shared__ double s64_phi[kCoarse64 * kCoarse64 * kCoarse64];
   __shared__ double s64_rhs[kCoarse64 * kCoarse64 * kCoarse64];
   const int L = d.levels;
   {
+    // every load is issued before the first LDS store: one memory round trip, not one
+    // per iteration (kBottomMax^3 <= kBottomLoads * kBottomThreads)
     const int nn = d.n[0] * d.n[0] * d.n[0];
     float* phi = lds + d.off[0];
     float* rhs = phi + nn;
-    for (int c = threadIdx.x; c < nn; c += blockDim.x) {
-      phi[c] = phi_top[c];
-      rhs[c] = rhs_top[c];
+    float vp[kBottomLoads], vr[kBottomLoads];
+#pragma unroll
+    for (int q = 0; q < kBottomLoads; ++q) {
+      const int c = threadIdx.x + q * kBottomThreads;
+      vp[q] = c < nn ? phi_top[c] : 0.0f;
+      vr[q] = c < nn ? rhs_top[c] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < kBottomLoads; ++q) {
+      const int c = threadIdx.x + q * kBottomThreads;
+      if (c < nn) {
+        phi[c] = vp[q];
+        rhs[c] = vr[q];
+      }
     }
     __syncthreads();
   }
@@ -458,20 +495,43 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
     const int nf = d.n[l], nc = d.n[l + 1];
     float* phi = lds + d.off[l];
     float* rhs = phi + nf * nf * nf;
-    lds_smooth(phi, rhs, nf, d.h2[l], d.pre);
+    lds_smooth(phi, rhs, nf, d.h2[l], d.pre, d.inv_in[l], d.inv_hi[l]);
+    // residual of fine nodes 1..nf-1 per axis once into LDS (packed (nf-1)^3; 0 at the
+    // face nf-1, which the restriction of an even level reaches; node 0 it never reads),
+    // then the 27-point restriction reads it: the same values and the same summation
+    // order as evaluating the residual inside the restriction
+    const int m = nf - 1, m2 = m * m;
+    for (int c = threadIdx.x; c < m2 * m; c += blockDim.x) {
+      const int q = qdiv(c, d.inv_m[l]);
+      const int k = c - q * m, i = qdiv(q, d.inv_m[l]), j = q - i * m;
+      res[c] = lds_residual(phi, rhs, nf, i + 1, j + 1, k + 1, d.inv_h2[l]);
+    }
+    __syncthreads();
     float* cphi = lds + d.off[l + 1];
     float* crhs = cphi + nc * nc * nc;
     const int ncc = nc * nc * nc;
     for (int c = threadIdx.x; c < ncc; c += blockDim.x) {
-      const int kc = c % nc, jc = (c / nc) % nc, ic = c / (nc * nc);
+      const int q = qdiv(c, d.inv_n[l + 1]);
+      const int kc = c - q * nc, ic = qdiv(q, d.inv_n[l + 1]), jc = q - ic * nc;
       float out = 0.0f;
       if (ic >= 1 && ic <= nc - 2 && jc >= 1 && jc <= nc - 2 && kc >= 1 && kc <= nc - 2) {
-        float acc = 0.0f;
+        const float* r = res + (2 * ic - 1) * m2 + (2 * jc - 1) * m + (2 * kc - 1);
+        float v[27];
+#pragma unroll
         for (int a = -1; a <= 1; ++a)
+#pragma unroll
           for (int b = -1; b <= 1; ++b)
+#pragma unroll
+            for (int e = -1; e <= 1; ++e) v[(a + 1) * 9 + (b + 1) * 3 + (e + 1)] = r[a * m2 + b * m + e];
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = -1; a <= 1; ++a)
+#pragma unroll
+          for (int b = -1; b <= 1; ++b)
+#pragma unroll
             for (int e = -1; e <= 1; ++e) {
               const float w = (a == 0 ? 2.0f : 1.0f) * (b == 0 ? 2.0f : 1.0f) * (e == 0 ? 2.0f : 1.0f);
-              acc += w * lds_residual(phi, rhs, nf, 2 * ic + a, 2 * jc + b, 2 * kc + e, d.inv_h2[l]);
+              acc += w * v[(a + 1) * 9 + (b + 1) * 3 + (e + 1)];
             }
         out = acc / 64.0f;
       }
@@ -498,7 +558,8 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
       for (int q = 0; q < sweeps; ++q) {
         for (int color = 0; color < 2; ++color) {
           for (int t = threadIdx.x; t < nc * nc * half; t += blockDim.x) {
-            const int kk = t % half, j = (t / half) % nc, i = t / (half * nc);
+            const int q = qdiv(t, d.inv_half[L - 1]);
+            const int kk = t - q * half, i = qdiv(q, d.inv_n[L - 1]), j = q - i * nc;
             const int k = 2 * kk + ((i + j + color) & 1);
             if (i < 1 || i > nc - 2 || j < 1 || j > nc - 2 || k < 1 || k > nc - 2) continue;
             const int c = i * n2 + j * nc + k;
@@ -512,7 +573,8 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
       for (int c = threadIdx.x; c < nn; c += blockDim.x) phi[c] = static_cast<float>(s64_phi[c]);
       __syncthreads();
     } else {
-      lds_smooth(phi, rhs, nc, d.h2[L - 1], nc == 3 && d.coarse_sweeps > 1 ? 1 : d.coarse_sweeps);
+      lds_smooth(phi, rhs, nc, d.h2[L - 1], nc == 3 && d.coarse_sweeps > 1 ? 1 : d.coarse_sweeps,
+                 d.inv_in[L - 1], d.inv_hi[L - 1]);
     }
   }
   // up
@@ -522,27 +584,72 @@ __global__ void __launch_bounds__(kBottomThreads) bottom_vcycle_kernel(float* __
     float* rhs = phi + nf * nf * nf;
     const float* cphi = lds + d.off[l + 1];
     const int nf2 = nf * nf, nc2 = nc * nc;
-    for (int c = threadIdx.x; c < nf2 * nf; c += blockDim.x) {
-      const int k = c % nf, j = (c / nf) % nf, i = c / nf2;
-      if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) continue;
-      const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
-      const int oi = i & 1, oj = j & 1, ok = k & 1;
-      float val = 0.0f;
-      for (int di = 0; di < 2; ++di) {
-        const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
-        for (int dj = 0; dj < 2; ++dj) {
+    // A fine node on an odd axis takes both coarse corners along it with weight 1/2, on
+    // an even axis the lower one with weight 1; each node sums its nonzero terms in
+    // (di, dj, dk) order, as the per-level kernel does, so the result is bitwise the same.
+    // Corner ci+1 is clamped: on a halving level it lies past the grid only where its
+    // weight is 0 (the odd fine node beyond it is a face).
+    if (nf2 * nf <= kBottomThreads) {
+      // small level: one lane per fine node (a single pass over the workgroup); all eight
+      // corners are loaded first and terms of weight 0 leave val unchanged, as if skipped
+      const int c = threadIdx.x;
+      const int q = qdiv(c, d.inv_n[l]);
+      const int k = c - q * nf, i = qdiv(q, d.inv_n[l]), j = q - i * nf;
+      if (c < nf2 * nf && i >= 1 && i <= nf - 2 && j >= 1 && j <= nf - 2 && k >= 1 &&
+          k <= nf - 2) {
+        const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
+        const int oi = i & 1, oj = j & 1, ok = k & 1;
+        const int i1 = min(ci + 1, nc - 1), j1 = min(cj + 1, nc - 1), k1 = min(ck + 1, nc - 1);
+        float cv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          cv[u] = cphi[((u & 4) ? i1 : ci) * nc2 + ((u & 2) ? j1 : cj) * nc + ((u & 1) ? k1 : ck)];
+        float val = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int di = u >> 2, dj = (u >> 1) & 1, dk = u & 1;
+          const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
           const float wj = oj ? 0.5f : (dj == 0 ? 1.0f : 0.0f);
-          for (int dk = 0; dk < 2; ++dk) {
-            const float wk = ok ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
-            const float w = wi * wj * wk;
-            if (w > 0.0f) val += w * cphi[(ci + di) * nc2 + (cj + dj) * nc + (ck + dk)];
+          const float wk = ok ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
+          const float w = wi * wj * wk;
+          const float t = val + w * cv[u];
+          val = w > 0.0f ? t : val;
+        }
+        phi[c] = phi[c] + val;
+      }
+    } else {
+      // larger level: one lane per coarse cell (ci, cj, ck), which loads the cell's eight
+      // corners once and corrects the eight fine nodes (2ci+oi, 2cj+oj, 2ck+ok)
+      const int cc = ((nf - 2) >> 1) + 1;  // cells per axis that hold interior fine nodes
+      const float inv_cc = d.inv_cells[l];
+      for (int t = threadIdx.x; t < cc * cc * cc; t += blockDim.x) {
+        const int q = qdiv(t, inv_cc);
+        const int ck = t - q * cc, ci = qdiv(q, inv_cc), cj = q - ci * cc;
+        const int i1 = min(ci + 1, nc - 1), j1 = min(cj + 1, nc - 1), k1 = min(ck + 1, nc - 1);
+        float cv[8];
+  #pragma unroll
+        for (int u = 0; u < 8; ++u)
+          cv[u] = cphi[((u & 4) ? i1 : ci) * nc2 + ((u & 2) ? j1 : cj) * nc + ((u & 1) ? k1 : ck)];
+  #pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          const int oi = o >> 2, oj = (o >> 1) & 1, ok = o & 1;
+          const int i = 2 * ci + oi, j = 2 * cj + oj, k = 2 * ck + ok;
+          if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) continue;
+          float val = 0.0f;
+  #pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int di = u >> 2, dj = (u >> 1) & 1, dk = u & 1;
+            if ((di && !oi) || (dj && !oj) || (dk && !ok)) continue;  // weight 0: skipped
+            const float w = (oi ? 0.5f : 1.0f) * (oj ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f);
+            val += w * cv[u];
           }
+          const int c = i * nf2 + j * nf + k;
+          phi[c] = phi[c] + val;
         }
       }
-      phi[c] = phi[c] + val;
     }
     __syncthreads();
-    lds_smooth(phi, rhs, nf, d.h2[l], d.post);
+    lds_smooth(phi, rhs, nf, d.h2[l], d.post, d.inv_in[l], d.inv_hi[l]);
   }
   {
     const int nn = d.n[0] * d.n[0] * d.n[0];
@@ -1269,6 +1376,7 @@ struct pm_mg {
   float* r32s = nullptr;
   int hn = 0;
   bool use_split = true;
+  bool use_bottom = true;  // levels from bottom_from on run in the one-workgroup solver
   // the 4-wide split reductions (float64 defect, fused update) run one quad per thread:
   // quad_blocks workgroups with their own partial sums
   unsigned quad_blocks = 0;
@@ -1380,10 +1488,18 @@ pm_status bottom(pm_mg* mg, size_t l, hipStream_t s) {
     d.n[q] = L.n;
     d.h2[q] = L.h2;
     d.inv_h2[q] = L.inv_h2;
+    d.inv_n[q] = 1.0f / static_cast<float>(L.n);
+    d.inv_half[q] = 1.0f / static_cast<float>((L.n + 1) >> 1);
+    d.inv_m[q] = 1.0f / static_cast<float>(L.n - 1);
+    d.inv_cells[q] = 1.0f / static_cast<float>(((L.n - 2) >> 1) + 1);
+    d.inv_in[q] = 1.0f / static_cast<float>(L.n - 2);
+    d.inv_hi[q] = 1.0f / static_cast<float>((L.n - 1) >> 1);
     d.off[q] = off;
     off += 2 * L.n * L.n * L.n;
   }
   const Level& top = mg->lv[l];
+  d.res_off = off;
+  off += (top.n - 1) * (top.n - 1) * (top.n - 1);
   d.h2_coarsest = mg->lv.back().h * mg->lv.back().h;
   d.pre = mg->d.pre_sweeps;
   d.post = mg->d.post_sweeps;
@@ -1402,7 +1518,8 @@ pm_status bottom(pm_mg* mg, size_t l, hipStream_t s) {
 
 pm_status vcycle(pm_mg* mg, size_t l, hipStream_t s) {
   const Level& L = mg->lv[l];
-  if (mg->bottom_from >= 0 && static_cast<int>(l) >= mg->bottom_from) return bottom(mg, l, s);
+  if (mg->use_bottom && mg->bottom_from >= 0 && static_cast<int>(l) >= mg->bottom_from)
+    return bottom(mg, l, s);
   if (l + 1 == mg->lv.size()) return coarsest(mg, L, s);
   pm_status st;
   if ((st = smooth(L, mg->d.pre_sweeps, s))) return st;
@@ -2114,15 +2231,22 @@ pm_status pm_mg_vcycle(pm_mg* mg, hipStream_t s) {
 pm_status pm_mg_set_option(pm_mg* mg, int32_t option, int32_t value) {
   pm::clear_error();
   PM_REQUIRE(mg, "pm_mg_set_option: null handle");
+  PM_REQUIRE(option == PM_MG_OPT_GRAPHS || option == PM_MG_OPT_SPLIT_FINE ||
+                 option == PM_MG_OPT_BOTTOM,
+             "pm_mg_set_option: unknown option");
+  // captured cycles bake in the launch sequence the options select
+  for (auto& g : mg->graphs) (void)hipGraphExecDestroy(g.exec);
+  mg->graphs.clear();
   switch (option) {
     case PM_MG_OPT_GRAPHS:
       mg->use_graphs = value != 0;
-      return PM_OK;
+      break;
     case PM_MG_OPT_SPLIT_FINE:
       mg->use_split = value != 0;
-      return PM_OK;
+      break;
     default:
-      PM_REQUIRE(false, "pm_mg_set_option: unknown option");
+      mg->use_bottom = value != 0;
+      break;
   }
   return PM_OK;
 }

@@ -319,9 +319,14 @@ pm_status pm_mg_warm_reset(pm_mg* mg);
  *   PM_MG_OPT_SPLIT_FINE (default 1): MIXED solves keep the finest level's correction and
  *     defect in colour-split layout, with the fill, the colour-0 prolongation and the
  *     update folded into the smoothing passes (needs pre/post sweeps >= 1 and a level below
- *     the finest; otherwise ignored). */
+ *     the finest; otherwise ignored).
+ *   PM_MG_OPT_BOTTOM (default 1): the levels of size <= 17 at the bottom of the hierarchy
+ *     run their part of each V-cycle in one workgroup with every level in LDS; 0 runs them
+ *     with the per-level kernels.
+ * Setting an option drops the handle's captured cycles; they are captured again on use. */
 #define PM_MG_OPT_GRAPHS 0
 #define PM_MG_OPT_SPLIT_FINE 1
+#define PM_MG_OPT_BOTTOM 2
 pm_status pm_mg_set_option(pm_mg* mg, int32_t option, int32_t value);
 
 /* One more V-cycle on the state of the last solve (reference multigrid.py:429-440

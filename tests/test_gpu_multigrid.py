@@ -310,3 +310,34 @@ def test_plain_solve_large_matches_c_port(device, n, warm):
     assert solver.last_cycles == cycles
     got = _np(tphi)
     assert np.abs(got - phi_c).max() <= 1e-6 * np.abs(phi_c).max()
+
+
+@pytest.mark.parametrize("mode,n", [("mixed", 17), ("mixed", 33), ("mixed", 37), ("mixed", 48),
+                                    ("mixed", 129), ("plain", 16), ("plain", 64)])
+def test_bottom_solver_equals_per_level_kernels(device, mode, n):
+    """The one-workgroup bottom solver (every level <= 17 in LDS: batched loads, the
+    residual held in LDS for the restriction, one lane per coarse cell in the
+    prolongation) gives bitwise the phi and the cycle counts of the same levels run by the
+    per-level kernels, cold and warm, on odd (17 -> 9), even (10 -> 6, a restriction that
+    reaches the far face) and halving (PLAIN 16 -> 8) hierarchies, with the bottom at the
+    finest level (17, 16) or below it."""
+    announce(f"bottom solver vs per-level kernels {mode} n={n}")
+    from galaxy_collision_amd import _native as nat
+    from galaxy_collision_amd.solver.multigrid import (MixedPrecisionMultigridSolver,
+                                                       MultigridPoissonSolver)
+
+    dx = 40.0 / n
+    rho = _t(_gaussian_rho(n, dx, 3.0 * dx, 1.0, centre=(0.47 * n * dx, 0.5 * n * dx,
+                                                          0.53 * n * dx)), device)
+    out = []
+    for bottom in (1, 0):
+        cls = MixedPrecisionMultigridSolver if mode == "mixed" else MultigridPoissonSolver
+        s = cls(n, dx=dx, grav_constant=1.0)
+        nat.call("pm_mg_set_option", s._h, nat.PM_MG_OPT_BOTTOM, bottom)
+        phi = torch.zeros((n,) * 3, device=device)
+        s.solve(rho, phi)
+        c0 = s.last_cycles
+        s.solve(rho * 1.01, phi, warm_start=True)
+        out.append((phi.clone(), c0, s.last_cycles))
+    assert torch.equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
