@@ -944,18 +944,38 @@ __device__ __forceinline__ float at4(const float4& v, int e) {
 // Neighbour sums of the 4 nodes of parity p = (i+j+color)&1 in a row of the colour-plane
 // `eo` (the other colour), in rb_pass_kernel's order. m = the other colour's same-row quad,
 // edge = eo[t0-1] (p == 0) or eo[t0+4] (p == 1).
-__device__ __forceinline__ void quad_nbr_sums(const float* __restrict__ eo, uint32_t t0,
-                                              uint32_t nh, uint32_t hn, int p, float s[4]) {
-  const float4 a = ld4(eo, t0 + nh), b = ld4(eo, t0 - nh), c = ld4(eo, t0 + hn),
-               d = ld4(eo, t0 - hn), m = ld4(eo, t0);
-  const float edge = p ? eo[t0 + 4] : eo[t0 - 1];
+// the loads of quad_nbr_sums, and the sums from them (split so that a kernel can issue
+// several quads' loads before any arithmetic)
+struct NbrQuad {
+  float4 a, b, c, d, m;
+  float edge;
+};
+
+__device__ __forceinline__ NbrQuad quad_nbr_load(const float* __restrict__ eo, uint32_t t0,
+                                                 uint32_t nh, uint32_t hn, int p) {
+  NbrQuad v;
+  v.a = ld4(eo, t0 + nh);
+  v.b = ld4(eo, t0 - nh);
+  v.c = ld4(eo, t0 + hn);
+  v.d = ld4(eo, t0 - hn);
+  v.m = ld4(eo, t0);
+  v.edge = p ? eo[t0 + 4] : eo[t0 - 1];
+  return v;
+}
+
+__device__ __forceinline__ void quad_nbr_sum(const NbrQuad& v, int p, float s[4]) {
   // k = 2(kk0+e)+p: the k+1 neighbour sits at kk0+e+p, the k-1 neighbour at kk0+e+p-1
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const float kp = p ? (e < 3 ? at4(m, e + 1) : edge) : at4(m, e);
-    const float km = p ? at4(m, e) : (e > 0 ? at4(m, e - 1) : edge);
-    s[e] = at4(a, e) + at4(b, e) + at4(c, e) + at4(d, e) + kp + km;
+    const float kp = p ? (e < 3 ? at4(v.m, e + 1) : v.edge) : at4(v.m, e);
+    const float km = p ? at4(v.m, e) : (e > 0 ? at4(v.m, e - 1) : v.edge);
+    s[e] = at4(v.a, e) + at4(v.b, e) + at4(v.c, e) + at4(v.d, e) + kp + km;
   }
+}
+
+__device__ __forceinline__ void quad_nbr_sums(const float* __restrict__ eo, uint32_t t0,
+                                              uint32_t nh, uint32_t hn, int p, float s[4]) {
+  quad_nbr_sum(quad_nbr_load(eo, t0, nh, hn, p), p, s);
 }
 
 template <bool ZERO>
@@ -972,10 +992,13 @@ __global__ void __launch_bounds__(256) rbs4_kernel(float* __restrict__ e,
   const uint32_t own = static_cast<uint32_t>(color) * plane;
   const int p = (q.i + q.j + color) & 1;
   float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  NbrQuad nb;
   if (!ZERO)
-    quad_nbr_sums(e + (plane - own), q.t0, static_cast<uint32_t>(n) * hn,
-                  static_cast<uint32_t>(hn), p, s);
+    nb = quad_nbr_load(e + (plane - own), q.t0, static_cast<uint32_t>(n) * hn,
+                       static_cast<uint32_t>(hn), p);
   const float4 rr = ld4(r, own + q.t0);
+  __builtin_amdgcn_sched_barrier(0);  // all loads leave before any arithmetic
+  if (!ZERO) quad_nbr_sum(nb, p, s);
   float out[4];
 #pragma unroll
   for (int k4 = 0; k4 < 4; ++k4) {
@@ -1045,25 +1068,30 @@ __global__ void __launch_bounds__(256) residual_s4_kernel(const float* __restric
     const uint32_t nh = static_cast<uint32_t>(n) * hn;
     // both colours' loads issue before either colour is evaluated
     float s[2][4];
+    NbrQuad nb[2];
     float4 ev[2], rv[2];
 #pragma unroll
     for (int color = 0; color < 2; ++color) {
       const uint32_t own = static_cast<uint32_t>(color) * plane;
-      quad_nbr_sums(e + (plane - own), q.t0, nh, static_cast<uint32_t>(hn),
-                    (q.i + q.j + color) & 1, s[color]);
+      nb[color] = quad_nbr_load(e + (plane - own), q.t0, nh, static_cast<uint32_t>(hn),
+                                (q.i + q.j + color) & 1);
       ev[color] = ld4(e, own + q.t0);
       rv[color] = ld4(r, own + q.t0);
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int color = 0; color < 2; ++color) quad_nbr_sum(nb[color], (q.i + q.j + color) & 1, s[color]);
+    // every node is evaluated and the faces masked afterwards (a branch around the face
+    // terms would let the compiler move the loads only they use behind it)
 #pragma unroll
     for (int color = 0; color < 2; ++color) {
       const int p = (q.i + q.j + color) & 1;
 #pragma unroll
       for (int k4 = 0; k4 < 4; ++k4) {
         const int k = 2 * (q.kk0 + k4) + p;
-        if (k >= 1 && k <= n - 2) {
-          const float lap = (s[color][k4] - 6.0f * at4(ev[color], k4)) * inv_h2;
-          out[2 * k4 + p] = at4(rv[color], k4) - lap;
-        }
+        const float lap = (s[color][k4] - 6.0f * at4(ev[color], k4)) * inv_h2;
+        const float val = at4(rv[color], k4) - lap;
+        out[2 * k4 + p] = (k >= 1 && k <= n - 2) ? val : 0.0f;
       }
     }
   }
@@ -1167,19 +1195,26 @@ __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restr
       load8(jp, phi, c0 + n);
       load8(jm, phi, c0 - n);
       load8(fv, rho, c0);
+      // all fourteen loads leave before any arithmetic: the scheduler would otherwise hold
+      // some back to save registers, a second round trip for them
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int x = 0; x < 8; ++x) fv[x] = factor * fv[x];  // f = 4*pi*G*rho, as the setup
+      // every node is evaluated and the faces (k = 0, n-1) are masked afterwards: were the
+      // face terms skipped by a branch, the compiler would move the loads that only they
+      // use behind it, a second round trip to memory for most rows
 #pragma unroll
       for (int x = 0; x < 8; ++x) {
         const int k = 2 * q.kk0 + x;
-        if (k < 1 || k > n - 2) continue;
+        const bool inner = k >= 1 && k <= n - 2;
         const double sum = static_cast<double>(ip[x]) + static_cast<double>(im[x]) +
                            static_cast<double>(jp[x]) + static_cast<double>(jm[x]) +
                            static_cast<double>(ctr[x + 2]) + static_cast<double>(ctr[x]);
         const double lap = (sum - 6.0 * static_cast<double>(ctr[x + 1])) * inv_h2;
         const double r = static_cast<double>(fv[x]) - lap;
-        r8[x] = static_cast<float>(r);
-        v[0] = v[0] + r * r;
+        r8[x] = inner ? static_cast<float>(r) : 0.0f;
+        const double acc = v[0] + r * r;
+        v[0] = inner ? acc : v[0];
       }
     }
     // colour of natural k = 2*kk0 + x is (p0 + x) & 1; its position kk0 + (x >> 1)
@@ -1196,7 +1231,7 @@ __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restr
   if (threadIdx.x == 0) partials[blockIdx.x] = v[0];  // one column (pm::fold_columns_kernel)
 }
 
-// black_update_kernel, 4 positions of both colours (8 natural nodes) per thread.
+// black_update_kernel, 4 positions of both colours (8 natural nodes) per thread (n % 8 == 0).
 template <int CARRY>
 __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restrict__ e,
                                                             const float* __restrict__ r,
@@ -1220,24 +1255,16 @@ __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restr
     float* dst = phi + row0;
     float* gdst = CARRY != kCarryNone ? guess + row0 : nullptr;
     float ph[8], gv[8];
-    const bool full = 2 * q.kk0 + 8 <= n;
+    // rows are whole quads (the kernel runs for n % 8 == 0 only)
     auto load8 = [&](float* out, const float* src) {
-      if (full) {
-        const float4 a = *reinterpret_cast<const float4*>(src);
-        const float4 b = *reinterpret_cast<const float4*>(src + 4);
-        out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
-        out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
-      } else {
-        for (int k = 0; k < 8; ++k) out[k] = 2 * q.kk0 + k < n ? src[k] : 0.0f;
-      }
+      const float4 a = *reinterpret_cast<const float4*>(src);
+      const float4 b = *reinterpret_cast<const float4*>(src + 4);
+      out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+      out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
     };
     auto store8 = [&](float* out, const float* val) {
-      if (full) {
-        *reinterpret_cast<float4*>(out) = make_float4(val[0], val[1], val[2], val[3]);
-        *reinterpret_cast<float4*>(out + 4) = make_float4(val[4], val[5], val[6], val[7]);
-      } else {
-        for (int k = 0; k < 8 && 2 * q.kk0 + k < n; ++k) out[k] = val[k];
-      }
+      *reinterpret_cast<float4*>(out) = make_float4(val[0], val[1], val[2], val[3]);
+      *reinterpret_cast<float4*>(out + 4) = make_float4(val[4], val[5], val[6], val[7]);
     };
     // every load of the quad issues before any is used: phi (and the guess), e, and the
     // neighbour sums, which a face row takes from row (1, 1) and does not use
@@ -1248,17 +1275,18 @@ __global__ void __launch_bounds__(256) black_update4_kernel(const float* __restr
     float s[4];
     quad_nbr_sums(e, tn, nh, static_cast<uint32_t>(hn), p1, s);
     const float4 rv = ld4(r, plane + q.t0);
-    float d1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (inner_row) {
+    __builtin_amdgcn_sched_barrier(0);  // all loads leave before any arithmetic
+    // every colour-1 value is evaluated and faces masked afterwards (a branch around them
+    // would let the compiler move the loads only they use behind it)
+    float d1[4];
 #pragma unroll
-      for (int k4 = 0; k4 < 4; ++k4) {
-        const int k = 2 * (q.kk0 + k4) + p1;
-        if (k >= 1 && k <= n - 2) d1[k4] = (s[k4] - h2 * at4(rv, k4)) / 6.0f;
-      }
+    for (int k4 = 0; k4 < 4; ++k4) {
+      const int k = 2 * (q.kk0 + k4) + p1;
+      const float val = (s[k4] - h2 * at4(rv, k4)) / 6.0f;
+      d1[k4] = (inner_row && k >= 1 && k <= n - 2) ? val : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      if (2 * q.kk0 + k >= n) continue;
       const int k4 = k >> 1;
       const float d = ((k & 1) == p0) ? at4(e0, k4) : d1[k4];
       float g = CARRY != kCarryNone ? gv[k] : 0.0f;
