@@ -80,9 +80,14 @@ __global__ void __launch_bounds__(256) radix_hist_kernel(const uint32_t* __restr
   if (base + kTile <= np) {
     const uint4* q4 = reinterpret_cast<const uint4*>(keys + base);
     const int lane = threadIdx.x & 63;
+    // the tile's loads all leave before the first count (one round trip per tile)
+    uint4 qs[kTile / 1024];
+#pragma unroll
+    for (int e = 0; e < kTile / 1024; ++e) qs[e] = q4[e * 256 + threadIdx.x];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int e = 0; e < kTile / 1024; ++e) {
-      const uint4 q = q4[e * 256 + threadIdx.x];
+      const uint4 q = qs[e];
       const uint32_t d[4] = {(q.x >> shift) & mask, (q.y >> shift) & mask,
                              (q.z >> shift) & mask, (q.w >> shift) & mask};
       const uint32_t prev = __shfl_up(d[3], 1, 64);   // digit before this lane's first key

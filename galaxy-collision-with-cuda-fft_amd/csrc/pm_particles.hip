@@ -18,21 +18,42 @@ namespace {
 // Central differences inside, one-sided on the faces; the neighbour offsets and the scale
 // are selected without branches so the six loads issue together (same a, b, s as the
 // branch-per-face form).
-__device__ __forceinline__ void grad_at_ijk(const float* __restrict__ phi, int64_t idx, int i,
-                                            int j, int k, int n, int64_t n2, float inv_dx,
-                                            float inv_2dx, float& vx, float& vy, float& vz) {
+// The six phi values the gradient at (i, j, k) reads (one-sided at the faces), and the
+// gradient from them (split so that a kernel can issue several nodes' loads first).
+struct GradTaps {
+  float ax, bx, ay, by, az, bz;
+};
+
+__device__ __forceinline__ GradTaps grad_taps(const float* __restrict__ phi, int64_t idx, int i,
+                                              int j, int k, int n, int64_t n2) {
   const int64_t px = i == n - 1 ? 0 : n2, mx = i == 0 ? 0 : n2;
   const int64_t py = j == n - 1 ? 0 : n, my = j == 0 ? 0 : n;
   const int64_t pz = k == n - 1 ? 0 : 1, mz = k == 0 ? 0 : 1;
-  const float ax = phi[idx + px], bx = phi[idx - mx];
-  const float ay = phi[idx + py], by = phi[idx - my];
-  const float az = phi[idx + pz], bz = phi[idx - mz];
+  GradTaps t;
+  t.ax = phi[idx + px];
+  t.bx = phi[idx - mx];
+  t.ay = phi[idx + py];
+  t.by = phi[idx - my];
+  t.az = phi[idx + pz];
+  t.bz = phi[idx - mz];
+  return t;
+}
+
+__device__ __forceinline__ void grad_from_taps(const GradTaps& t, int i, int j, int k, int n,
+                                               float inv_dx, float inv_2dx, float& vx,
+                                               float& vy, float& vz) {
   const float sx = (i == 0 || i == n - 1) ? inv_dx : inv_2dx;
   const float sy = (j == 0 || j == n - 1) ? inv_dx : inv_2dx;
   const float sz = (k == 0 || k == n - 1) ? inv_dx : inv_2dx;
-  vx = -(ax - bx) * sx;
-  vy = -(ay - by) * sy;
-  vz = -(az - bz) * sz;
+  vx = -(t.ax - t.bx) * sx;
+  vy = -(t.ay - t.by) * sy;
+  vz = -(t.az - t.bz) * sz;
+}
+
+__device__ __forceinline__ void grad_at_ijk(const float* __restrict__ phi, int64_t idx, int i,
+                                            int j, int k, int n, int64_t n2, float inv_dx,
+                                            float inv_2dx, float& vx, float& vy, float& vz) {
+  grad_from_taps(grad_taps(phi, idx, i, j, k, n, n2), i, j, k, n, inv_dx, inv_2dx, vx, vy, vz);
 }
 
 __device__ __forceinline__ void grad_at(const float* __restrict__ phi, int64_t idx, int n,
@@ -151,22 +172,36 @@ __global__ void __launch_bounds__(256) grad4_listed_kernel(const float* __restri
   const int nb = (n + kGradBrick - 1) / kGradBrick;
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const uint32_t cnt = *count;
+  const uint32_t unb = static_cast<uint32_t>(nb);
+  uint32_t b = blockIdx.x < cnt ? static_cast<uint32_t>(list[blockIdx.x]) : 0u;
   for (uint32_t e = blockIdx.x; e < cnt; e += gridDim.x) {
-    const int64_t b = list[e];
-    const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
-              bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+    // the next brick's list entry and both nodes' taps leave before any arithmetic. A node
+    // past the grid (n % 8 != 0) is clamped to the last node on that axis, whose gradient
+    // it computes and stores: the value the node's own lane stores, so the store need not
+    // be skipped (a skipped store would let the compiler move the loads behind the test)
+    const uint32_t en = e + gridDim.x;
+    const uint32_t b_next = en < cnt ? static_cast<uint32_t>(list[en]) : 0u;
+    const int bk = static_cast<int>(b % unb), bj = static_cast<int>((b / unb) % unb),
+              bi = static_cast<int>(b / (unb * unb));  // brick index < nb^3 <= 2^21
+    int ii[2], jj[2], kk[2];
+    GradTaps t[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int q = threadIdx.x + 256 * h;
-      const int i = bi * kGradBrick + (q >> 6), j = bj * kGradBrick + ((q >> 3) & 7),
-                k = bk * kGradBrick + (q & 7);
-      if (i < n && j < n && k < n) {
-        const int64_t idx = i * n2 + static_cast<int64_t>(j) * n + k;
-        float vx, vy, vz;
-        grad_at_ijk(phi, idx, i, j, k, n, n2, inv_dx, inv_2dx, vx, vy, vz);
-        g4[idx] = make_float4(vx, vy, vz, 0.0f);
-      }
+      ii[h] = min(bi * kGradBrick + (q >> 6), n - 1);
+      jj[h] = min(bj * kGradBrick + ((q >> 3) & 7), n - 1);
+      kk[h] = min(bk * kGradBrick + (q & 7), n - 1);
+      t[h] = grad_taps(phi, ii[h] * n2 + static_cast<int64_t>(jj[h]) * n + kk[h], ii[h], jj[h],
+                       kk[h], n, n2);
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float vx, vy, vz;
+      grad_from_taps(t[h], ii[h], jj[h], kk[h], n, inv_dx, inv_2dx, vx, vy, vz);
+      g4[ii[h] * n2 + static_cast<int64_t>(jj[h]) * n + kk[h]] = make_float4(vx, vy, vz, 0.0f);
+    }
+    b = b_next;
   }
 }
 
