@@ -21,17 +21,40 @@ inline unsigned red_blocks(int64_t count) {
   return static_cast<unsigned>(b);
 }
 
+// x from lane + N of the same row of 16 lanes (a DPP row shift; a lane whose source lies
+// past its row keeps its own x, which no lane that feeds lane 0 reads).
+template <int N>
+__device__ __forceinline__ double row_down(double x) {
+  static_assert(N >= 1 && N <= 15, "row shift");
+  const int lo = __double2loint(x), hi = __double2hiint(x);
+  const int l = __builtin_amdgcn_update_dpp(lo, lo, 0x100 + N, 0xF, 0xF, false);
+  const int h = __builtin_amdgcn_update_dpp(hi, hi, 0x100 + N, 0xF, 0xF, false);
+  return __hiloint2double(h, l);
+}
+
 // Sum Q doubles across a 256-thread workgroup; the result is valid on thread 0.
 // s_red must hold 4*Q doubles. Contains __syncthreads(): call from uniform control flow.
+// Each wave sums with a shuffle-down tree (offsets 32, 16, ..., 1). From offset 8 on, the
+// lanes whose sums reach lane 0 read lanes of their own row of 16, so those levels use DPP
+// row shifts (no LDS round trip), which hand them the same operands: lane 0's sum is the
+// shuffle tree's.
 template <int Q>
 __device__ __forceinline__ void block_sum(double (&v)[Q], double* s_red) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int off = 32; off >= 16; off >>= 1) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) v[q] += __shfl_down(v[q], off, 64);
   }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] += row_down<8>(v[q]);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] += row_down<4>(v[q]);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] += row_down<2>(v[q]);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] += row_down<1>(v[q]);
   if (lane == 0) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) s_red[wave * Q + q] = v[q];
