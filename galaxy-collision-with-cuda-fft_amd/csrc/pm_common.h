@@ -100,6 +100,45 @@ __device__ __forceinline__ float sqrt_rn(float x) {
   return s;
 }
 
+// Exact x / d for 32-bit unsigned x and a divisor fixed for a launch (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI 1994, Fig. 4.1):
+// with l = ceil(log2 d) and M = floor(2^32 (2^l - d) / d) + 1 (< 2^32),
+// x / d = (t + ((x - t) >> min(l, 1))) >> max(l - 1, 0), t = mulhi(M, x). A mulhi and a few
+// adds and shifts instead of the ~15-instruction division sequence; t <= x, so the sum
+// cannot overflow.
+struct FastDiv {
+  uint32_t d, mul;
+  int s1, s2;
+};
+
+inline FastDiv make_fast_div(uint32_t d) {
+  int l = 0;
+  while ((uint64_t{1} << l) < d) ++l;
+  FastDiv f;
+  f.d = d;
+  f.mul = static_cast<uint32_t>(((uint64_t{1} << 32) * ((uint64_t{1} << l) - d)) / d + 1);
+  f.s1 = l < 1 ? l : 1;
+  f.s2 = l > 1 ? l - 1 : 0;
+  return f;
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t x, const FastDiv& f) {
+  const uint32_t t = __umulhi(f.mul, x);
+  return (t + ((x - t) >> f.s1)) >> f.s2;
+}
+
+// idx3 with the divisions by n taken by a FastDiv of n.
+__device__ __forceinline__ Idx3 idx3(int64_t c, const FastDiv& fn) {
+  const uint32_t u = static_cast<uint32_t>(c), un = fn.d;
+  const uint32_t q = fast_div(u, fn);
+  const uint32_t i = fast_div(q, fn);
+  Idx3 r;
+  r.k = static_cast<int>(u - q * un);
+  r.j = static_cast<int>(q - i * un);
+  r.i = static_cast<int>(i);
+  return r;
+}
+
 }  // namespace pm
 
 #define PM_TRY_HIP(expr)                                   \

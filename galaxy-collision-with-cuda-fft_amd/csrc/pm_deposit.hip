@@ -338,11 +338,15 @@ static_assert(kPiecePos % 256 == 0, "phase 1 covers whole positions per lane");
 constexpr int kBrick = pm::kNodeBrick;
 
 __device__ __forceinline__ void mark_corner_brick(uint32_t key, int c, int n,
+                                                  const pm::FastDiv& m1,
                                                   uint8_t* __restrict__ bricks) {
-  const uint32_t m = static_cast<uint32_t>(n) + 1u;
-  const int ez = static_cast<int>(key % m);
-  const int ey = static_cast<int>((key / m) % m);
-  const int ex = static_cast<int>(key / (m * m));
+  // key = (ex * m + ey) * m + ez with m = n + 1 (m1 divides by m)
+  const uint32_t m = m1.d;
+  const uint32_t q = pm::fast_div(key, m1);
+  const int ez = static_cast<int>(key - q * m);
+  const uint32_t ex_u = pm::fast_div(q, m1);
+  const int ey = static_cast<int>(q - ex_u * m);
+  const int ex = static_cast<int>(ex_u);
   const int nb = (n + kBrick - 1) / kBrick;
   const int i = ex - 1 + (c >> 2), j = ey - 1 + ((c >> 1) & 1), k = ez - 1 + (c & 1);
   if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n)
@@ -355,7 +359,7 @@ __global__ void __launch_bounds__(256) piece_kernel(
     const float* __restrict__ mass, float dx, float inv_vol, int64_t np, uint32_t sentinel,
     int64_t chunks, float* __restrict__ sums, int32_t* __restrict__ map,
     float* __restrict__ head, float* __restrict__ tail, int64_t* __restrict__ tail_slot,
-    uint8_t* __restrict__ head_end, int n, uint8_t* __restrict__ bricks) {
+    uint8_t* __restrict__ head_end, int n, uint8_t* __restrict__ bricks, pm::FastDiv m1) {
   __shared__ float4 terms4[kPieceChunks * kTermStride / 4];
   __shared__ uint32_t keys_l[kPiecePos + 2];  // keys of positions base-1 .. base+kPiecePos
   float* terms = reinterpret_cast<float*>(terms4);
@@ -431,7 +435,7 @@ __global__ void __launch_bounds__(256) piece_kernel(
     } else {
       head[ch * 8 + c] = acc;
     }
-    if (starts) mark_corner_brick(rk, c, n, bricks);
+    if (starts) mark_corner_brick(rk, c, n, m1, bricks);
     if (!starts && ends) closing = true;
   };
   uint32_t kr[kChunk];
@@ -1250,7 +1254,8 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
     const unsigned piece_blocks = static_cast<unsigned>((np + kPiecePos - 1) / kPiecePos);
     piece_kernel<<<piece_blocks, 256, 0, stream>>>(
         skey, sidx, px, py, pz, mass, dx, inv_vol, np, plan->sentinel, chunks, plan->sums,
-        plan->map, plan->head, plan->tail, plan->tail_slot, plan->head_end, n, plan->bricks);
+        plan->map, plan->head, plan->tail, plan->tail_slot, plan->head_end, n, plan->bricks,
+        pm::make_fast_div(static_cast<uint32_t>(n) + 1u));
     PM_CHECK_LAUNCH("piece_kernel");
   }
   {

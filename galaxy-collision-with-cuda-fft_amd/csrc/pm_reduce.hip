@@ -41,10 +41,11 @@ __global__ void __launch_bounds__(256) sumsq_f64_kernel(const double* __restrict
 // With F, the same pass also sums f^2 (f = factor * rho, multigrid.py:249-252) into partial
 // slot 10 and, with STORE, writes f.
 template <bool F, bool STORE>
-__global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ rho, int n,
-                                                      double dxd, double vol,
+__global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ rho,
+                                                      pm::FastDiv fn, double dxd, double vol,
                                                       double* __restrict__ partials,
                                                       float* __restrict__ f, float factor) {
+  const int n = static_cast<int>(fn.d);
   constexpr int Q = F ? 11 : 10;
   __shared__ double s_red[4 * Q];
   double v[Q];
@@ -55,7 +56,7 @@ __global__ void __launch_bounds__(256) moments_kernel(const float* __restrict__ 
   int64_t r_begin, r_end;
   pm::block_range(total, r_begin, r_end);
   for (int64_t idx = r_begin + threadIdx.x; idx < r_end; idx += blockDim.x) {
-    const pm::Idx3 q3 = pm::idx3(idx, n);
+    const pm::Idx3 q3 = pm::idx3(idx, fn);
     const int i = q3.i, j = q3.j, k = q3.k;
     const float r = rho[idx];
     if (F) {
@@ -268,10 +269,11 @@ pm_status red_scratch(RedScratch** out) {
 // moments_kernel with 4 consecutive nodes (one row, n % 4 == 0) per iteration and 16-byte
 // loads and stores: a quarter of the dependent iterations per thread.
 template <bool F, bool STORE>
-__global__ void __launch_bounds__(256) moments4_kernel(const float* __restrict__ rho, int n,
+__global__ void __launch_bounds__(256) moments4_kernel(const float* __restrict__ rho, FastDiv fn,
                                                        double dxd, double vol,
                                                        double* __restrict__ partials,
                                                        float* __restrict__ f, float factor) {
+  const int n = static_cast<int>(fn.d);
   constexpr int Q = F ? 11 : 10;
   __shared__ double s_red[4 * Q];
   double v[Q];
@@ -281,7 +283,7 @@ __global__ void __launch_bounds__(256) moments4_kernel(const float* __restrict__
   int64_t r_begin, r_end;
   pm::block_range(total, r_begin, r_end);  // multiples of 256 nodes
   for (int64_t c = r_begin + 4 * threadIdx.x; c < r_end; c += 4 * blockDim.x) {
-    const pm::Idx3 q3 = pm::idx3(c, n);
+    const pm::Idx3 q3 = pm::idx3(c, fn);
     const float4 r4 = *reinterpret_cast<const float4*>(rho + c);
     const float r[4] = {r4.x, r4.y, r4.z, r4.w};
     if (F) {
@@ -358,7 +360,8 @@ pm_status launch_moments(const float* rho, int n, float dx, double grav, double*
   const bool sq = f_sumsq_dev != nullptr;
   const double vol = dxd * dxd * dxd;
 #define PM_MOMENTS(K, F_, S_) \
-  K<F_, S_><<<b, kBlock, 0, stream>>>(rho, n, dxd, vol, partials, f, factor)
+  K<F_, S_><<<b, kBlock, 0, stream>>>(rho, make_fast_div(static_cast<uint32_t>(n)), dxd, vol, \
+                                      partials, f, factor)
   if (vec && sq && f)
     PM_MOMENTS(moments4_kernel, true, true);
   else if (vec && sq)
