@@ -866,8 +866,11 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
   const uint32_t cnt = *count;
   for (uint32_t e = blockIdx.x; e < cnt; e += gridDim.x) {
     const int64_t b = list[e];
-    const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
-              bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+    // brick index < nb^3 <= 2^21: 32-bit division (the 64-bit sequence is several times
+    // longer, and every wave runs it per brick)
+    const uint32_t ub = static_cast<uint32_t>(b), unb = static_cast<uint32_t>(nb);
+    const int bk = static_cast<int>(ub % unb), bj = static_cast<int>((ub / unb) % unb),
+              bi = static_cast<int>(ub / (unb * unb));
     const int i0 = bi * kBrick, j0 = bj * kBrick, k0 = bk * kBrick;
     for (int q = tid; q < kBrickCells; q += blockDim.x) {
       const int ex = i0 + q / (kCellSpan * kCellSpan), ey = j0 + (q / kCellSpan) % kCellSpan,
@@ -929,8 +932,11 @@ __global__ void __launch_bounds__(256) reset_listed_kernel(int32_t* __restrict__
   for (uint32_t e = blockIdx.x; e < cnt; e += gridDim.x) {
     const int64_t b = list[e];
     if (threadIdx.x == 0) bricks[b] = 0;
-    const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
-              bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+    // brick index < nb^3 <= 2^21: 32-bit division (the 64-bit sequence is several times
+    // longer, and every wave runs it per brick)
+    const uint32_t ub = static_cast<uint32_t>(b), unb = static_cast<uint32_t>(nb);
+    const int bk = static_cast<int>(ub % unb), bj = static_cast<int>((ub / unb) % unb),
+              bi = static_cast<int>(ub / (unb * unb));
     for (int q = threadIdx.x; q < kSpan * kSpan * kSpan; q += blockDim.x) {
       const int ex = bi * kBrick + q / (kSpan * kSpan), ey = bj * kBrick + (q / kSpan) % kSpan,
                 ez = bk * kBrick + q % kSpan;

@@ -145,8 +145,9 @@ __global__ void __launch_bounds__(256) grad4_bricks_kernel(const float* __restri
   const int nb = (n + kGradBrick - 1) / kGradBrick;
   const int64_t b = pm::xcd_block();
   if (b >= static_cast<int64_t>(nb) * nb * nb || bricks[b] == 0) return;
-  const int bk = static_cast<int>(b % nb), bj = static_cast<int>((b / nb) % nb),
-            bi = static_cast<int>(b / (static_cast<int64_t>(nb) * nb));
+  const uint32_t ub = static_cast<uint32_t>(b), unb = static_cast<uint32_t>(nb);  // b < 2^21
+  const int bk = static_cast<int>(ub % unb), bj = static_cast<int>((ub / unb) % unb),
+            bi = static_cast<int>(ub / (unb * unb));
   const int64_t n2 = static_cast<int64_t>(n) * n;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
