@@ -29,7 +29,7 @@ import torch
 
 from galaxy_collision_amd import _native as nat
 from galaxy_collision_amd.data import GridState, as_tensor
-from galaxy_collision_amd.deposit import deposit_plan
+from galaxy_collision_amd.deposit import DepositPlan
 
 
 class PMChain:
@@ -61,7 +61,9 @@ class PMChain:
         dev = as_tensor(parts.pos_x).device
         self.acc = [torch.zeros(parts.n, device=dev) for _ in range(3)]
         self.g4 = torch.empty((self.n, self.n, self.n, 4), device=dev)
-        self.plan = deposit_plan(self.n, parts.n)
+        # the chain's own plan: the shared per-grid plan of deposit_density() is re-created
+        # when a larger particle set arrives, which would pull the scratch from under a chain
+        self.plan = DepositPlan(self.n, max(parts.n, 1))
         # cell keys of the current positions, written by the fused drift pass for the
         # next deposit (valid while keys_valid)
         self.keys = torch.empty(max(parts.n, 1), dtype=torch.int32, device=dev)

@@ -2,9 +2,9 @@
 
 - ``deposit_density(parts, rho, dx, mode="sorted")``: CIC deposit of mass density
   (M/dx^3), deterministic either way: "sorted" (cell-key sort + segmented reduction in a
-  defined float32 order, also yields the keys and particle order) or "fixed" (64-bit
-  fixed-point integer sums, order independent, the fused step's default). Reference
-  deposit.py:177-182.
+  defined float32 order, also yields the keys and particle order; the default here and in
+  the fused step) or "fixed" (64-bit fixed-point integer sums, order independent).
+  Reference deposit.py:177-182.
 - ``potential_to_accel(phi, ax, ay, az, dx)``: g = -grad(phi), central inside, one-sided on
   the faces. Reference deposit.py:204-207.
 - ``gather_acceleration(parts, ax, ay, az, acc_x, acc_y, acc_z, dx)``: CIC interpolation of

@@ -68,7 +68,10 @@ def write_snapshot(snap: Snapshot, path) -> Path:
         arrays.update({k: getattr(snap, k) for k in PARTICLE_KEYS})
         arrays.update({k: getattr(snap, k) for k in GRID_KEYS if getattr(snap, k) is not None})
         arrays.update({DIAG_PREFIX + k: np.asarray(v) for k, v in snap.diagnostics.items()})
-        np.savez_compressed(path, **arrays)
+        # through a file object: given a path, numpy appends ".npz" unless the name ends in
+        # exactly that (lower case), and the file would not be where the caller asked
+        with open(path, "wb") as fh:
+            np.savez_compressed(fh, **arrays)
     else:
         h5py = _h5py()
         with h5py.File(path, "w") as f:

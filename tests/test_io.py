@@ -1,9 +1,10 @@
-"""Snapshot I/O (reference io.py; tests mirror reference tests/test_io.py:16-110).
+"""Snapshot files: the npz layout of reference io.py:25-28 and 157-175, checked from the format.
 
-npz round-trips a Snapshot losslessly (values and dtypes of particles, optional grids,
-diagnostics always read back as ndarrays, scalar metadata) and uses the reference's key
-layout (`diag__` prefix), so files are interchangeable. HDF5 is exercised only where h5py
-is installed. CPU only.
+The layout is: one array per particle quantity (``pos`` (N,3), ``vel`` (N,3), ``mass`` (N,),
+``gid`` (N,) int32), optional ``rho``/``phi`` grids, the scalars ``step``/``time``/``dx`` as
+0-d arrays and every diagnostic under ``diag__<name>``. These tests build files by hand in
+that layout and read them, inspect what the writer puts on disk key by key, check that float
+bit patterns survive, and that snapshots of a device state come back in load order. CPU only.
 """
 
 from __future__ import annotations
@@ -11,98 +12,144 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from galaxy_collision_amd.io import Snapshot, read_snapshot, write_snapshot
-
-try:
-    import h5py  # noqa: F401
-
-    EXTS = [".npz", ".h5", ".hdf5"]
-except ImportError:
-    EXTS = [".npz"]
+from galaxy_collision_amd.io import Snapshot, read_snapshot, snapshot_from_states, write_snapshot
 
 
-def _make_snapshot(rng, n=200, g=8, with_grid=True, with_diag=True):
-    return Snapshot(
-        step=12, time=0.12,
-        pos=rng.uniform(0, g, size=(n, 3)).astype(np.float32),
-        vel=rng.normal(size=(n, 3)).astype(np.float32),
-        mass=rng.uniform(1.0, 2.0, size=n).astype(np.float32),
-        gid=rng.integers(0, 2, size=n).astype(np.int32),
-        dx=1.5,
-        rho=rng.random((g, g, g)).astype(np.float32) if with_grid else None,
-        phi=rng.random((g, g, g)).astype(np.float32) if with_grid else None,
-        diagnostics=({"energy_total": np.float64(-1.23e10), "momentum": np.array([0.1, -0.2, 0.3]),
-                      "ang_momentum": np.array([1.0, 2.0, 3.0])} if with_diag else {}),
-    )
+def _bits(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a)
+    return a.view({4: np.uint32, 8: np.uint64}[a.dtype.itemsize])
 
 
-def _assert_equal(a: Snapshot, b: Snapshot):
-    assert a.step == b.step and a.time == pytest.approx(b.time) and a.dx == pytest.approx(b.dx)
-    for key in ("pos", "vel", "mass", "gid"):
-        va, vb = getattr(a, key), getattr(b, key)
-        np.testing.assert_array_equal(va, vb)
-        assert va.dtype == vb.dtype, f"{key} dtype drift: {va.dtype} -> {vb.dtype}"
-    for key in ("rho", "phi"):
-        ga, gb = getattr(a, key), getattr(b, key)
-        assert (ga is None) == (gb is None)
-        if ga is not None:
-            np.testing.assert_array_equal(ga, gb)
-            assert ga.dtype == gb.dtype
-    assert set(a.diagnostics) == set(b.diagnostics)
-    for key in a.diagnostics:
-        assert isinstance(b.diagnostics[key], np.ndarray)
-        np.testing.assert_allclose(a.diagnostics[key], b.diagnostics[key])
-        assert np.asarray(a.diagnostics[key]).dtype == b.diagnostics[key].dtype
+def _awkward_floats(count: int) -> np.ndarray:
+    """float32 values whose bit patterns a lossy path would change: signed zeros,
+    subnormals, the extremes, infinities and a quiet NaN."""
+    special = np.array([0.0, -0.0, 1e-45, -1e-45, 3.4028235e38, -3.4028235e38, np.inf, -np.inf,
+                        np.nan, 1.0 / 3.0], dtype=np.float32)
+    return np.resize(special, count).astype(np.float32)
 
 
-@pytest.mark.parametrize("ext", EXTS)
-def test_round_trip_full(tmp_path, ext):
-    snap = _make_snapshot(np.random.default_rng(0))
-    _assert_equal(snap, read_snapshot(write_snapshot(snap, tmp_path / f"s{ext}")))
+def test_reads_a_hand_written_reference_layout(tmp_path):
+    """A file written directly with np.savez in the documented layout reads back field by field."""
+    pos = np.arange(15, dtype=np.float32).reshape(5, 3) * 0.5
+    vel = -pos[::-1].copy()
+    mass = np.array([1, 2, 4, 8, 16], dtype=np.float32)
+    gid = np.array([0, 0, 1, 1, 1], dtype=np.int32)
+    path = tmp_path / "hand.npz"
+    np.savez(path, step=np.array(7), time=np.array(3.5), dx=np.array(0.25), pos=pos, vel=vel,
+             mass=mass, gid=gid, diag__virial=np.array(0.48), diag__momentum=np.zeros(3))
+    snap = read_snapshot(path)
+    assert (snap.step, snap.time, snap.dx, snap.n) == (7, 3.5, 0.25, 5)
+    assert type(snap.step) is int and type(snap.time) is float
+    assert np.array_equal(snap.pos, pos) and np.array_equal(snap.vel, vel)
+    assert np.array_equal(snap.mass, mass) and np.array_equal(snap.gid, gid)
+    assert snap.rho is None and snap.phi is None
+    assert sorted(snap.diagnostics) == ["momentum", "virial"]
+    assert snap.diagnostics["virial"].shape == () and float(snap.diagnostics["virial"]) == 0.48
 
 
-@pytest.mark.parametrize("ext", EXTS)
-def test_round_trip_minimal_no_grid_no_diag(tmp_path, ext):
-    snap = _make_snapshot(np.random.default_rng(1), with_grid=False, with_diag=False)
-    back = read_snapshot(write_snapshot(snap, tmp_path / f"m{ext}"))
-    _assert_equal(snap, back)
-    assert back.rho is None and back.phi is None and back.diagnostics == {}
-
-
-def test_npz_key_layout_matches_reference_format(tmp_path):
-    """Arrays under their own names, scalars as 0-d arrays, diagnostics as diag__<name>
-    (reference io.py:157-175)."""
-    path = write_snapshot(_make_snapshot(np.random.default_rng(2)), tmp_path / "k.npz")
-    with np.load(path, allow_pickle=False) as d:
-        assert set(d.files) == {"step", "time", "dx", "pos", "vel", "mass", "gid", "rho", "phi",
-                                "diag__energy_total", "diag__momentum", "diag__ang_momentum"}
-        assert d["step"].shape == () and int(d["step"]) == 12
-        assert d["pos"].shape == (200, 3) and d["gid"].dtype == np.int32
-
-
-def test_write_creates_parent_dirs(tmp_path):
-    path = write_snapshot(_make_snapshot(np.random.default_rng(3)), tmp_path / "a" / "b" / "s.npz")
+def test_writer_layout_on_disk(tmp_path):
+    """What the writer stores, key by key: scalars as 0-d arrays, particle arrays with their
+    dtypes, grids only when present, diagnostics under the ``diag__`` prefix."""
+    n, g = 9, 4
+    snap = Snapshot(step=41, time=0.41, pos=np.ones((n, 3), np.float32),
+                    vel=np.zeros((n, 3), np.float32), mass=np.full(n, 0.125, np.float32),
+                    gid=np.arange(n, dtype=np.int32) % 2, dx=0.3125,
+                    rho=np.full((g, g, g), 2.0, np.float32),
+                    diagnostics={"energy": -0.25, "ang_momentum": [0.0, 0.0, 1.5]})
+    path = write_snapshot(snap, tmp_path / "deep" / "er" / "w.npz")
     assert path.is_file()
+    with np.load(path, allow_pickle=False) as d:
+        assert sorted(d.files) == sorted(["step", "time", "dx", "pos", "vel", "mass", "gid", "rho",
+                                          "diag__energy", "diag__ang_momentum"])
+        for key in ("step", "time", "dx", "diag__energy"):
+            assert d[key].ndim == 0, key
+        assert int(d["step"]) == 41 and float(d["dx"]) == 0.3125
+        assert d["pos"].dtype == np.float32 and d["pos"].shape == (n, 3)
+        assert d["gid"].dtype == np.int32 and d["mass"].dtype == np.float32
+        assert d["rho"].shape == (g, g, g) and "phi" not in d.files
+        assert d["diag__ang_momentum"].tolist() == [0.0, 0.0, 1.5]
 
 
-def test_unsupported_extension(tmp_path):
-    with pytest.raises(ValueError, match="unsupported snapshot extension"):
-        write_snapshot(_make_snapshot(np.random.default_rng(4)), tmp_path / "s.txt")
-    (tmp_path / "x.txt").write_text("x")
-    with pytest.raises(ValueError, match="unsupported snapshot extension"):
-        read_snapshot(tmp_path / "x.txt")
+def test_float_bit_patterns_survive(tmp_path):
+    n, g = 40, 3
+    pos = _awkward_floats(3 * n).reshape(n, 3)
+    vel = _awkward_floats(3 * n)[::-1].reshape(n, 3).copy()
+    mass = _awkward_floats(n)
+    phi = _awkward_floats(g ** 3).reshape(g, g, g)
+    snap = Snapshot(step=0, time=0.0, pos=pos, vel=vel, mass=mass,
+                    gid=np.full(n, -3, np.int32), phi=phi,
+                    diagnostics={"kinetic": np.float64(5e-324)})
+    back = read_snapshot(write_snapshot(snap, tmp_path / "bits.npz"))
+    for name in ("pos", "vel", "mass", "phi"):
+        got, exp = getattr(back, name), getattr(snap, name)
+        assert got.dtype == exp.dtype and np.array_equal(_bits(got), _bits(exp)), name
+    assert back.rho is None and np.all(back.gid == -3)
+    assert _bits(back.diagnostics["kinetic"].reshape(1))[0] == 1  # smallest float64 subnormal
 
 
-def test_read_missing_file(tmp_path):
-    with pytest.raises(FileNotFoundError):
-        read_snapshot(tmp_path / "nope.npz")
-
-
-def test_sim_config_validation():
-    from galaxy_collision_amd.sim import SimConfig
-
-    SimConfig(solver="multigrid", ic_preset="two_galaxy")
-    for bad in ({"solver": "cg"}, {"ic_preset": "hello"}, {"dt": 0.0}, {"steps": -1},
-                {"n_particles": 0}, {"output_cadence": -2}):
+def test_suffix_rules(tmp_path):
+    snap = Snapshot(step=1, time=0.5, pos=np.zeros((2, 3), np.float32),
+                    vel=np.zeros((2, 3), np.float32), mass=np.ones(2, np.float32),
+                    gid=np.zeros(2, np.int32))
+    # the suffix decides the format, case-insensitively
+    assert read_snapshot(write_snapshot(snap, tmp_path / "UPPER.NPZ")).step == 1
+    for bad in ("s.json", "s", "s.npy"):
         with pytest.raises(ValueError):
-            SimConfig(**bad)
+            write_snapshot(snap, tmp_path / bad)
+    (tmp_path / "notes.csv").write_text("step,1\n")
+    with pytest.raises(ValueError):
+        read_snapshot(tmp_path / "notes.csv")
+    with pytest.raises(FileNotFoundError):
+        read_snapshot(tmp_path / "never_written.npz")
+
+
+class _HostField:
+    def __init__(self, a):
+        self._a = np.asarray(a)
+
+    def to_numpy(self):
+        return self._a.copy()
+
+
+class _ReorderedParticles:
+    """Host stand-in for a ParticleState whose slots a chain has permuted: slot s holds the
+    particle loaded at index ``pid[s]``."""
+
+    def __init__(self, pos, vel, mass, gid, pid):
+        self.pos_x, self.pos_y, self.pos_z = (_HostField(pos[pid, a]) for a in range(3))
+        self.vel_x, self.vel_y, self.vel_z = (_HostField(vel[pid, a]) for a in range(3))
+        self.mass, self.gid = _HostField(mass[pid]), _HostField(gid[pid])
+        self._pid = pid
+
+    def in_load_order(self):
+        return np.argsort(self._pid)
+
+
+def test_snapshot_of_reordered_state_is_in_load_order(tmp_path):
+    rng = np.random.default_rng(5)
+    n = 64
+    pos = rng.uniform(0, 10, (n, 3)).astype(np.float32)
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(1, 2, n).astype(np.float32)
+    gid = (np.arange(n) >= n // 2).astype(np.int32)
+    parts = _ReorderedParticles(pos, vel, mass, gid, rng.permutation(n))
+    snap = snapshot_from_states(3, 0.03, parts, dx=0.5, diagnostics={"energy": -1.0})
+    back = read_snapshot(write_snapshot(snap, tmp_path / "order.npz"))
+    assert np.array_equal(back.pos, pos) and np.array_equal(back.vel, vel)
+    assert np.array_equal(back.mass, mass) and np.array_equal(back.gid, gid)
+    assert back.step == 3 and back.dx == 0.5 and float(back.diagnostics["energy"]) == -1.0
+
+
+def test_hdf5_needs_h5py(tmp_path):
+    snap = Snapshot(step=2, time=1.0, pos=np.zeros((3, 3), np.float32),
+                    vel=np.ones((3, 3), np.float32), mass=np.ones(3, np.float32),
+                    gid=np.zeros(3, np.int32), diagnostics={"energy": 1.0})
+    try:
+        import h5py  # noqa: F401
+    except ImportError:
+        with pytest.raises(ImportError, match="h5py"):
+            write_snapshot(snap, tmp_path / "s.h5")
+        return
+    back = read_snapshot(write_snapshot(snap, tmp_path / "s.hdf5"))
+    assert back.step == 2 and np.array_equal(back.vel, snap.vel)
+    assert float(back.diagnostics["energy"]) == 1.0

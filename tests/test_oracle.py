@@ -396,6 +396,63 @@ def test_particle_scalars_known_values():
     assert np.allclose(s[8:11], [2.0, 6.0, 0.0])
 
 
+def _symmetric_pairs(radii, masses, centre=(0.0, 0.0, 0.0)):
+    """Equal-mass pairs at +-r along rotating axes: centre of mass exactly at ``centre``."""
+    pos, mass = [], []
+    for k, (r, m) in enumerate(zip(radii, masses)):
+        e = np.zeros(3)
+        e[k % 3] = r
+        pos += [e, -e]
+        mass += [m, m]
+    return np.array(pos) + np.asarray(centre), np.array(mass, dtype=np.float64)
+
+
+def test_lagrangian_radius_hand_built_sets():
+    """Symmetric pairs (CM at the centre by construction): the radius is that of the first
+    particle, in radius order, at which the enclosed mass reaches the fraction."""
+    pos, m = _symmetric_pairs([1.0, 2.0, 3.0, 4.0], [1.0] * 4, centre=(5.0, -7.0, 2.0))
+    assert O.lagrangian_radius(m, pos, 0.5) == pytest.approx(2.0, rel=1e-12)
+    assert O.lagrangian_radius(m, pos, 0.25) == pytest.approx(1.0, rel=1e-12)
+    assert O.lagrangian_radius(m, pos, 0.51) == pytest.approx(3.0, rel=1e-12)
+    assert O.lagrangian_radius(m, pos, 1.0) == pytest.approx(4.0, rel=1e-12)
+    # mass weighting: the heavy inner pair alone holds 60% of the mass
+    pos, m = _symmetric_pairs([1.0, 2.0, 3.0], [3.0, 1.0, 1.0])
+    assert O.lagrangian_radius(m, pos, 0.5) == pytest.approx(1.0, rel=1e-12)
+    assert O.lagrangian_radius(m, pos, 0.7) == pytest.approx(2.0, rel=1e-12)
+    # radii are measured about the centre of mass, not the origin
+    pos, m = _symmetric_pairs([2.0, 6.0], [1.0, 1.0], centre=(100.0, 100.0, 100.0))
+    assert O.lagrangian_radius(m, pos, 0.5) == pytest.approx(2.0, rel=1e-12)
+
+
+def test_lagrangian_radius_uniform_ball():
+    """Uniform ball of radius R: M(<r) = (r/R)^3, so r_half = R * 2^(-1/3)."""
+    rng = np.random.default_rng(11)
+    n, R = 200_000, 3.0
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = d * (R * rng.random(n) ** (1.0 / 3.0))[:, None] + np.array([4.0, 4.0, 4.0])
+    r_half = O.lagrangian_radius(np.ones(n), pos, 0.5)
+    # sampling error of the median radius ~0.03%; 0.5% is many sigma
+    assert r_half == pytest.approx(R * 2.0 ** (-1.0 / 3.0), rel=5e-3)
+
+
+def test_lagrangian_radius_plummer_closed_form():
+    """Plummer sphere: M(<r)/M = r^3/(r^2+a^2)^(3/2) gives r_half = a/sqrt(2^(2/3)-1) ~ 1.305a;
+    also the 10% and 90% radii from the same inverse."""
+    rng = np.random.default_rng(12)
+    n, a = 400_000, 1.5
+    u = rng.random(n)
+    r = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = d * r[:, None]
+    m = np.full(n, 1.0 / n)
+    for q, tol in ((0.5, 1e-2), (0.1, 2e-2), (0.9, 2e-2)):
+        exact = a / math.sqrt(q ** (-2.0 / 3.0) - 1.0)
+        assert O.lagrangian_radius(m, pos, q) == pytest.approx(exact, rel=tol), q
+    assert O.lagrangian_radius(m, pos, 0.5) == pytest.approx(1.3048 * a, rel=1e-2)
+
+
 # ---------------------------------------------------------------- C baseline port
 
 def test_cpu_port_kernels_match_numpy_restatement():
@@ -433,3 +490,27 @@ def test_cpu_port_mg_matches_numpy_mg():
     L.cpu_mg_destroy(h)
     assert cyc_c == cyc_np
     assert np.array_equal(phi_c, phi_np)
+
+
+def test_cpu_port_kick_drift_match_numpy_restatement():
+    """The baseline port's kick and drift (reference integrator.py:27-60) equal the float32
+    restatement bit for bit, and both equal v + a*h exactly on dyadic inputs."""
+    rng = np.random.default_rng(21)
+    npart, h, dt = 5000, 0.005, 0.01
+    v = [rng.normal(size=npart).astype(F32) for _ in range(3)]
+    a = [rng.normal(size=npart).astype(F32) for _ in range(3)]
+    x = [rng.uniform(0, 10, npart).astype(F32) for _ in range(3)]
+    L = O.lib()
+    vc = [c.copy() for c in v]
+    L.cpu_kick(*(O._fp(c) for c in vc), *(O._fp(c) for c in a), npart, h)
+    for k in range(3):
+        assert np.array_equal(vc[k], O.kick(v[k], a[k], h))
+    xc = [c.copy() for c in x]
+    L.cpu_drift(*(O._fp(c) for c in xc), *(O._fp(c) for c in vc), npart, dt)
+    for k in range(3):
+        assert np.array_equal(xc[k], O.drift(x[k], vc[k], dt))
+    # dyadic values: the float32 update is exact
+    vq = [np.full(8, 0.5, F32) for _ in range(3)]
+    aq = [np.full(8, 2.0, F32) for _ in range(3)]
+    L.cpu_kick(*(O._fp(c) for c in vq), *(O._fp(c) for c in aq), 8, 0.25)
+    assert all(np.all(c == 1.0) for c in vq)
