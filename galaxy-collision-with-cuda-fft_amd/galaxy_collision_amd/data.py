@@ -184,6 +184,23 @@ class ParticleState:
         t = self.pid.tensor
         t.copy_(torch.arange(t.numel(), dtype=torch.int32, device=t.device))
 
+    def load_tensors(self, pos: torch.Tensor, vel: torch.Tensor | None = None,
+                     mass: torch.Tensor | None = None, gid: torch.Tensor | None = None) -> None:
+        """``load`` from (N,3)/(N,) torch tensors (device to device when they are on it)."""
+        if pos.shape != (self.n, 3):
+            raise ValueError(f"pos must be ({self.n}, 3), got {tuple(pos.shape)}")
+        for a, f in enumerate((self.pos_x, self.pos_y, self.pos_z)):
+            f.tensor.copy_(pos[:, a])
+        if vel is not None:
+            for a, f in enumerate((self.vel_x, self.vel_y, self.vel_z)):
+                f.tensor.copy_(vel[:, a])
+        if mass is not None:
+            self.mass.tensor.copy_(mass)
+        if gid is not None:
+            self.gid.tensor.copy_(gid)
+        t = self.pid.tensor
+        t.copy_(torch.arange(t.numel(), dtype=torch.int32, device=t.device))
+
     def positions(self) -> np.ndarray:
         return np.stack([f.to_numpy() for f in (self.pos_x, self.pos_y, self.pos_z)], axis=1)
 
