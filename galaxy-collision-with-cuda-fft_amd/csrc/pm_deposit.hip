@@ -14,6 +14,9 @@
 //   5. Node value = 0 + sum over corners c = 0..7 (bits cx,cy,cz) of the corner-c sum of
 //      cell (i-cx, j-cy, k-cz), skipping absent cells, in that order.
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <type_traits>
 
 #include "pm_cic.h"
 #include "pm_common.h"
@@ -1019,6 +1022,53 @@ pm_status alloc(T** ptr, int64_t count, const char* what) {
   return PM_OK;
 }
 
+// Grid and sort parameters of a plan for `cap` particles on an n^3 grid.
+void init_plan_shape(pm_deposit_plan* p, int n, int64_t cap) {
+  p->n = n;
+  p->cap = cap;
+  const int64_t m = static_cast<int64_t>(n) + 1;
+  p->sentinel = static_cast<uint32_t>(m * m * m);
+  int key_bits = 1;
+  while ((int64_t(1) << key_bits) <= static_cast<int64_t>(p->sentinel)) ++key_bits;
+  p->passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  for (int s = 0; s < p->passes; ++s)
+    p->digit_bits[s] = key_bits / p->passes + (s < key_bits % p->passes ? 1 : 0);
+}
+
+// Calls f(buffer pointer, element count, name) for every sorted-mode scratch buffer of a
+// plan whose shape init_plan_shape set, stopping at the first error. The allocating plan
+// and the no-allocation entry point lay out this one list.
+template <class F>
+pm_status for_each_buffer(pm_deposit_plan* p, F&& f) {
+  const int64_t cap = p->cap;
+  const int64_t m = static_cast<int64_t>(p->n) + 1;
+  const int64_t tiles = (cap + kTile - 1) / kTile;
+  const int64_t chunks = (cap + kChunk - 1) / kChunk;
+  const int64_t nbricks = nbricks_of(p->n);
+  pm_status st = PM_OK;
+  if ((st = f(p->keys[0], cap, "hipMalloc keys")) ||
+      (st = f(p->keys[1], cap, "hipMalloc keys")) ||
+      (st = f(p->vals[0], cap, "hipMalloc vals")) ||
+      (st = f(p->vals[1], cap, "hipMalloc vals")) ||
+      (st = f(p->hist, tiles << kMaxDigitBits, "hipMalloc hist")) ||
+      (st = f(p->digit, int64_t(1) << kMaxDigitBits, "hipMalloc digit")) ||
+      (st = f(p->sums, 8 * cap, "hipMalloc sums")) ||
+      (st = f(p->head, 8 * chunks, "hipMalloc head")) ||
+      (st = f(p->tail, 8 * chunks, "hipMalloc tail")) ||
+      (st = f(p->tail_slot, chunks, "hipMalloc tail_slot")) ||
+      (st = f(p->head_end, chunks + 16, "hipMalloc head_end")) ||
+      (st = f(p->mid_pieces, chunks, "hipMalloc mid_pieces")) ||
+      (st = f(p->bricks, nbricks, "hipMalloc bricks")) ||
+      (st = f(p->brick_list, nbricks, "hipMalloc brick_list")) ||
+      (st = f(p->brick_count, 1, "hipMalloc brick_count")) ||
+      (st = f(p->map, m * m * m, "hipMalloc map")) ||
+      (st = f(p->big, chunks, "hipMalloc big")) ||
+      (st = f(p->big_count, 1, "hipMalloc big_count")) ||
+      (st = f(p->big_off, chunks + 1, "hipMalloc big_off")))
+    return st;
+  return PM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1030,37 +1080,11 @@ pm_status pm_deposit_plan_create(int32_t n, int64_t cap, pm_deposit_plan** out) 
   PM_REQUIRE(cap >= 1 && cap < (int64_t(1) << 31),
              "pm_deposit_plan_create: max_particles must be in [1, 2^31)");
   auto* p = new pm_deposit_plan();
-  p->n = n;
-  p->cap = cap;
-  const int64_t m = static_cast<int64_t>(n) + 1;
-  p->sentinel = static_cast<uint32_t>(m * m * m);
-  int key_bits = 1;
-  while ((int64_t(1) << key_bits) <= static_cast<int64_t>(p->sentinel)) ++key_bits;
-  p->passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
-  for (int s = 0; s < p->passes; ++s)
-    p->digit_bits[s] = key_bits / p->passes + (s < key_bits % p->passes ? 1 : 0);
-  const int64_t tiles = (cap + kTile - 1) / kTile;
-  const int64_t chunks = (cap + kChunk - 1) / kChunk;
-  pm_status st = PM_OK;
-  if ((st = alloc(&p->keys[0], cap, "hipMalloc keys")) ||
-      (st = alloc(&p->keys[1], cap, "hipMalloc keys")) ||
-      (st = alloc(&p->vals[0], cap, "hipMalloc vals")) ||
-      (st = alloc(&p->vals[1], cap, "hipMalloc vals")) ||
-      (st = alloc(&p->hist, tiles << kMaxDigitBits, "hipMalloc hist")) ||
-      (st = alloc(&p->digit, int64_t(1) << kMaxDigitBits, "hipMalloc digit")) ||
-      (st = alloc(&p->sums, 8 * cap, "hipMalloc sums")) ||
-      (st = alloc(&p->head, 8 * chunks, "hipMalloc head")) ||
-      (st = alloc(&p->tail, 8 * chunks, "hipMalloc tail")) ||
-      (st = alloc(&p->tail_slot, chunks, "hipMalloc tail_slot")) ||
-      (st = alloc(&p->head_end, chunks + 16, "hipMalloc head_end")) ||
-      (st = alloc(&p->mid_pieces, chunks, "hipMalloc mid_pieces")) ||
-      (st = alloc(&p->bricks, nbricks_of(n), "hipMalloc bricks")) ||
-      (st = alloc(&p->brick_list, nbricks_of(n), "hipMalloc brick_list")) ||
-      (st = alloc(&p->brick_count, 1, "hipMalloc brick_count")) ||
-      (st = alloc(&p->map, m * m * m, "hipMalloc map")) ||
-      (st = alloc(&p->big, chunks, "hipMalloc big")) ||
-      (st = alloc(&p->big_count, 1, "hipMalloc big_count")) ||
-      (st = alloc(&p->big_off, chunks + 1, "hipMalloc big_off"))) {
+  init_plan_shape(p, n, cap);
+  const pm_status st = for_each_buffer(p, [](auto*& ptr, int64_t count, const char* what) {
+    return alloc(&ptr, count, what);
+  });
+  if (st != PM_OK) {
     free_plan(p);
     return st;
   }
@@ -1323,6 +1347,92 @@ pm_status pm_deposit_cic_keyed(pm_deposit_plan* plan, const float* px, const flo
                                hipStream_t stream) {
   PM_REQUIRE(np == 0 || keys, "pm_deposit_cic_keyed: null keys");
   return deposit_cic(plan, px, py, pz, mass, np, dx, rho, keys, nullptr, perm_out, stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// The deposit without a plan: its scratch is a static device array, carved per call.
+// ------------------------------------------------------------------------------------
+
+// Every device the library runs on holds one copy of this array until the process ends.
+// Host code never passes the array's name to a kernel: it stands for the device array only
+// in hipGetSymbolAddress, which gives each device's own address (noalloc_base).
+static __device__ __align__(256) unsigned char scratch[PM_DEPOSIT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// The scratch array's address on the current device, looked up once per device.
+pm_status noalloc_base(unsigned char** out) {
+  int dev = 0;
+  PM_TRY_HIP(hipGetDevice(&dev));
+  PM_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices,
+             "pm_deposit_cic_noalloc: device ordinal beyond the supported 64");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return pm::hip_fail("hipGetSymbolAddress (deposit scratch)", e);
+    PM_REQUIRE(ptr, "hipGetSymbolAddress gave no address for the deposit scratch");
+    g_noalloc_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_noalloc_ptr[dev];
+  return PM_OK;
+}
+
+// Bytes the plan's buffers take when each starts on a 256-byte boundary.
+int64_t carved_bytes(pm_deposit_plan* p) {
+  int64_t off = 0;
+  (void)for_each_buffer(p, [&](auto*& ptr, int64_t count, const char*) {
+    using T = std::remove_reference_t<decltype(*ptr)>;
+    off = (off + 255) & ~int64_t(255);
+    off += static_cast<int64_t>(sizeof(T)) * (count > 0 ? count : 1);
+    return PM_OK;
+  });
+  return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+pm_status pm_deposit_cic_noalloc(int32_t n, const float* px, const float* py, const float* pz,
+                                 const float* mass, int64_t np, float dx, float* rho,
+                                 uint32_t* keys_out, int32_t* perm_out, hipStream_t stream) {
+  pm::clear_error();
+  PM_REQUIRE(n >= 1 && n <= 1024, "pm_deposit_cic_noalloc: grid_size must be in [1, 1024]");
+  PM_REQUIRE(np >= 0 && np < (int64_t(1) << 31),
+             "pm_deposit_cic_noalloc: particle count must be in [0, 2^31)");
+  pm_deposit_plan plan;  // host bookkeeping only; every buffer lies in the scratch array
+  init_plan_shape(&plan, n, np > 0 ? np : 1);
+  const int64_t need = carved_bytes(&plan);
+  if (need > PM_DEPOSIT_NOALLOC_SCRATCH_BYTES) {
+    pm::set_error("pm_deposit_cic_noalloc: " + std::to_string(np) + " particles on a " +
+                  std::to_string(n) + "^3 grid need " + std::to_string(need) +
+                  " bytes of scratch; the static scratch holds " +
+                  std::to_string(int64_t(PM_DEPOSIT_NOALLOC_SCRATCH_BYTES)));
+    return PM_ERR_CAPACITY;
+  }
+  unsigned char* base = nullptr;
+  pm_status st = noalloc_base(&base);
+  if (st != PM_OK) return st;
+  int64_t off = 0;
+  (void)for_each_buffer(&plan, [&](auto*& ptr, int64_t count, const char*) {
+    using T = std::remove_reference_t<decltype(*ptr)>;
+    off = (off + 255) & ~int64_t(255);
+    ptr = reinterpret_cast<T*>(base + off);
+    off += static_cast<int64_t>(sizeof(T)) * (count > 0 ? count : 1);
+    return PM_OK;
+  });
+  // map_clean is false: the deposit resets the whole cell map, the brick marks and the big
+  // cell count itself, whatever an earlier call left in the array; no timer records
+  pm::TimingPause no_timers;
+  return deposit_cic(&plan, px, py, pz, mass, np, dx, rho, nullptr, keys_out, perm_out,
+                     stream);
 }
 
 }  // extern "C"

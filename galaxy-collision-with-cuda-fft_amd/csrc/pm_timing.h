@@ -14,6 +14,15 @@ namespace pm {
 
 // Event-pair timers (KernelTimer) record: mode 1.
 bool timing_enabled();
+// While a TimingPause lives on the calling thread no timer records (and no timing event is
+// created): for entry points that must not acquire resources.
+class TimingPause {
+ public:
+  TimingPause();
+  ~TimingPause();
+  TimingPause(const TimingPause&) = delete;
+  TimingPause& operator=(const TimingPause&) = delete;
+};
 // Dispatch-recorded timers (LaunchTimer, PM_LAUNCH_TIMED) record: modes 1 and 2. Mode 2 adds
 // no packets to the streams, so the timed work runs as it does untimed (graphs included).
 bool launch_timing_enabled();

@@ -65,9 +65,17 @@ pm_status drain() {
 
 namespace pm {
 
-bool timing_enabled() { return g_mode.load(std::memory_order_relaxed) == 1; }
+thread_local int t_paused = 0;
 
-bool launch_timing_enabled() { return g_mode.load(std::memory_order_relaxed) != 0; }
+bool timing_enabled() { return t_paused == 0 && g_mode.load(std::memory_order_relaxed) == 1; }
+
+bool launch_timing_enabled() {
+  return t_paused == 0 && g_mode.load(std::memory_order_relaxed) != 0;
+}
+
+TimingPause::TimingPause() { ++t_paused; }
+
+TimingPause::~TimingPause() { --t_paused; }
 
 KernelTimer::KernelTimer(const char* name, double bytes, hipStream_t stream) {
   if (!timing_enabled()) return;

@@ -146,6 +146,7 @@ _SIGNATURES = {
     "pm_deposit_plan_destroy": [_vp],
     "pm_deposit_cic": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "pm_deposit_cic_keyed": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
+    "pm_deposit_cic_noalloc": [_i32, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "pm_deposit_plan_set_timing": [_vp, _i32],
     "pm_deposit_plan_set_mode": [_vp, _i32],
     "pm_deposit_last_timing": [_vp, ctypes.POINTER(_f)],

@@ -83,6 +83,24 @@ pm_status pm_deposit_cic_keyed(pm_deposit_plan* plan, const float* pos_x, const 
                                float dx, float* rho, const uint32_t* keys, int32_t* perm_out,
                                hipStream_t stream);
 
+/* pm_deposit_cic without a plan and without allocating: no counterpart in the reference. It
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The sort and reduction scratch is a
+ * static device array of PM_DEPOSIT_NOALLOC_SCRATCH_BYTES (96 MiB) that each device the
+ * library runs on holds until the process ends; one call needs about 55.3 bytes per
+ * particle plus 4 bytes per cell of the (n+1)^3 extended grid (at most 1,799,896 particles
+ * on 64^3, 1,664,192 on 128^3, 589,312 on 256^3) and returns PM_ERR_CAPACITY, before any device
+ * work, when that exceeds the array. The result (rho, keys_out, perm_out) is bit for bit
+ * pm_deposit_cic's in sorted mode. Calls no hipMalloc/hipFree and records no timers. Every
+ * call on one device shares that device's array and finds the last call's contents there
+ * (the call resets what it needs itself): calls that overlap on one device (other streams
+ * or threads) race and must be serialized by the caller. */
+#define PM_DEPOSIT_NOALLOC_SCRATCH_BYTES (96ll << 20)
+pm_status pm_deposit_cic_noalloc(int32_t grid_size, const float* pos_x, const float* pos_y,
+                                 const float* pos_z, const float* mass, int64_t n_particles,
+                                 float dx, float* rho, uint32_t* keys_out, int32_t* perm_out,
+                                 hipStream_t stream);
+
 /* Timing hook: per-phase device times (ms) of the last pm_deposit_cic on this plan when
  * pm_deposit_plan_set_timing(plan, 1) was set before it; phases = key, sort, reduce, assemble. */
 pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable);
