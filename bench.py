@@ -13,12 +13,22 @@ Workloads (seeded synthetic inputs, built on the host, resident in HBM before ti
 A step is one fused KDK step (kick+drift, deposit, Poisson solve, -grad, gather+kick); the
 timed steps run back to back, so one step's closing kick shares a particle pass with the next
 step's opening kick and drift (bitwise the same results as separate steps).
-Pass 1 times --steps steps with no instrumentation (the reported value). Pass 2 repeats
-them with per-kernel HIP events on the launch stream, giving per-kernel times and the
-roofline figures of the dominant kernel. Pass 3 (collisions) restarts from the initial
-state once per multigrid warm start (previous phi, extrapolation, extrapolation plus the
-trained network) and for the float32 reference multigrid: the learned-vs-plain comparison. The CPU baseline runs the reference algorithm
-(OpenMP C port: atomic CIC scatter, plain float32 multigrid tol 1e-3) on the same input.
+The collision inputs are launched with their disks on the rotation curve the grid produces
+plus the recipe's 10% dispersion (ic.equilibrate_disk_velocities, as reference
+bench.py:119-123 does). Pass 1 times --steps steps with no instrumentation (the reported
+value). Pass 2 repeats them with per-kernel HIP events on the launch stream, giving
+per-kernel times, the roofline figures of the dominant kernel and of every finest-level
+multigrid kernel. Pass 3 is the reference's per-stage protocol (bench.py:66-72, 176-186):
+the unfused stages, each bracketed by device synchronizations. Pass 4 continues the run to
+step --steady-after and times --steps more steps (the steady state after the particles have
+mixed). The accuracy series restarts from the initial state and runs --accuracy-steps steps,
+comparing phi and the particle forces every --accuracy-cadence steps with the exact float64
+solution of the discrete problem and with the zero-padded 2n^3 float64 FFT (the 10M / 256^3
+configuration's check). The A/B pass (collisions) restarts from the initial state once per
+multigrid warm start (previous phi, extrapolation, extrapolation plus the trained network)
+and for the float32 reference multigrid: the learned-vs-plain comparison. The CPU baseline
+runs the reference algorithm (OpenMP C port: atomic CIC scatter, plain float32 multigrid tol
+1e-3) on the same input with the same warmup / throughput / per-stage protocol.
 
 Prints one JSON line. ``--gpus`` above 1 is refused: this build runs on one GPU.
 """
@@ -37,6 +47,9 @@ sys.path.insert(0, str(ROOT / "galaxy-collision-with-cuda-fft_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 REFERENCE_STEPS_PER_S = 24.8  # reference docs/performance.md:17 (RTX 3070, 256^3, 10M)
+# reference docs/performance.md:28-32, 10M particles on 256^3, RTX 3070, ms per step
+REFERENCE_STAGE_MS = {"deposit": 2.54, "solve": 27.8, "grad": 1.17, "gather": 5.88,
+                      "integrate": 3.18}
 
 WORKLOADS = {
     "collision10m": dict(kind="two_galaxy", n_per=5_000_000, grid=256, box=40.0, dt=0.01,
@@ -49,6 +62,20 @@ WORKLOADS = {
                           tol=2e-6, sweeps=(1, 1), reorder_every=8),
     "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
 }
+
+
+def equilibrate(icr, w: dict, dx: float, grav: float, seed: int) -> None:
+    """Disk velocities from the grid's rotation curve plus the recipe's 10% dispersion,
+    measured with a cold solve of the workload's solver settings (before any timing)."""
+    from galaxy_collision_amd.data import GridState
+    from galaxy_collision_amd.ic import equilibrate_disk_velocities
+    from galaxy_collision_amd.solver import make_solver
+
+    grid = GridState(w["grid"])
+    solver = make_solver("multigrid_mp", w["grid"], dx=dx, grav_constant=grav)
+    equilibrate_disk_velocities(icr, dx=dx, solver=solver, rho=grid.rho, phi=grid.phi,
+                                ax=grid.ax, ay=grid.ay, az=grid.az, dispersion=0.1, seed=seed)
+    del grid, solver
 
 
 def build_ic(w: dict, seed: int):
@@ -94,15 +121,50 @@ def phi_accuracy(chain, grav: float) -> dict:
     return acc(chain.grid.rho, chain.grid.phi, chain.n, chain.dx, grav)
 
 
+def accuracy_series(icr, w: dict, dx: float, grav: float, make, steps: int,
+                    cadence: int) -> dict:
+    """A run of `steps` steps from the initial state with the benchmark's solver, checked
+    every `cadence` steps (and at step 0) against the exact discrete solution and the
+    zero-padded float64 FFT: phi and particle-force relative L2 errors."""
+    import torch
+
+    from galaxy_collision_amd.accuracy import Checkpoint
+    from galaxy_collision_amd.chain import PMChain
+    from galaxy_collision_amd.data import ParticleState
+
+    parts = ParticleState(icr.n)
+    parts.load(icr.pos, icr.vel, icr.mass, icr.gid)
+    chain = PMChain(parts, w["grid"], dx, make(), reorder_every=w.get("reorder_every", 0))
+    check = Checkpoint(w["grid"], dx, grav)
+    chain.prime()
+    points = [{"step": 0, **check.measure(parts, chain.grid.rho, chain.grid.phi)}]
+    done = 0
+    while done < steps:
+        k = min(cadence, steps - done)
+        chain.run(k, w["dt"])
+        done += k
+        points.append({"step": done, **check.measure(parts, chain.grid.rho, chain.grid.phi)})
+        print(f"[bench] accuracy step {done}: {points[-1]}", file=sys.stderr, flush=True)
+    keys = [k for k in points[0] if k != "step"]
+    out = {"steps": steps, "cadence": cadence,
+           "max": {k: max(p[k] for p in points) for k in keys},
+           "last": points[-1], "points": points,
+           "vcycles_per_step": float(sum(chain.last_cycles[1:]) / max(len(chain.last_cycles) - 1, 1))}
+    del chain, check, parts
+    torch.cuda.empty_cache()
+    return out
+
+
 def warm_start_ab(icr, w: dict, dx: float, grav: float, steps: int) -> dict:
     """The same collision from the same initial state, stepped with each multigrid warm start
     (previous phi as in the reference, time extrapolation, extrapolation plus the trained
-    warm-start network) at the same tolerance, and with the float32 reference multigrid at
-    its default tolerance: V-cycles and wall time per step, phi error of the last step."""
+    warm-start network) at the benchmark's tolerance and V-cycle shape, and with the float32
+    reference multigrid at its default tolerance: V-cycles and wall time per step, phi error
+    of the last step."""
     import numpy as np
     import torch
 
-    from galaxy_collision_amd.accuracy import phi_rel_error_discrete
+    from galaxy_collision_amd.accuracy import phi_rel_error_discrete_device
     from galaxy_collision_amd.chain import PMChain
     from galaxy_collision_amd.data import ParticleState
     from galaxy_collision_amd.solver import make_solver
@@ -121,7 +183,12 @@ def warm_start_ab(icr, w: dict, dx: float, grav: float, steps: int) -> dict:
         parts.load(icr.pos, icr.vel, icr.mass, icr.gid)
         kw = {"dx": dx, "grav_constant": grav}
         if name == "multigrid_mp":
+            # the benchmark's own solver settings (tolerance, V-cycle shape)
             kw.update(warm=warm, pool=pool, net=net if warm == "learned" else None)
+            if "tol" in w:
+                kw["tol"] = w["tol"]
+            if "sweeps" in w:
+                kw["pre_sweeps"], kw["post_sweeps"] = w["sweeps"]
         solver = make_solver(name, w["grid"], **kw)
         chain = PMChain(parts, w["grid"], dx, solver)
         chain.prime()
@@ -142,8 +209,8 @@ def warm_start_ab(icr, w: dict, dx: float, grav: float, steps: int) -> dict:
         out[key] = {
             "vcycles_per_step": float(np.mean(chain.last_cycles)),
             "ms_per_step": wall / steps * 1e3,
-            "phi_rel_l2_vs_exact_discrete": phi_rel_error_discrete(
-                chain.grid.rho.to_numpy(), chain.grid.phi.to_numpy(), dx, grav),
+            "phi_rel_l2_vs_exact_discrete": phi_rel_error_discrete_device(
+                chain.grid.rho, chain.grid.phi, dx, grav),
         }
         if kept:
             out[key]["net_kept_fraction"] = float(np.mean(kept))
@@ -154,7 +221,8 @@ def warm_start_ab(icr, w: dict, dx: float, grav: float, steps: int) -> dict:
 
 
 def cpu_baseline(icr, w: dict, dx: float, budget_s: float, threads: int | None) -> dict:
-    """The reference algorithm on host cores: prime once, then time whole steps."""
+    """The reference algorithm on host cores with the reference's protocol: prime, one
+    untimed warmup step, a throughput pass, then a per-stage pass (each stage timed)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pm_oracle
 
@@ -162,17 +230,27 @@ def cpu_baseline(icr, w: dict, dx: float, budget_s: float, threads: int | None) 
     t0 = time.perf_counter()
     chain.accel()
     prime_s = time.perf_counter() - t0
-    steps, elapsed = 0, 0.0
-    while steps < 1 or (elapsed < budget_s and steps < 10):
-        t = time.perf_counter()
+    t0 = time.perf_counter()
+    chain.step(w["dt"])  # warmup
+    step_s = time.perf_counter() - t0
+    # the two passes share the budget; at most 10 steps each
+    k = max(1, min(10, int(0.5 * budget_s / max(step_s, 1e-9))))
+    t0 = time.perf_counter()
+    for _ in range(k):
         chain.step(w["dt"])
-        elapsed += time.perf_counter() - t
-        steps += 1
+    elapsed = time.perf_counter() - t0
+    stages: dict = {}
+    for _ in range(k):
+        for name, sec in chain.stage_times(w["dt"]).items():
+            stages[name] = stages.get(name, 0.0) + sec
     chain.close()
-    return {"value": steps / elapsed, "unit": "steps/s", "cores": chain.threads, "kind": "port",
+    return {"value": k / elapsed, "unit": "steps/s", "cores": chain.threads, "kind": "port",
+            "per_stage_ms": {s: stages[s] / k * 1e3 for s in ("integrate", "deposit", "solve",
+                                                              "grad", "gather")},
             "sample": (f"{icr.n:,} particles on {w['grid']}^3, reference algorithm (OpenMP C "
-                       f"port, atomic CIC, plain float32 multigrid tol 1e-3): {steps} KDK steps "
-                       f"timed after a cold prime of {prime_s:.1f} s")}
+                       f"port, atomic CIC, plain float32 multigrid tol 1e-3): prime "
+                       f"{prime_s:.1f} s, 1 warmup step, {k} steps timed back to back, then "
+                       f"{k} steps with every stage timed")}
 
 
 def main() -> int:
@@ -197,6 +275,14 @@ def main() -> int:
     ap.add_argument("--no-ab", action="store_true",
                     help="skip the warm-start A/B pass (learned vs plain multigrid)")
     ap.add_argument("--ab-steps", type=int, default=20)
+    ap.add_argument("--no-equilibrate", action="store_true",
+                    help="keep the analytic disk velocities (no grid rotation curve)")
+    ap.add_argument("--no-stage-pass", action="store_true", help="skip the per-stage pass")
+    ap.add_argument("--steady-after", type=int, default=300,
+                    help="time --steps more steps after the run reaches this step (0: skip)")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the accuracy series from the initial state (0: skip)")
+    ap.add_argument("--accuracy-cadence", type=int, default=50)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy a 256 MiB buffer 3 times (a known-size kernel for scaling "
                          "profiler byte counters)")
@@ -233,6 +319,9 @@ def main() -> int:
     grav = 1.0
     t_ic = time.perf_counter()
     icr, dx = build_ic(w, args.seed)
+    equilibrated = w["kind"] == "two_galaxy" and not args.no_equilibrate
+    if equilibrated:
+        equilibrate(icr, w, dx, grav, args.seed)
     ic_s = time.perf_counter() - t_ic
     outside = box_fit_fraction(icr, w["grid"], dx)
     print(f"[bench] workload={args.workload} N={icr.n:,} grid={w['grid']}^3 dx={dx:.5f} "
@@ -250,7 +339,8 @@ def main() -> int:
     warm = args.warm or (w.get("warm", "previous") if solver_name == w["solver"] else "previous")
     if solver_name in ("multigrid", "multigrid_mp") and warm != "previous":
         kw["warm"] = warm
-    solver = make_solver(solver_name, w["grid"], **kw)
+    make = lambda: make_solver(solver_name, w["grid"], **kw)  # noqa: E731
+    solver = make()
     reorder = args.reorder_every if args.reorder_every is not None else w.get("reorder_every", 0)
     chain = PMChain(parts, w["grid"], dx, solver, reorder_every=reorder)
     chain.prime()
@@ -321,11 +411,50 @@ def main() -> int:
         print(f"[bench] roofline: {name} {avg_s * 1e6:.2f} us/launch, {achieved:.0f} GB/s",
               file=sys.stderr, flush=True)
 
+    mg_fractions = {}
+    for k, r in kernels.items():
+        finest = k.endswith(f"[{w['grid']}]") or "[" not in k
+        if k.startswith("mg.") and finest and r["GBps"] > 0:
+            mg_fractions[k] = {"GBps": r["GBps"], "frac": r["GBps"] / HBM_PEAK_GBS,
+                               "avg_launch_us": r["avg_launch_us"],
+                               "launches_per_step": r["launches_per_step"]}
+
+    # pass 3: the reference's per-stage protocol (unfused stages, each synchronized)
+    per_stage = None
+    if not args.no_stage_pass:
+        from galaxy_collision_amd.benchmark import UnfusedStep
+
+        per_stage = UnfusedStep(chain, w["dt"]).stage_pass(args.steps)
+        print(f"[bench] pass 3 per stage (ms): {per_stage}", file=sys.stderr, flush=True)
+
+    # pass 4: the steady state after the particles have mixed
+    steady = None
+    if args.steady_after > 0:
+        ahead = args.steady_after - chain.steps_done
+        if ahead > 0:
+            chain.run(ahead, w["dt"])
+        torch.cuda.synchronize()
+        chain.last_cycles.clear()
+        t2 = time.perf_counter()
+        chain.run(args.steps, w["dt"])
+        torch.cuda.synchronize()
+        dt_s = time.perf_counter() - t2
+        steady = {"first_step": chain.steps_done - args.steps + 1, "steps": args.steps,
+                  "ms_per_step": dt_s / args.steps * 1e3, "steps_per_s": args.steps / dt_s,
+                  "vcycles_per_step": float(np.mean(chain.last_cycles))}
+        print(f"[bench] pass 4 steady state: {steady}", file=sys.stderr, flush=True)
+
     acc = {}
     if not args.no_accuracy:
         acc = phi_accuracy(chain, grav)
         print(f"[bench] accuracy {acc}", file=sys.stderr, flush=True)
     stats = getattr(solver, "last_stats", None) or {}
+    series = None
+    if args.accuracy_steps > 0 and not args.no_accuracy:
+        del chain
+        torch.cuda.empty_cache()
+        series = accuracy_series(icr, w, dx, grav, make, args.accuracy_steps,
+                                 args.accuracy_cadence)
 
     ab, learned_vs_plain = None, None
     if not args.no_ab and w["kind"] == "two_galaxy":
@@ -379,6 +508,9 @@ def main() -> int:
             "parallelism": "single GPU",
             "particle_order": "sorted by initial cell" + (
                 f"; moved into cell order every {reorder} steps" if reorder else ""),
+            "disk_velocities": ("rotation curve measured on the grid + 10% dispersion"
+                                if equilibrated else "analytic enclosed-mass curve + 10% "
+                                "dispersion" if w["kind"] == "two_galaxy" else None),
         },
         "particle_steps_per_s": value * icr.n,
         "mean_vcycles_per_step": float(np.mean(cycles)) if cycles else None,
@@ -387,8 +519,16 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": dict(sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"])[:16]),
+        "mg_finest_roofline": mg_fractions or None,
+        "per_stage_ms": per_stage,
+        "per_stage_protocol": "unfused stages, each bracketed by device synchronizations "
+                              "(reference bench.py:66-72, 176-186); serializes the device",
+        "per_stage_reference_ms_rtx3070": REFERENCE_STAGE_MS,
+        "steady_state": steady,
         "mem_estimate_gb": estimate_memory(icr.n, w["grid"]).total_gb,
         **acc,
+        "accuracy_series": ({k: v for k, v in series.items() if k != "points"}
+                            if series else None),
         "learned_vs_plain_mg": learned_vs_plain,
         "warm_start_ab": ab,
     }

@@ -1,11 +1,20 @@
-"""Potential accuracy measurements (host float64; a diagnostic, not a solver path).
+"""Potential and force accuracy measurements (float64; a diagnostic, not a solver path).
 
-``phi_accuracy`` compares a solved phi with
+A solved phi is compared with
 (a) the exact float64 solution of the same discrete problem (7-point Laplacian, the solve's
     own Dirichlet faces, the same float32 right-hand side 4*pi*G*rho), diagonalised by the
-    type-I discrete sine transform (scipy.fft.dstn): the algebraic error of the solver;
+    type-I discrete sine transform: the algebraic error of the solver. ``dst_exact`` runs
+    scipy on the host; ``dst_exact_device`` runs the same transform on the device (torch
+    FFTs of the odd extension, float64), which makes a 256^3 check take milliseconds;
 (b) the zero-padded continuum-kernel FFT solve (float64, on the device), the reference's
     validation solver (solver/fft_oracle.py:54-109): algebraic plus discretization error.
+
+Forces are compared where the particles feel them: g = -grad(phi) of each potential with the
+chain's own gradient and CIC gather kernels, ||a - a_ref|| / ||a_ref|| over the particles
+and, against the continuum FFT, also the median per-particle relative difference (the L2
+ratio is set by the particles within a cell or two of a point mass, where the 7-point
+operator and the 1/r kernel differ most).
+``Checkpoint`` bundles both measurements for runs that check at a cadence.
 """
 
 from __future__ import annotations
@@ -35,30 +44,134 @@ def dst_exact(f: np.ndarray, boundary: np.ndarray, dx: float) -> np.ndarray:
     return out
 
 
+def _dst1(x: torch.Tensor, dim: int) -> torch.Tensor:
+    """Unnormalised type-I DST along ``dim`` (y_k = 2 sum_j x_j sin(pi (j+1)(k+1)/(m+1)))
+    as minus the imaginary part of the FFT of the odd extension [0, x, 0, -reverse(x)]."""
+    m = x.shape[dim]
+    zshape = list(x.shape)
+    zshape[dim] = 1
+    z = torch.zeros(zshape, dtype=x.dtype, device=x.device)
+    ext = torch.cat([z, x, z, -torch.flip(x, (dim,))], dim=dim)
+    return -torch.fft.rfft(ext, dim=dim).narrow(dim, 1, m).imag
+
+
+def dst_exact_device(f: torch.Tensor, boundary: torch.Tensor, dx: float) -> torch.Tensor:
+    """``dst_exact`` on the device: float64 (n, n, n) result for float32/64 inputs."""
+    n = f.shape[0]
+    inner = (slice(1, -1),) * 3
+    b = f[inner].double() * (float(dx) * float(dx))
+    bd = boundary.double().clone()
+    bd[inner] = 0.0
+    b -= (bd[2:, 1:-1, 1:-1] + bd[:-2, 1:-1, 1:-1] + bd[1:-1, 2:, 1:-1] + bd[1:-1, :-2, 1:-1]
+          + bd[1:-1, 1:-1, 2:] + bd[1:-1, 1:-1, :-2])
+    del bd
+    m = n - 2
+    k = torch.arange(1, m + 1, dtype=torch.float64, device=f.device)
+    lam1 = 2.0 * (torch.cos(math.pi * k / (m + 1)) - 1.0)
+    for d in range(3):
+        b = _dst1(b, d)
+    b /= lam1[:, None, None] + lam1[None, :, None] + lam1[None, None, :]
+    for d in range(3):
+        b = _dst1(b, d)
+    b /= float(2 * (m + 1)) ** 3  # the inverse of the unnormalised DST-I, per axis
+    out = boundary.double().clone()
+    out[inner] = b
+    return out
+
+
 def phi_rel_error_discrete(rho: np.ndarray, phi: np.ndarray, dx: float, grav: float) -> float:
-    """||phi - exact|| / ||exact|| for the discrete problem phi was solved for."""
+    """||phi - exact|| / ||exact|| for the discrete problem phi was solved for (host)."""
     f = (np.float32(4.0 * math.pi * grav) * rho).astype(np.float32)
     exact = dst_exact(f, phi.astype(np.float64), dx)
     return float(np.linalg.norm(phi.astype(np.float64) - exact) / np.linalg.norm(exact))
 
 
+def exact_discrete_device(rho: torch.Tensor, phi: torch.Tensor, dx: float,
+                          grav: float) -> torch.Tensor:
+    """Exact float64 solution of the discrete problem phi was solved for (on the device)."""
+    f = torch.tensor(4.0 * math.pi * grav, dtype=torch.float32, device=rho.device) * rho
+    return dst_exact_device(f, phi, dx)
+
+
+def _rel(a: torch.Tensor, ref: torch.Tensor) -> float:
+    return float(torch.linalg.vector_norm(a.double() - ref) / torch.linalg.vector_norm(ref))
+
+
+def _median_rel(a: torch.Tensor, ref: torch.Tensor) -> float:
+    """Median over particles of |a_i - ref_i| / |ref_i|: unlike the L2 ratio it is not set
+    by the few particles next to a point mass, where the two kernels differ most."""
+    num = torch.linalg.vector_norm(a.double() - ref, dim=1)
+    den = torch.linalg.vector_norm(ref, dim=1)
+    return float(torch.median(num / torch.clamp(den, min=1e-300)))
+
+
+def phi_rel_error_discrete_device(rho, phi, dx: float, grav: float) -> float:
+    """``phi_rel_error_discrete`` for device fields/tensors, computed on the device."""
+    from galaxy_collision_amd.data import as_tensor
+
+    rho_t, phi_t = as_tensor(rho), as_tensor(phi)
+    return _rel(phi_t, exact_discrete_device(rho_t, phi_t, dx, grav))
+
+
+def particle_forces(parts, phi: torch.Tensor, dx: float) -> torch.Tensor:
+    """(N, 3) float32 accelerations of the particles in the field -grad(phi), computed with
+    the chain's gradient and CIC gather kernels (phi rounded to float32)."""
+    from galaxy_collision_amd.deposit import gather_acceleration, potential_to_accel
+
+    p32 = phi.float().contiguous()
+    g = [torch.empty_like(p32) for _ in range(3)]
+    potential_to_accel(p32, *g, dx)
+    acc = [torch.empty(parts.n, device=p32.device) for _ in range(3)]
+    gather_acceleration(parts, *g, *acc, dx)
+    return torch.stack(acc, 1)
+
+
+class Checkpoint:
+    """phi and force accuracy of a chain's current state against the exact discrete solution
+    and the zero-padded float64 continuum FFT (the FFT's Green's function is built once)."""
+
+    def __init__(self, grid_size: int, dx: float, grav: float):
+        from galaxy_collision_amd.solver.fft import FFTPoissonSolver
+
+        self.n, self.dx, self.grav = int(grid_size), float(dx), float(grav)
+        self.fft = FFTPoissonSolver(self.n, dx=self.dx, grav_constant=self.grav)
+
+    def measure(self, parts, rho, phi) -> dict:
+        from galaxy_collision_amd.data import as_tensor
+
+        rho_t, phi_t = as_tensor(rho), as_tensor(phi)
+        exact = exact_discrete_device(rho_t, phi_t, self.dx, self.grav)
+        fft_phi = torch.empty(phi_t.shape, dtype=torch.float64, device=phi_t.device)
+        self.fft.solve(rho_t, fft_phi)
+        a = particle_forces(parts, phi_t, self.dx)
+        a_exact = particle_forces(parts, exact, self.dx)
+        a_fft = particle_forces(parts, fft_phi, self.dx)
+        out = {
+            "phi_rel_l2_vs_exact_discrete": _rel(phi_t, exact),
+            "phi_rel_l2_vs_fft_continuum": _rel(phi_t, fft_phi),
+            "force_rel_l2_vs_exact_discrete": _rel(a, a_exact.double()),
+            "force_rel_l2_vs_fft_continuum": _rel(a, a_fft.double()),
+            "force_median_rel_vs_fft_continuum": _median_rel(a, a_fft.double()),
+        }
+        del exact, fft_phi, a, a_exact, a_fft
+        return out
+
+
 def phi_accuracy(rho_field, phi_field, n: int, dx: float, grav: float) -> dict:
-    """Both errors of the solved phi (fields: DeviceField or device tensors)."""
+    """Both phi errors of the solved phi (fields: DeviceField or device tensors)."""
     from galaxy_collision_amd.data import as_tensor
     from galaxy_collision_amd.solver.fft import FFTPoissonSolver
 
     rho_t, phi_t = as_tensor(rho_field), as_tensor(phi_field)
-    rho = rho_t.cpu().numpy()
-    phi = phi_t.cpu().numpy().astype(np.float64)
-    alg = phi_rel_error_discrete(rho, phi, dx, grav)
+    alg = _rel(phi_t, exact_discrete_device(rho_t, phi_t, dx, grav))
     fft = FFTPoissonSolver(n, dx=dx, grav_constant=grav)
-    out = torch.empty_like(phi_t)
+    out = torch.empty(phi_t.shape, dtype=torch.float64, device=phi_t.device)
     fft.solve(rho_t, out)
-    ref = out.cpu().numpy().astype(np.float64)
+    tot = _rel(phi_t, out)
     del fft, out
     torch.cuda.empty_cache()
-    tot = float(np.linalg.norm(phi - ref) / np.linalg.norm(ref))
     return {"phi_rel_l2_vs_exact_discrete": alg, "phi_rel_l2_vs_fft_continuum": tot}
 
 
-__all__ = ["dst_exact", "phi_rel_error_discrete", "phi_accuracy"]
+__all__ = ["dst_exact", "dst_exact_device", "exact_discrete_device", "phi_rel_error_discrete",
+           "phi_rel_error_discrete_device", "particle_forces", "Checkpoint", "phi_accuracy"]

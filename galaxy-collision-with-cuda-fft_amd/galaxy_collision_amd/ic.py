@@ -312,6 +312,121 @@ def two_galaxy_ic_tensors(n_per_galaxy, *, box_centre, separation: float = 20.0,
     return ic
 
 
+# ---------------------------------------------------------------------------------------
+# disks launched on the rotation curve the grid produces (reference ic.py:408-525)
+# ---------------------------------------------------------------------------------------
+
+def equilibrate_disk_velocities(icr, *, dx: float, solver, rho, phi, ax, ay, az,
+                                n_radii: int = 64, n_azimuth: int = 64, parts=None,
+                                dispersion: float = 0.0, seed: int = 0) -> dict:
+    """Give every disk particle the circular speed of the force the PM chain will apply.
+
+    For each galaxy its own particles alone are deposited on the production grid, solved with
+    the production ``solver`` (cold) and differentiated; rings of probes in the disk plane
+    through the galaxy's black hole sample the azimuthally averaged inward radial
+    acceleration <-a_R>(R), so v_c(R) = sqrt(R <-a_R>) is the circular speed the grid
+    supports (the clamp to >= 0 is applied to the mean, not per sample). Each disk particle
+    gets v = R*omega(R) along n x R_hat plus the galaxy's bulk velocity, omega = v_c/R
+    interpolated so that particles inside the first ring rotate as a solid body; with
+    ``dispersion`` > 0 a Gaussian of that fraction of v_c is added per component (the
+    benchmark recipe's 10%). Bulge and black-hole velocities are untouched. The disk plane
+    is the one normal to the disk's angular momentum about the black hole (the reference's
+    disks lie in the xy plane; the benchmark's are inclined).
+
+    ``icr``: ICResult (numpy) or TensorIC (torch) with ``component``; its velocities are
+    replaced; with ``parts`` (a ParticleState holding the same particles, in load order or
+    reordered with ``pid``) the device velocities are written too. ``rho``..``az`` are
+    scratch grid fields (the production ones work: a run re-primes them). Returns
+    {gid: (r_table, vc_table)}.
+    """
+    import torch
+
+    from galaxy_collision_amd.data import ParticleState, as_tensor
+    from galaxy_collision_amd.deposit import (deposit_density, gather_acceleration,
+                                              potential_to_accel)
+
+    if getattr(icr, "component", None) is None:
+        raise ValueError("equilibrate_disk_velocities needs the component tags of a "
+                         "two-galaxy IC")
+    dev = as_tensor(rho).device
+    as_t = lambda a: (a if isinstance(a, torch.Tensor) else torch.from_numpy(np.asarray(a)))  # noqa: E731
+    pos = as_t(icr.pos).to(dev, torch.float64)
+    vel = as_t(icr.vel).to(dev, torch.float64)
+    mass = as_t(icr.mass).to(dev, torch.float32)
+    gid = as_t(icr.gid).to(dev)
+    comp = as_t(icr.component).to(dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    curves: dict = {}
+    for g in torch.unique(gid).tolist():
+        sel = torch.nonzero(gid == g).squeeze(1)
+        bh = sel[comp[sel] == COMPONENT_BH]
+        if bh.numel():
+            centre, bulk = pos[bh[0]].clone(), vel[bh[0]].clone()
+        else:  # no black hole: the stellar centroid, at rest
+            centre, bulk = pos[sel].mean(0), torch.zeros(3, dtype=torch.float64, device=dev)
+        disk = sel[comp[sel] == COMPONENT_DISK]
+        if disk.numel() == 0:
+            continue
+        rel = pos[disk] - centre
+        L = (mass[disk].double()[:, None] * torch.linalg.cross(rel, vel[disk] - bulk)).sum(0)
+        nrm = L / torch.linalg.vector_norm(L)
+        rvec = rel - (rel @ nrm)[:, None] * nrm
+        R = torch.linalg.vector_norm(rvec, dim=1)
+        del rel
+        # this galaxy's self-gravity on the production grid
+        sub = ParticleState(int(sel.numel()), device=dev)
+        sub.load_tensors(pos[sel].float(), None, mass[sel])
+        deposit_density(sub, rho, dx)
+        solver.solve(rho, phi, warm_start=False)
+        potential_to_accel(phi, ax, ay, az, dx)
+        del sub
+        # probe rings in the disk plane
+        helper = torch.tensor([1.0, 0.0, 0.0] if abs(float(nrm[0])) < 0.9 else [0.0, 1.0, 0.0],
+                              dtype=torch.float64, device=dev)
+        e1 = torch.linalg.cross(nrm, helper)
+        e1 /= torch.linalg.vector_norm(e1)
+        e2 = torch.linalg.cross(nrm, e1)
+        r_tab = torch.linspace(float(dx), float(R.max()), n_radii, dtype=torch.float64,
+                               device=dev)
+        theta = torch.arange(n_azimuth, dtype=torch.float64, device=dev) * (2 * np.pi / n_azimuth)
+        dirs = torch.cos(theta)[:, None] * e1 + torch.sin(theta)[:, None] * e2  # (az, 3)
+        ppos = centre + r_tab[:, None, None] * dirs[None]                        # (r, az, 3)
+        probes = ParticleState(n_radii * n_azimuth, device=dev)
+        probes.load_tensors(ppos.reshape(-1, 3).float())
+        acc = [torch.empty(probes.n, device=dev) for _ in range(3)]
+        gather_acceleration(probes, ax, ay, az, *acc, dx)
+        a = torch.stack(acc, 1).double().reshape(n_radii, n_azimuth, 3)
+        a_rad = (a * dirs[None]).sum(-1)
+        inward = torch.clamp(-a_rad.mean(1), min=0.0)
+        vc_tab = torch.sqrt(r_tab * inward)
+        curves[int(g)] = (r_tab.cpu().numpy(), vc_tab.cpu().numpy())
+        # omega = v_c/R interpolated linearly in R, held at the end values outside the table
+        omega_tab = vc_tab / r_tab
+        k = torch.clamp(torch.searchsorted(r_tab, R), 1, n_radii - 1)
+        r0, r1 = r_tab[k - 1], r_tab[k]
+        t = torch.clamp((R - r0) / (r1 - r0), 0.0, 1.0)
+        omega = omega_tab[k - 1] + t * (omega_tab[k] - omega_tab[k - 1])
+        vc = R * omega
+        that = torch.linalg.cross(nrm.expand_as(rvec), rvec / torch.clamp(R, min=1e-6)[:, None])
+        v_new = vc[:, None] * that
+        if dispersion > 0.0:
+            v_new += dispersion * vc[:, None] * torch.randn(v_new.shape, generator=gen,
+                                                            dtype=torch.float64, device=dev)
+        vel[disk] = v_new + bulk
+        del rvec, R, vc, that, v_new
+    vel32 = vel.float()
+    if isinstance(icr.vel, torch.Tensor):
+        icr.vel = vel32.to(icr.vel.device)
+    else:
+        icr.vel = vel32.cpu().numpy()
+    if parts is not None:
+        pid = parts.pid.tensor.long()
+        for a_, f in enumerate((parts.vel_x, parts.vel_y, parts.vel_z)):
+            f.tensor.copy_(vel32[pid, a_])
+    return curves
+
+
 def sort_by_cell(ic: ICResult, grid_size: int, dx: float) -> ICResult:
     """Reorders particles by their CIC cell (C order): the memory layout the fused step
     prefers (neighbouring threads touch neighbouring nodes). Particle identity is kept in
@@ -333,4 +448,4 @@ def box_fit_fraction(ic: ICResult, grid_size: int, dx: float) -> float:
 
 
 __all__ = ["ICResult", "TensorIC", "plummer_ic", "two_galaxy_ic", "two_galaxy_ic_tensors", "galaxy",
-           "galaxy_tensors", "sort_by_cell", "box_fit_fraction"]
+           "galaxy_tensors", "equilibrate_disk_velocities", "sort_by_cell", "box_fit_fraction"]

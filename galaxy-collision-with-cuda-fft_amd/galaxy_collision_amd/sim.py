@@ -107,14 +107,22 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
                    write_snapshots: bool = True, solver_kwargs: dict | None = None,
                    direct_pe_softening: float | None = None, icr=None, tracer_indices=None,
                    frame_cadence: int | None = None, frame_bins: int = 512,
-                   frame_axes: tuple[int, int] = (0, 1)) -> dict[str, Any]:
+                   frame_axes: tuple[int, int] = (0, 1), accuracy_cadence: int | None = None,
+                   equilibrate: bool | None = None,
+                   equilibrate_kwargs: dict | None = None) -> dict[str, Any]:
     """Run the PM pipeline and return its summary and diagnostics history.
 
     history_cadence: steps between diagnostics samples (default ~20 over the run, plus the
     first and last); direct_pe_softening: also record the O(N^2) softened pair-sum energy
     (small N only); tracer_indices: particle indices whose positions are recorded at each
     sample; frame_cadence: capture a device-projected surface-density frame every that many
-    steps (plus first and last).
+    steps (plus first and last); accuracy_cadence: every that many steps (plus the first and
+    last) compare phi and the particle forces with the exact float64 solution of the same
+    discrete problem and with the zero-padded float64 continuum FFT (``accuracy.Checkpoint``;
+    results under "accuracy"); equilibrate: launch the disks of a two-galaxy IC on the
+    rotation curve the grid produces plus the recipe's 10% dispersion
+    (``ic.equilibrate_disk_velocities``, options in ``equilibrate_kwargs``; default: on for
+    two-galaxy presets as in reference sim.py:310-314).
     """
     from galaxy_collision_amd import _native as nat
     from galaxy_collision_amd.chain import PMChain
@@ -135,6 +143,25 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
     grid = GridState(gsize)
     solver = make_solver(config.solver, gsize, dx=dx, grav_constant=config.grav,
                          **(solver_kwargs or {}))
+    if equilibrate is None:
+        equilibrate = config.ic_preset == "two_galaxy"
+    curves = None
+    if equilibrate:
+        from galaxy_collision_amd.ic import equilibrate_disk_velocities
+
+        # the recipe's 10% dispersion is kept on top of the measured curve
+        ekw = {"dispersion": 0.1, "seed": config.seed, **(equilibrate_kwargs or {})}
+        curves = equilibrate_disk_velocities(icr, dx=dx, solver=solver, rho=grid.rho,
+                                             phi=grid.phi, ax=grid.ax, ay=grid.ay, az=grid.az,
+                                             parts=parts, **ekw)
+    if accuracy_cadence is not None and int(accuracy_cadence) <= 0:
+        raise ValueError(f"accuracy_cadence must be positive, got {accuracy_cadence}")
+    checker = None
+    accuracy: list[dict] = []
+    if accuracy_cadence:
+        from galaxy_collision_amd.accuracy import Checkpoint
+
+        checker = Checkpoint(gsize, dx, config.grav)
     chain = PMChain(parts, gsize, dx, solver, grid=grid, reorder_every=config.reorder_every)
     diag = DeviceDiagnostics(gsize, dx=dx)
     tracer_idx = None if tracer_indices is None else [int(i) for i in np.atleast_1d(tracer_indices)]
@@ -193,15 +220,28 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
     def frame_due(step: int) -> bool:
         return bool(frame_cadence) and (step % frame_cadence == 0 or step == config.steps)
 
+    def accuracy_due(step: int) -> bool:
+        return bool(accuracy_cadence) and (step % accuracy_cadence == 0 or step == config.steps)
+
+    def check_accuracy(step: int) -> None:
+        rec = {"step": step, "time": step * config.dt}
+        rec.update(checker.measure(parts, grid.rho, grid.phi))
+        accuracy.append(rec)
+
+    if checker is not None:
+        check_accuracy(0)
     step = 0
     while step < config.steps:
-        # run up to the next step that is sampled, written or captured, in one fused run
+        # run up to the next step that is sampled, written, captured or checked, in one
+        # fused run
         nxt = step + 1
         while not (nxt % cadence == 0 or nxt == config.steps or snapshot_due(nxt)
-                   or frame_due(nxt)):
+                   or frame_due(nxt) or accuracy_due(nxt)):
             nxt += 1
         chain.run(nxt - step, config.dt)
         step = nxt
+        if accuracy_due(step):
+            check_accuracy(step)
         want_hist = step % cadence == 0 or step == config.steps
         want_snap = snapshot_due(step)
         if want_hist or want_snap:
@@ -235,6 +275,10 @@ def run_simulation(config: SimConfig, *, history_cadence: int | None = None,
         "frame_times": np.array(frame_times) if frames else None,
         "frame_extent": extent if frame_cadence else None,
         "history": history, "snapshots": snapshots, "status": "ok",
+        "accuracy": accuracy if checker is not None else None,
+        "accuracy_max": ({k: max(a[k] for a in accuracy) for k in accuracy[0]
+                          if k not in ("step", "time")} if accuracy else None),
+        "rotation_curves": curves,
     }
 
 
@@ -254,12 +298,18 @@ def main(argv=None) -> int:
     ap.add_argument("--output-cadence", type=int, default=0)
     ap.add_argument("--output-dir", default="outputs")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--accuracy-cadence", type=int, default=None,
+                    help="check phi and forces against the exact and FFT solutions every K steps")
+    ap.add_argument("--no-equilibrate", action="store_true",
+                    help="keep the analytic disk velocities of a two-galaxy IC")
     a = ap.parse_args(argv)
     cfg = SimConfig(name=a.preset, n_particles=a.n, dt=a.dt, steps=a.steps, ic_preset=a.preset,
                     solver=a.solver, grid_size=a.grid, dx=a.dx, output_cadence=a.output_cadence,
                     output_dir=a.output_dir, seed=a.seed)
-    res = run_simulation(cfg)
-    keep = {k: v for k, v in res.items() if k not in ("history", "frames", "tracer_path")}
+    res = run_simulation(cfg, accuracy_cadence=a.accuracy_cadence,
+                         equilibrate=False if a.no_equilibrate else None)
+    keep = {k: v for k, v in res.items()
+            if k not in ("history", "frames", "tracer_path", "rotation_curves")}
     print(json.dumps(keep, default=float), file=sys.stdout)
     return 0
 

@@ -51,7 +51,7 @@ class FFTPoissonSolver(PoissonSolver):
         del padded
         spec *= self._greens_ft
         conv = torch.fft.irfftn(spec, s=(m, m, m))
-        p.copy_(conv[:n, :n, :n].to(torch.float32))
+        p.copy_(conv[:n, :n, :n])  # rounded once to phi's dtype (float32 fields; float64 kept)
 
 
 __all__ = ["FFTPoissonSolver"]

@@ -506,27 +506,62 @@ class CpuChain:
         self.warm = 0
         self.last_cycles = 0
 
-    def accel(self):
-        L = self.L
-        L.cpu_deposit_atomic(_fp(self.x), _fp(self.y), _fp(self.z), _fp(self.m), self.np, self.n,
-                             self.dx, _fp(self.rho))
-        self.last_cycles = L.cpu_mg_solve(self.mg, _fp(self.rho), _fp(self.phi), self.warm)
+    # the stages of the reference step (deposit.py:47-174, multigrid solve, integrator.py:27-60)
+    def deposit(self):
+        self.L.cpu_deposit_atomic(_fp(self.x), _fp(self.y), _fp(self.z), _fp(self.m), self.np,
+                                  self.n, self.dx, _fp(self.rho))
+
+    def solve(self):
+        self.last_cycles = self.L.cpu_mg_solve(self.mg, _fp(self.rho), _fp(self.phi), self.warm)
         self.warm = 1
-        L.cpu_grad(_fp(self.phi), _fp(self.gx), _fp(self.gy), _fp(self.gz), self.n, self.dx)
-        L.cpu_gather(_fp(self.x), _fp(self.y), _fp(self.z), _fp(self.gx), _fp(self.gy),
-                     _fp(self.gz), _fp(self.ax), _fp(self.ay), _fp(self.az), self.np, self.n,
-                     self.dx)
+
+    def grad(self):
+        self.L.cpu_grad(_fp(self.phi), _fp(self.gx), _fp(self.gy), _fp(self.gz), self.n, self.dx)
+
+    def gather(self):
+        self.L.cpu_gather(_fp(self.x), _fp(self.y), _fp(self.z), _fp(self.gx), _fp(self.gy),
+                          _fp(self.gz), _fp(self.ax), _fp(self.ay), _fp(self.az), self.np,
+                          self.n, self.dx)
+
+    def kick(self, h: float):
+        self.L.cpu_kick(_fp(self.vx), _fp(self.vy), _fp(self.vz), _fp(self.ax), _fp(self.ay),
+                        _fp(self.az), self.np, h)
+
+    def drift(self, dt: float):
+        self.L.cpu_drift(_fp(self.x), _fp(self.y), _fp(self.z), _fp(self.vx), _fp(self.vy),
+                         _fp(self.vz), self.np, dt)
+
+    def accel(self):
+        self.deposit()
+        self.solve()
+        self.grad()
+        self.gather()
 
     def step(self, dt: float):
-        L = self.L
-        h = 0.5 * dt
-        L.cpu_kick(_fp(self.vx), _fp(self.vy), _fp(self.vz), _fp(self.ax), _fp(self.ay),
-                   _fp(self.az), self.np, h)
-        L.cpu_drift(_fp(self.x), _fp(self.y), _fp(self.z), _fp(self.vx), _fp(self.vy),
-                    _fp(self.vz), self.np, dt)
+        self.kick(0.5 * dt)
+        self.drift(dt)
         self.accel()
-        L.cpu_kick(_fp(self.vx), _fp(self.vy), _fp(self.vz), _fp(self.ax), _fp(self.ay),
-                   _fp(self.az), self.np, h)
+        self.kick(0.5 * dt)
+
+    def stage_times(self, dt: float) -> dict:
+        """One step with each stage timed (seconds; the reference's per-stage protocol)."""
+        import time
+
+        out = {}
+
+        def timed(name, fn, *a):
+            t = time.perf_counter()
+            fn(*a)
+            out[name] = out.get(name, 0.0) + time.perf_counter() - t
+
+        timed("integrate", self.kick, 0.5 * dt)
+        timed("integrate", self.drift, dt)
+        timed("deposit", self.deposit)
+        timed("solve", self.solve)
+        timed("grad", self.grad)
+        timed("gather", self.gather)
+        timed("integrate", self.kick, 0.5 * dt)
+        return out
 
     def close(self):
         if self.mg:
