@@ -575,15 +575,20 @@ __global__ void __launch_bounds__(256) mid_cell_kernel(
     const int32_t* __restrict__ mid_pieces, int64_t chunks, const float* __restrict__ head,
     const float* __restrict__ tail, float* __restrict__ sums, int32_t* __restrict__ map) {
   const int lane = threadIdx.x & 63;
-  const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
-  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  // each wave scans 64 chunks at a time and takes the marked ones in turn
-  for (int64_t base = wave * 64; base < chunks; base += waves * 64) {
+  const int wid = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  // each workgroup scans windows of 64 chunks; its waves take the window's marked chunks in
+  // turn (a window in a dense region starts up to 7 such cells: shared by the waves they do
+  // not form one wave's serial chain)
+  const int64_t windows = (chunks + 63) / 64;
+  for (int64_t win = blockIdx.x; win < windows; win += gridDim.x) {
+    const int64_t base = win * 64;
     const int32_t mine = base + lane < chunks ? mid_pieces[base + lane] : 0;
     uint64_t todo = __ballot(mine > 0);
+    int rank = 0;
     while (todo) {
       const int l = __builtin_ctzll(todo);
       todo &= todo - 1;
+      if ((rank++ % nwaves) != wid) continue;
       const int64_t chunk = base + l;
       const int64_t m = __shfl(mine, l, 64);
       const int64_t slot = tail_slot[chunk];
@@ -1297,7 +1302,9 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
   }
   {
     PM_TIMED("deposit.mid_cell", 0.0, stream);
-    mid_cell_kernel<<<1024, 256, 0, stream>>>(skey, plan->tail_slot, plan->mid_pieces, chunks,
+    const unsigned mid_blocks =
+        static_cast<unsigned>(std::min<int64_t>((chunks + 63) / 64, int64_t(4096)));
+    mid_cell_kernel<<<mid_blocks, 256, 0, stream>>>(skey, plan->tail_slot, plan->mid_pieces, chunks,
                                               plan->head, plan->tail, plan->sums, plan->map);
     PM_CHECK_LAUNCH("mid_cell_kernel");
   }

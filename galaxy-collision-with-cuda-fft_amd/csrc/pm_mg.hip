@@ -283,16 +283,78 @@ __device__ __forceinline__ float prolong_value(const float* __restrict__ phi_c, 
   return val;
 }
 
+// Staged coarse values of a prolong_add workgroup: rows (ci0 + a, cj0 + b), a < 2,
+// b < ncj, each nc floats; row indices past the coarse grid are clamped to its last row as
+// prolong_value clamps them.
+constexpr int kProlongAddMaxLds = 2 * 4 * 513;
+
 __global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restrict__ phi_c,
                                                           float* __restrict__ phi_f, int nf,
                                                           int nc) {
+  __shared__ float s_c[kProlongAddMaxLds];
   const int64_t nf2 = static_cast<int64_t>(nf) * nf;
-  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= nf2 * nf) return;
+  const int64_t total = nf2 * nf;
+  const int64_t c0 = pm::xcd_block() * blockDim.x;
+  const int64_t c = c0 + threadIdx.x;
+  // When the workgroup's nodes share one i, every coarse value they read lies in coarse
+  // rows ci0..ci0+1 x cj0..cj0+ncj-1 (clamped): staged with coalesced loads instead of 8
+  // scalar loads per node; the values and the arithmetic are prolong_value's.
+  const int64_t last = c0 + blockDim.x - 1 < total ? c0 + blockDim.x - 1 : total - 1;
+  bool staged = false;
+  int ci0 = 0, cj0 = 0, ncj = 0;
+  if (c0 < total) {
+    const pm::Idx3 f0 = pm::idx3(c0, nf), f1 = pm::idx3(last, nf);
+    ci0 = f0.i >> 1;
+    cj0 = f0.j >> 1;
+    ncj = (f1.j >> 1) + 2 - cj0;
+    staged = f0.i == f1.i && ncj >= 1 && 2 * ncj * nc <= kProlongAddMaxLds;
+  }
+  if (staged) {
+    const int per = ncj * nc;
+    const int64_t nc2 = static_cast<int64_t>(nc) * nc;
+    for (int t = threadIdx.x; t < 2 * per; t += blockDim.x) {
+      const int a = t / per, rem = t - a * per;
+      const int b = rem / nc, k = rem - b * nc;
+      const int ca = ci0 + a < nc ? ci0 + a : nc - 1;
+      const int cb = cj0 + b < nc ? cj0 + b : nc - 1;
+      s_c[t] = phi_c[ca * nc2 + static_cast<int64_t>(cb) * nc + k];
+    }
+  }
+  __syncthreads();
+  if (c >= total) return;
   const pm::Idx3 q3 = pm::idx3(c, nf);
   const int i = q3.i, j = q3.j, k = q3.k;
   if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) return;
-  phi_f[c] = phi_f[c] + prolong_value(phi_c, nc, i, j, k);
+  if (!staged) {
+    phi_f[c] = phi_f[c] + prolong_value(phi_c, nc, i, j, k);
+    return;
+  }
+  const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
+  const int oi = i & 1, oj = j & 1, ok = k & 1;
+  float cv[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    // the same clamped coarse node as prolong_value: rows via the staged clamp, k here
+    const int a = ci + (t >> 2) - ci0, b = cj + ((t >> 1) & 1) - cj0, d = ck + (t & 1);
+    cv[t] = s_c[(a * ncj + b) * nc + (d < nc ? d : nc - 1)];
+  }
+  float val = 0.0f;
+#pragma unroll
+  for (int di = 0; di < 2; ++di) {
+    const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const float wj = oj ? 0.5f : (dj == 0 ? 1.0f : 0.0f);
+#pragma unroll
+      for (int dk = 0; dk < 2; ++dk) {
+        const float wk = ok ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
+        const float w = wi * wj * wk;
+        const float tt = val + w * cv[di * 4 + dj * 2 + dk];
+        val = w > 0.0f ? tt : val;
+      }
+    }
+  }
+  phi_f[c] = phi_f[c] + val;
 }
 
 __global__ void __launch_bounds__(256) fill_f32_kernel(float* __restrict__ p, int64_t count,
@@ -1107,29 +1169,68 @@ __global__ void __launch_bounds__(256) residual_s4_kernel(const float* __restric
 // prolong_s_kernel, 4 colour-1 positions per thread. The 4 nodes share i, j and the parity
 // of k, so their coarse terms come from the same (di, dj) rows at 4 or 5 consecutive coarse
 // k, loaded once; each node's sum keeps prolong_value's order.
+// Coarse rows (ci0 + a, cj0 + b), a < 2, b < kProlongMaxCj, of a prolongation workgroup,
+// staged whole (nc floats each) in LDS.
+constexpr int kProlongMaxLds = 2100;  // 2*(1024/n + 1)*(n/2 + 1) <= 2052 floats for n <= 1024
+
 __global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict__ phi_c,
                                                          float* __restrict__ e, int n, int hn,
                                                          int nc) {
+  __shared__ float s_c[kProlongMaxLds];
   const int hq = hn >> 2;
   const uint32_t quads = static_cast<uint32_t>(n) * n * hq;
-  const int64_t u = pm::xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t u0 = pm::xcd_block() * blockDim.x;
+  const int64_t u = u0 + threadIdx.x;
+  const uint32_t nc2 = static_cast<uint32_t>(nc) * nc;
+  // The workgroup's quads cover fine rows r0..r1 (row = i*n + j). When they share one i,
+  // the coarse values they read lie in coarse rows ci0..ci0+1 x cj0..cj1, staged in LDS
+  // with coalesced loads (each thread would otherwise issue 20 strided scalar loads); the
+  // values and the arithmetic are those of the direct loads.
+  const uint32_t last = static_cast<uint32_t>(u0) + blockDim.x - 1 < quads
+                            ? static_cast<uint32_t>(u0) + blockDim.x - 1
+                            : quads - 1;
+  const uint32_t r0 = static_cast<uint32_t>(u0) / static_cast<uint32_t>(hq);
+  const uint32_t r1 = last / static_cast<uint32_t>(hq);
+  const int i0 = static_cast<int>(r0 / static_cast<uint32_t>(n));
+  const int ci0 = i0 >> 1;
+  const int cj0 = static_cast<int>(r0 % static_cast<uint32_t>(n)) >> 1;
+  const int ncj = (static_cast<int>(r1 % static_cast<uint32_t>(n)) >> 1) + 2 - cj0;
+  const bool staged = static_cast<uint32_t>(u0) < quads &&
+                      static_cast<int>(r1 / static_cast<uint32_t>(n)) == i0 &&
+                      ci0 + 1 < nc && cj0 + ncj <= nc && 2 * ncj * nc <= kProlongMaxLds;
+  if (staged) {
+    const int per = ncj * nc;
+    for (int t = threadIdx.x; t < 2 * per; t += blockDim.x) {
+      const int a = t / per, rem = t - a * per;
+      const int b = rem / nc, k = rem - b * nc;
+      s_c[t] = phi_c[static_cast<uint32_t>(ci0 + a) * nc2 +
+                     static_cast<uint32_t>(cj0 + b) * nc + k];
+    }
+  }
+  __syncthreads();
   if (u >= quads) return;
   const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
   if (q.i < 1 || q.i > n - 2 || q.j < 1 || q.j > n - 2) return;
   const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
   const int p = (q.i + q.j + 1) & 1;  // parity of k; k>>1 = kk0 + e
   const int ci = q.i >> 1, cj = q.j >> 1, oi = q.i & 1, oj = q.j & 1;
-  const uint32_t nc2 = static_cast<uint32_t>(nc) * nc;
   float c[2][2][5];
 #pragma unroll
   for (int di = 0; di < 2; ++di) {
 #pragma unroll
     for (int dj = 0; dj < 2; ++dj) {
       const bool need = (di == 0 || oi) && (dj == 0 || oj);
-      const uint32_t row = (ci + di) * nc2 + static_cast<uint32_t>(cj + dj) * nc + q.kk0;
+      if (staged) {
+        const int row = ((ci + di - ci0) * ncj + (cj + dj - cj0)) * nc + q.kk0;
 #pragma unroll
-      for (int x = 0; x < 5; ++x)
-        c[di][dj][x] = (need && (x < 4 || p)) ? phi_c[row + x] : 0.0f;
+        for (int x = 0; x < 5; ++x)
+          c[di][dj][x] = (need && (x < 4 || p)) ? s_c[row + x] : 0.0f;
+      } else {
+        const uint32_t row = (ci + di) * nc2 + static_cast<uint32_t>(cj + dj) * nc + q.kk0;
+#pragma unroll
+        for (int x = 0; x < 5; ++x)
+          c[di][dj][x] = (need && (x < 4 || p)) ? phi_c[row + x] : 0.0f;
+      }
     }
   }
   const float4 v = ld4(e, plane + q.t0);

@@ -83,17 +83,20 @@ def test_fused_sweep_equals_two_passes(device, n):
         assert_bitwise(_np(a), phi, "sweep vs oracle")
 
 
-@pytest.mark.parametrize("nf", [6, 16, 64])
-def test_restrict_prolong_bitwise(device, nf):
-    announce(f"restrict/prolong nf={nf}")
+@pytest.mark.parametrize("nf,nc", [(6, 3), (16, 8), (64, 32), (34, 18), (33, 17), (65, 33),
+                                   (129, 65)])
+def test_restrict_prolong_bitwise(device, nf, nc):
+    """Halving (PLAIN) and aligned (MIXED, nf = 2nc - 1) hierarchies; workgroups inside one
+    i-plane (staged coarse rows) and across planes."""
+    announce(f"restrict/prolong nf={nf} nc={nc}")
     from galaxy_collision_amd import _native as nat
 
-    nc = nf // 2
     rng = np.random.default_rng(nf)
-    res = rng.normal(size=(nf, nf, nf)).astype(F32)
-    out = torch.full((nc, nc, nc), 3.0, device=device)
-    nat.call("pm_restrict", nat.ptr(_t(res, device)), nat.ptr(out), nf, nc, nat.stream())
-    assert_bitwise(_np(out), O.restrict(res))
+    if nf == 2 * nc:
+        res = rng.normal(size=(nf, nf, nf)).astype(F32)
+        out = torch.full((nc, nc, nc), 3.0, device=device)
+        nat.call("pm_restrict", nat.ptr(_t(res, device)), nat.ptr(out), nf, nc, nat.stream())
+        assert_bitwise(_np(out), O.restrict(res))
     coarse = rng.normal(size=(nc, nc, nc)).astype(F32)
     fine = rng.normal(size=(nf, nf, nf)).astype(F32)
     tf = _t(fine, device)
