@@ -121,29 +121,44 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvFwdArgs a) {
     wrow[nb] = a.w + static_cast<int64_t>(wvalid[nb] ? co0 + 16 * nb + row : 0) * a.kpad + 8 * grp;
   }
 
-  for (int kc = kbeg; kc < kend; kc += 32) {
-    u32x4_t av[4];
-    const bool kin = kc + 8 * grp < a.kreal;
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  // G consecutive 32-wide chunks per round: every operand load of the round is issued
+  // before its first MFMA (a chunk's few MFMAs cannot hide a load's latency on their own),
+  // and the chunks are accumulated in order, as one chunk per round would
+  constexpr int G = NB <= 2 ? 2 : 1;
+  for (int kc = kbeg; kc < kend; kc += 32 * G) {
+    u32x4_t av[G][4];
+    u32x4_t bv[G][NB];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-      const uint16_t* p = kin ? conv_src(a.xa, a.xb, a.ca, a.cb, r, a.ra, a.up, vz[mb] + dz,
-                                         vy[mb] + dy, vx[mb] + dx, ci)
-                              : nullptr;
-      av[mb] = p ? load16(p) : u32x4_t{0u, 0u, 0u, 0u};
+    for (int g = 0; g < G; ++g) {
+      const int kcg = kc + 32 * g;
+      const bool live = kcg < kend;
+      const bool kin = live && kcg + 8 * grp < a.kreal;
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const uint16_t* p = kin ? conv_src(a.xa, a.xb, a.ca, a.cb, r, a.ra, a.up, vz[mb] + dz,
+                                           vy[mb] + dy, vx[mb] + dx, ci)
+                                : nullptr;
+        av[g][mb] = p ? load16(p) : u32x4_t{0u, 0u, 0u, 0u};
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        bv[g][nb] = (live && wvalid[nb]) ? load16(wrow[nb] + kcg) : u32x4_t{0u, 0u, 0u, 0u};
+      ci += 32;
+      while (ci >= a.cin) {
+        ci -= a.cin;
+        ++tap;
+      }
     }
-    u32x4_t bv[NB];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-      bv[nb] = wvalid[nb] ? load16(wrow[nb] + kc) : u32x4_t{0u, 0u, 0u, 0u};
+    for (int g = 0; g < G; ++g) {
+      if (kc + 32 * g >= kend) break;  // uniform
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+      for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma_bf16(av[mb], bv[nb], acc[mb][nb]);
-    ci += 32;
-    while (ci >= a.cin) {
-      ci -= a.cin;
-      ++tap;
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mb][nb] = mfma_bf16(av[g][mb], bv[g][nb], acc[mb][nb]);
     }
   }
 

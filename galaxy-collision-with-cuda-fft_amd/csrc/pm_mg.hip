@@ -1171,7 +1171,10 @@ __global__ void __launch_bounds__(256) residual_s4_kernel(const float* __restric
 // k, loaded once; each node's sum keeps prolong_value's order.
 // Coarse rows (ci0 + a, cj0 + b), a < 2, b < kProlongMaxCj, of a prolongation workgroup,
 // staged whole (nc floats each) in LDS.
-constexpr int kProlongMaxLds = 2100;  // 2*(1024/n + 1)*(n/2 + 1) <= 2052 floats for n <= 1024
+constexpr int kProlongMaxLds = 4200;  // 2*(2048/n + 1)*(n/2 + 1) <= 4100 floats for n <= 1024
+
+// Quads per thread of prolong_s4_kernel: quads u0 + tid + 256*m, m < kProlongQ.
+constexpr int kProlongQ = 2;
 
 __global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict__ phi_c,
                                                          float* __restrict__ e, int n, int hn,
@@ -1179,15 +1182,28 @@ __global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict
   __shared__ float s_c[kProlongMaxLds];
   const int hq = hn >> 2;
   const uint32_t quads = static_cast<uint32_t>(n) * n * hq;
-  const int64_t u0 = pm::xcd_block() * blockDim.x;
-  const int64_t u = u0 + threadIdx.x;
+  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
+  const int64_t u0 = pm::xcd_block() * (blockDim.x * kProlongQ);
   const uint32_t nc2 = static_cast<uint32_t>(nc) * nc;
+  // this thread's quads and their colour-1 values, loaded before the staging below
+  Quad q[kProlongQ];
+  bool live[kProlongQ];
+  float4 v[kProlongQ];
+#pragma unroll
+  for (int m = 0; m < kProlongQ; ++m) {
+    const int64_t u = u0 + threadIdx.x + static_cast<int64_t>(m) * blockDim.x;
+    live[m] = u < quads;
+    q[m] = quad_idx(live[m] ? static_cast<uint32_t>(u) : 0u, n, hq);
+    live[m] = live[m] && q[m].i >= 1 && q[m].i <= n - 2 && q[m].j >= 1 && q[m].j <= n - 2;
+    v[m] = live[m] ? ld4(e, plane + q[m].t0) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   // The workgroup's quads cover fine rows r0..r1 (row = i*n + j). When they share one i,
   // the coarse values they read lie in coarse rows ci0..ci0+1 x cj0..cj1, staged in LDS
-  // with coalesced loads (each thread would otherwise issue 20 strided scalar loads); the
+  // with coalesced loads (each quad would otherwise issue 20 strided scalar loads); the
   // values and the arithmetic are those of the direct loads.
-  const uint32_t last = static_cast<uint32_t>(u0) + blockDim.x - 1 < quads
-                            ? static_cast<uint32_t>(u0) + blockDim.x - 1
+  const uint32_t span = blockDim.x * kProlongQ;
+  const uint32_t last = static_cast<uint32_t>(u0) + span - 1 < quads
+                            ? static_cast<uint32_t>(u0) + span - 1
                             : quads - 1;
   const uint32_t r0 = static_cast<uint32_t>(u0) / static_cast<uint32_t>(hq);
   const uint32_t r1 = last / static_cast<uint32_t>(hq);
@@ -1208,55 +1224,55 @@ __global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict
     }
   }
   __syncthreads();
-  if (u >= quads) return;
-  const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
-  if (q.i < 1 || q.i > n - 2 || q.j < 1 || q.j > n - 2) return;
-  const uint32_t plane = static_cast<uint32_t>(n) * n * hn;
-  const int p = (q.i + q.j + 1) & 1;  // parity of k; k>>1 = kk0 + e
-  const int ci = q.i >> 1, cj = q.j >> 1, oi = q.i & 1, oj = q.j & 1;
-  float c[2][2][5];
 #pragma unroll
-  for (int di = 0; di < 2; ++di) {
-#pragma unroll
-    for (int dj = 0; dj < 2; ++dj) {
-      const bool need = (di == 0 || oi) && (dj == 0 || oj);
-      if (staged) {
-        const int row = ((ci + di - ci0) * ncj + (cj + dj - cj0)) * nc + q.kk0;
-#pragma unroll
-        for (int x = 0; x < 5; ++x)
-          c[di][dj][x] = (need && (x < 4 || p)) ? s_c[row + x] : 0.0f;
-      } else {
-        const uint32_t row = (ci + di) * nc2 + static_cast<uint32_t>(cj + dj) * nc + q.kk0;
-#pragma unroll
-        for (int x = 0; x < 5; ++x)
-          c[di][dj][x] = (need && (x < 4 || p)) ? phi_c[row + x] : 0.0f;
-      }
-    }
-  }
-  const float4 v = ld4(e, plane + q.t0);
-  float out[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-  for (int k4 = 0; k4 < 4; ++k4) {
-    const int k = 2 * (q.kk0 + k4) + p;
-    if (k < 1 || k > n - 2) continue;
-    float val = 0.0f;
+  for (int m = 0; m < kProlongQ; ++m) {
+    if (!live[m]) continue;
+    const Quad& qq = q[m];
+    const int p = (qq.i + qq.j + 1) & 1;  // parity of k; k>>1 = kk0 + e
+    const int ci = qq.i >> 1, cj = qq.j >> 1, oi = qq.i & 1, oj = qq.j & 1;
+    float c[2][2][5];
 #pragma unroll
     for (int di = 0; di < 2; ++di) {
-      const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
 #pragma unroll
       for (int dj = 0; dj < 2; ++dj) {
-        const float wj = oj ? 0.5f : (dj == 0 ? 1.0f : 0.0f);
+        const bool need = (di == 0 || oi) && (dj == 0 || oj);
+        if (staged) {
+          const int row = ((ci + di - ci0) * ncj + (cj + dj - cj0)) * nc + qq.kk0;
 #pragma unroll
-        for (int dk = 0; dk < 2; ++dk) {
-          const float wk = p ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
-          const float w = wi * wj * wk;
-          if (w > 0.0f) val += w * c[di][dj][k4 + dk];
+          for (int x = 0; x < 5; ++x)
+            c[di][dj][x] = (need && (x < 4 || p)) ? s_c[row + x] : 0.0f;
+        } else {
+          const uint32_t row = (ci + di) * nc2 + static_cast<uint32_t>(cj + dj) * nc + qq.kk0;
+#pragma unroll
+          for (int x = 0; x < 5; ++x)
+            c[di][dj][x] = (need && (x < 4 || p)) ? phi_c[row + x] : 0.0f;
         }
       }
     }
-    out[k4] = out[k4] + val;
+    float out[4] = {v[m].x, v[m].y, v[m].z, v[m].w};
+#pragma unroll
+    for (int k4 = 0; k4 < 4; ++k4) {
+      const int k = 2 * (qq.kk0 + k4) + p;
+      if (k < 1 || k > n - 2) continue;
+      float val = 0.0f;
+#pragma unroll
+      for (int di = 0; di < 2; ++di) {
+        const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+          const float wj = oj ? 0.5f : (dj == 0 ? 1.0f : 0.0f);
+#pragma unroll
+          for (int dk = 0; dk < 2; ++dk) {
+            const float wk = p ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
+            const float w = wi * wj * wk;
+            if (w > 0.0f) val += w * c[di][dj][k4 + dk];
+          }
+        }
+      }
+      out[k4] = out[k4] + val;
+    }
+    *reinterpret_cast<float4*>(e + plane + qq.t0) = make_float4(out[0], out[1], out[2], out[3]);
   }
-  *reinterpret_cast<float4*>(e + plane + q.t0) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
 // residual64_s_kernel, 8 consecutive natural nodes per thread (n % 8 == 0): the r values
@@ -1743,7 +1759,8 @@ pm_status vcycle_split0(pm_mg* mg, float* phi, int carry, hipStream_t s) {
   {
     PM_TIMED(level_name("mg.prolong_s", n), 4.0 * nodes, s);
     if (quad)
-      prolong_s4_kernel<<<pm::blocks_for(quads), pm::kBlock, 0, s>>>(C.phi, e, n, hn, C.n);
+      prolong_s4_kernel<<<pm::blocks_for((quads + kProlongQ - 1) / kProlongQ), pm::kBlock, 0,
+                          s>>>(C.phi, e, n, hn, C.n);
     else
       prolong_s_kernel<<<pm::blocks_for(plane), pm::kBlock, 0, s>>>(C.phi, e, n, hn, C.n);
     PM_CHECK_LAUNCH("prolong_s_kernel");
