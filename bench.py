@@ -518,7 +518,7 @@ def main() -> int:
         "ms_per_step_instrumented": pass2_ms,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels": dict(sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"])[:16]),
+        "kernels": dict(sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_step"])),
         "mg_finest_roofline": mg_fractions or None,
         "per_stage_ms": per_stage,
         "per_stage_protocol": "unfused stages, each bracketed by device synchronizations "

@@ -56,6 +56,49 @@ __global__ void __launch_bounds__(256) rb_pass_kernel(float* __restrict__ phi,
   phi[c] = (s - h2 * rhs[c]) / 6.0f;
 }
 
+// New colour-0 value of interior node c from the current values (rb_pass_kernel's
+// expression; all six neighbours have colour 1).
+__device__ __forceinline__ float red_value(const float* __restrict__ phi,
+                                           const float* __restrict__ rhs, int64_t c, int64_t n2,
+                                           int n, float h2) {
+  const float s = phi[c + n2] + phi[c - n2] + phi[c + n] + phi[c - n] + phi[c + 1] + phi[c - 1];
+  return (s - h2 * rhs[c]) / 6.0f;
+}
+
+// One red-black sweep (colour 0 then colour 1) of a small level in one launch, out of place,
+// one thread per node: a colour-0 node takes its new value from the old colour-1 values, a
+// colour-1 node recomputes its six colour-0 neighbours' new values (faces keep theirs) and
+// then takes its own, each with rb_pass_kernel's expression, so phi_out is bitwise what the
+// two colour passes leave. Each colour-1 node repeats six neighbour updates: worth it only
+// where a launch costs more than that work (small levels).
+__global__ void __launch_bounds__(256) rb_sweep_small_kernel(const float* __restrict__ phi,
+                                                             float* __restrict__ phi_out,
+                                                             const float* __restrict__ rhs,
+                                                             int n, float h2) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= n2 * n) return;
+  const pm::Idx3 q3 = pm::idx3(c, n);
+  const int i = q3.i, j = q3.j, k = q3.k;
+  auto inner = [n](int a, int b, int d) {
+    return a >= 1 && a <= n - 2 && b >= 1 && b <= n - 2 && d >= 1 && d <= n - 2;
+  };
+  if (!inner(i, j, k)) {
+    phi_out[c] = phi[c];
+    return;
+  }
+  if (((i + j + k) & 1) == 0) {
+    phi_out[c] = red_value(phi, rhs, c, n2, n, h2);
+    return;
+  }
+  auto nb = [&](int64_t cc, int a, int b, int d) {
+    return inner(a, b, d) ? red_value(phi, rhs, cc, n2, n, h2) : phi[cc];
+  };
+  const float s = nb(c + n2, i + 1, j, k) + nb(c - n2, i - 1, j, k) + nb(c + n, i, j + 1, k) +
+                  nb(c - n, i, j - 1, k) + nb(c + 1, i, j, k + 1) + nb(c - 1, i, j, k - 1);
+  phi_out[c] = (s - h2 * rhs[c]) / 6.0f;
+}
+
 // One full red-black sweep (colour 0 then colour 1) in a single pass over the level, out of
 // place: phi_in is only read and every node of phi_out is written, so workgroups never see
 // each other's updates. A workgroup owns a (kSwTJ x kSwTK) tile of a (j,k) plane and marches
@@ -283,78 +326,16 @@ __device__ __forceinline__ float prolong_value(const float* __restrict__ phi_c, 
   return val;
 }
 
-// Staged coarse values of a prolong_add workgroup: rows (ci0 + a, cj0 + b), a < 2,
-// b < ncj, each nc floats; row indices past the coarse grid are clamped to its last row as
-// prolong_value clamps them.
-constexpr int kProlongAddMaxLds = 2 * 4 * 513;
-
 __global__ void __launch_bounds__(256) prolong_add_kernel(const float* __restrict__ phi_c,
                                                           float* __restrict__ phi_f, int nf,
                                                           int nc) {
-  __shared__ float s_c[kProlongAddMaxLds];
   const int64_t nf2 = static_cast<int64_t>(nf) * nf;
-  const int64_t total = nf2 * nf;
-  const int64_t c0 = pm::xcd_block() * blockDim.x;
-  const int64_t c = c0 + threadIdx.x;
-  // When the workgroup's nodes share one i, every coarse value they read lies in coarse
-  // rows ci0..ci0+1 x cj0..cj0+ncj-1 (clamped): staged with coalesced loads instead of 8
-  // scalar loads per node; the values and the arithmetic are prolong_value's.
-  const int64_t last = c0 + blockDim.x - 1 < total ? c0 + blockDim.x - 1 : total - 1;
-  bool staged = false;
-  int ci0 = 0, cj0 = 0, ncj = 0;
-  if (c0 < total) {
-    const pm::Idx3 f0 = pm::idx3(c0, nf), f1 = pm::idx3(last, nf);
-    ci0 = f0.i >> 1;
-    cj0 = f0.j >> 1;
-    ncj = (f1.j >> 1) + 2 - cj0;
-    staged = f0.i == f1.i && ncj >= 1 && 2 * ncj * nc <= kProlongAddMaxLds;
-  }
-  if (staged) {
-    const int per = ncj * nc;
-    const int64_t nc2 = static_cast<int64_t>(nc) * nc;
-    for (int t = threadIdx.x; t < 2 * per; t += blockDim.x) {
-      const int a = t / per, rem = t - a * per;
-      const int b = rem / nc, k = rem - b * nc;
-      const int ca = ci0 + a < nc ? ci0 + a : nc - 1;
-      const int cb = cj0 + b < nc ? cj0 + b : nc - 1;
-      s_c[t] = phi_c[ca * nc2 + static_cast<int64_t>(cb) * nc + k];
-    }
-  }
-  __syncthreads();
-  if (c >= total) return;
+  const int64_t c = pm::xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= nf2 * nf) return;
   const pm::Idx3 q3 = pm::idx3(c, nf);
   const int i = q3.i, j = q3.j, k = q3.k;
   if (i < 1 || i > nf - 2 || j < 1 || j > nf - 2 || k < 1 || k > nf - 2) return;
-  if (!staged) {
-    phi_f[c] = phi_f[c] + prolong_value(phi_c, nc, i, j, k);
-    return;
-  }
-  const int ci = i >> 1, cj = j >> 1, ck = k >> 1;
-  const int oi = i & 1, oj = j & 1, ok = k & 1;
-  float cv[8];
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    // the same clamped coarse node as prolong_value: rows via the staged clamp, k here
-    const int a = ci + (t >> 2) - ci0, b = cj + ((t >> 1) & 1) - cj0, d = ck + (t & 1);
-    cv[t] = s_c[(a * ncj + b) * nc + (d < nc ? d : nc - 1)];
-  }
-  float val = 0.0f;
-#pragma unroll
-  for (int di = 0; di < 2; ++di) {
-    const float wi = oi ? 0.5f : (di == 0 ? 1.0f : 0.0f);
-#pragma unroll
-    for (int dj = 0; dj < 2; ++dj) {
-      const float wj = oj ? 0.5f : (dj == 0 ? 1.0f : 0.0f);
-#pragma unroll
-      for (int dk = 0; dk < 2; ++dk) {
-        const float wk = ok ? 0.5f : (dk == 0 ? 1.0f : 0.0f);
-        const float w = wi * wj * wk;
-        const float tt = val + w * cv[di * 4 + dj * 2 + dk];
-        val = w > 0.0f ? tt : val;
-      }
-    }
-  }
-  phi_f[c] = phi_f[c] + val;
+  phi_f[c] = phi_f[c] + prolong_value(phi_c, nc, i, j, k);
 }
 
 __global__ void __launch_bounds__(256) fill_f32_kernel(float* __restrict__ p, int64_t count,
@@ -1522,6 +1503,7 @@ struct pm_mg {
   int hn = 0;
   bool use_split = true;
   bool use_bottom = true;  // levels from bottom_from on run in the one-workgroup solver
+  bool use_small_sweep = true;  // levels <= kSmallSweepMax smooth with rb_sweep_small
   // the 4-wide split reductions (float64 defect, fused update) run one quad per thread:
   // quad_blocks workgroups with their own partial sums
   unsigned quad_blocks = 0;
@@ -1579,12 +1561,39 @@ pm_status rb_sweep(const float* phi_in, float* phi_out, const float* rhs, int n,
   return PM_OK;
 }
 
+// Levels up to this size (below the finest) smooth with rb_sweep_small_kernel, one launch
+// per sweep ping-ponging through L.tmp, instead of two colour passes: there a launch costs
+// more than the repeated neighbour updates (PM_MG_OPT_SMALL_SWEEP).
+constexpr int kSmallSweepMax = 65;
+
+pm_status rb_sweep_small(const float* phi_in, float* phi_out, const float* rhs, int n,
+                         float h2, hipStream_t s) {
+  const int64_t nodes = static_cast<int64_t>(n) * n * n;
+  PM_TIMED(level_name("mg.rb_sweep_small", n), 12.0 * nodes, s);
+  rb_sweep_small_kernel<<<pm::blocks_for(nodes), pm::kBlock, 0, s>>>(phi_in, phi_out, rhs, n,
+                                                                      h2);
+  PM_CHECK_LAUNCH("rb_sweep_small_kernel");
+  return PM_OK;
+}
+
+// `sweeps` sweeps from *cur (L.phi or L.tmp), each into the other buffer; *cur becomes the
+// buffer holding the result.
+pm_status smooth_pp(const Level& L, int sweeps, float** cur, hipStream_t s) {
+  pm_status st;
+  for (int q = 0; q < sweeps; ++q) {
+    float* out = *cur == L.phi ? L.tmp : L.phi;
+    if ((st = rb_sweep_small(*cur, out, L.rhs, L.n, L.h2, s))) return st;
+    *cur = out;
+  }
+  return PM_OK;
+}
+
 // `sweeps` red-black sweeps in place on L.phi. Large levels ping-pong pairs of fused sweeps
 // through L.tmp (ending back in L.phi); a leftover single sweep uses two colour passes.
 pm_status smooth(const Level& L, int sweeps, hipStream_t s) {
   pm_status st;
   int q = 0;
-  if (L.tmp) {
+  if (L.tmp && L.n >= kSweepMin) {
     for (; q + 2 <= sweeps; q += 2) {
       if ((st = rb_sweep(L.phi, L.tmp, L.rhs, L.n, L.h2, s))) return st;
       if ((st = rb_sweep(L.tmp, L.phi, L.rhs, L.n, L.h2, s))) return st;
@@ -1667,10 +1676,19 @@ pm_status vcycle(pm_mg* mg, size_t l, hipStream_t s) {
     return bottom(mg, l, s);
   if (l + 1 == mg->lv.size()) return coarsest(mg, L, s);
   pm_status st;
-  if ((st = smooth(L, mg->d.pre_sweeps, s))) return st;
+  // small levels: one launch per sweep through L.tmp; an even number of sweeps in the cycle
+  // leaves the result in L.phi
+  const bool pp = mg->use_small_sweep && L.tmp && L.n <= kSmallSweepMax &&
+                  (mg->d.pre_sweeps + mg->d.post_sweeps) % 2 == 0;
+  float* cur = L.phi;
+  if (pp) {
+    if ((st = smooth_pp(L, mg->d.pre_sweeps, &cur, s))) return st;
+  } else if ((st = smooth(L, mg->d.pre_sweeps, s))) {
+    return st;
+  }
   {
     PM_TIMED(level_name("mg.residual", L.n), kResidualBytes * L.nodes(), s);
-    residual_kernel<<<pm::blocks_for(L.nodes()), pm::kBlock, 0, s>>>(L.phi, L.rhs, L.res, L.n,
+    residual_kernel<<<pm::blocks_for(L.nodes()), pm::kBlock, 0, s>>>(cur, L.rhs, L.res, L.n,
                                                                       L.inv_h2);
     PM_CHECK_LAUNCH("residual_kernel");
   }
@@ -1684,9 +1702,10 @@ pm_status vcycle(pm_mg* mg, size_t l, hipStream_t s) {
   if ((st = vcycle(mg, l + 1, s))) return st;
   {
     PM_TIMED(level_name("mg.prolong", L.n), kProlongBytes * L.nodes(), s);
-    prolong_add_kernel<<<pm::blocks_for(L.nodes()), pm::kBlock, 0, s>>>(C.phi, L.phi, L.n, C.n);
+    prolong_add_kernel<<<pm::blocks_for(L.nodes()), pm::kBlock, 0, s>>>(C.phi, cur, L.n, C.n);
     PM_CHECK_LAUNCH("prolong_add_kernel");
   }
+  if (pp) return smooth_pp(L, mg->d.post_sweeps, &cur, s);  // ends in L.phi
   return smooth(L, mg->d.post_sweeps, s);
 }
 
@@ -2129,7 +2148,7 @@ pm_status pm_mg_create(const pm_mg_desc* desc, pm_mg** out) {
         return st;
       }
     }
-    if (L.n >= kSweepMin) {
+    if (L.n >= kSweepMin || (l > 0 && l + 1 < sizes.size() && L.n <= kSmallSweepMax)) {
       if ((st = dalloc(mg, reinterpret_cast<void**>(&L.tmp), bytes))) {
         destroy(mg);
         return st;
@@ -2378,7 +2397,7 @@ pm_status pm_mg_set_option(pm_mg* mg, int32_t option, int32_t value) {
   pm::clear_error();
   PM_REQUIRE(mg, "pm_mg_set_option: null handle");
   PM_REQUIRE(option == PM_MG_OPT_GRAPHS || option == PM_MG_OPT_SPLIT_FINE ||
-                 option == PM_MG_OPT_BOTTOM,
+                 option == PM_MG_OPT_BOTTOM || option == PM_MG_OPT_SMALL_SWEEP,
              "pm_mg_set_option: unknown option");
   // captured cycles bake in the launch sequence the options select
   for (auto& g : mg->graphs) (void)hipGraphExecDestroy(g.exec);
@@ -2389,6 +2408,9 @@ pm_status pm_mg_set_option(pm_mg* mg, int32_t option, int32_t value) {
       break;
     case PM_MG_OPT_SPLIT_FINE:
       mg->use_split = value != 0;
+      break;
+    case PM_MG_OPT_SMALL_SWEEP:
+      mg->use_small_sweep = value != 0;
       break;
     default:
       mg->use_bottom = value != 0;

@@ -21,6 +21,7 @@ PM_MG_MIXED = 1
 PM_MG_OPT_GRAPHS = 0
 PM_MG_OPT_SPLIT_FINE = 1
 PM_MG_OPT_BOTTOM = 2
+PM_MG_OPT_SMALL_SWEEP = 3
 PM_DEPOSIT_SORTED = 0
 PM_DEPOSIT_FIXED = 1
 PM_WARM_PREVIOUS = 0

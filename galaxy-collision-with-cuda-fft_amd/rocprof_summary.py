@@ -46,8 +46,30 @@ def _timed_dispatches(db, kernel: str, last: int) -> str:
         return f"# {kernel}: no dispatches"
     d = sum(durs) / len(durs) / 1e3
     g = sum(gaps) / len(gaps) / 1e3
+    # device occupancy from the first of those dispatches to the end of the trace: the union
+    # of kernel intervals against the span (the rest is time with no kernel running)
+    starts = [s1 for (n1, s1, _) in rows if short(n1) == kernel][-last:]
+    t0, t1 = starts[0], rows[-1][2]
+    busy, cur_s, cur_e, kernels = 0, None, None, 0
+    for _, s1, e1 in rows:
+        if e1 <= t0:
+            continue
+        s1 = max(s1, t0)
+        kernels += 1
+        if cur_e is None or s1 > cur_e:
+            if cur_e is not None:
+                busy += cur_e - cur_s
+            cur_s, cur_e = s1, e1
+        else:
+            cur_e = max(cur_e, e1)
+    if cur_e is not None:
+        busy += cur_e - cur_s
+    span = t1 - t0
     return (f"# {kernel}, last {len(durs)} dispatches: duration {d:.2f} us, gap since the "
-            f"previous kernel {g:.2f} us, duration + gap {d + g:.2f} us")
+            f"previous kernel {g:.2f} us, duration + gap {d + g:.2f} us\n"
+            f"# from the first of them to the end: {span / 1e3:.1f} us, {kernels} kernels, "
+            f"some kernel running {busy / 1e3:.1f} us ({100.0 * busy / span:.1f}%), none "
+            f"{(span - busy) / 1e3:.1f} us")
 
 
 def trace(db_path: str, kernel: str | None = None, last: int = 0) -> str:

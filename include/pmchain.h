@@ -342,10 +342,15 @@ pm_status pm_mg_warm_reset(pm_mg* mg);
  *   PM_MG_OPT_BOTTOM (default 1): the levels of size <= 17 at the bottom of the hierarchy
  *     run their part of each V-cycle in one workgroup with every level in LDS; 0 runs them
  *     with the per-level kernels.
+ *   PM_MG_OPT_SMALL_SWEEP (default 1): levels of size <= 65 below the finest smooth with one
+ *     launch per red-black sweep (out of place, each colour-1 node recomputing its
+ *     colour-0 neighbours) when the cycle's pre + post sweeps are even; 0 uses two colour
+ *     passes per sweep.
  * Setting an option drops the handle's captured cycles; they are captured again on use. */
 #define PM_MG_OPT_GRAPHS 0
 #define PM_MG_OPT_SPLIT_FINE 1
 #define PM_MG_OPT_BOTTOM 2
+#define PM_MG_OPT_SMALL_SWEEP 3
 pm_status pm_mg_set_option(pm_mg* mg, int32_t option, int32_t value);
 
 /* One more V-cycle on the state of the last solve (reference multigrid.py:429-440

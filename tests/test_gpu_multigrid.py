@@ -344,3 +344,34 @@ def test_bottom_solver_equals_per_level_kernels(device, mode, n):
         out.append((phi.clone(), c0, s.last_cycles))
     assert torch.equal(out[0][0], out[1][0])
     assert out[0][1:] == out[1][1:]
+
+
+@pytest.mark.parametrize("mode,n,sweeps,bottom", [("mixed", 129, (1, 1), 1), ("mixed", 129, (2, 2), 1),
+                                                  ("mixed", 65, (1, 1), 0), ("plain", 128, (2, 2), 1),
+                                                  ("plain", 64, (1, 1), 0), ("mixed", 129, (1, 2), 1)])
+def test_small_level_fused_sweep_equals_colour_passes(device, mode, n, sweeps, bottom):
+    """One launch per red-black sweep on the levels <= 65 below the finest (out of place,
+    colour-1 nodes recomputing their colour-0 neighbours) gives bitwise the phi and cycle
+    counts of two colour passes per sweep, cold and warm, with the bottom solver on or off;
+    an odd number of sweeps per cycle keeps the colour passes."""
+    announce(f"small-level fused sweep {mode} n={n} sweeps={sweeps} bottom={bottom}")
+    from galaxy_collision_amd import _native as nat
+    from galaxy_collision_amd.solver.multigrid import (MixedPrecisionMultigridSolver,
+                                                       MultigridPoissonSolver)
+
+    dx = 40.0 / n
+    rho = _t(_gaussian_rho(n, dx, 3.0 * dx, 1.0, centre=(0.46 * n * dx, 0.52 * n * dx,
+                                                          0.5 * n * dx)), device)
+    out = []
+    for small in (1, 0):
+        cls = MixedPrecisionMultigridSolver if mode == "mixed" else MultigridPoissonSolver
+        s = cls(n, dx=dx, grav_constant=1.0, pre_sweeps=sweeps[0], post_sweeps=sweeps[1])
+        nat.call("pm_mg_set_option", s._h, nat.PM_MG_OPT_SMALL_SWEEP, small)
+        nat.call("pm_mg_set_option", s._h, nat.PM_MG_OPT_BOTTOM, bottom)
+        phi = torch.zeros((n,) * 3, device=device)
+        s.solve(rho, phi)
+        c0 = s.last_cycles
+        s.solve(rho * 1.01, phi, warm_start=True)
+        out.append((phi.clone(), c0, s.last_cycles))
+    assert torch.equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
