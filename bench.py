@@ -134,7 +134,8 @@ def accuracy_series(icr, w: dict, dx: float, grav: float, make, steps: int,
 
     parts = ParticleState(icr.n)
     parts.load(icr.pos, icr.vel, icr.mass, icr.gid)
-    chain = PMChain(parts, w["grid"], dx, make(), reorder_every=w.get("reorder_every", 0))
+    chain = PMChain(parts, w["grid"], dx, make(), reorder_every=w.get("reorder_every", 0),
+                    deposit_mode=w.get("deposit", "sorted"))
     check = Checkpoint(w["grid"], dx, grav)
     chain.prime()
     points = [{"step": 0, **check.measure(parts, chain.grid.rho, chain.grid.phi)}]
@@ -190,7 +191,7 @@ def warm_start_ab(icr, w: dict, dx: float, grav: float, steps: int) -> dict:
             if "sweeps" in w:
                 kw["pre_sweeps"], kw["post_sweeps"] = w["sweeps"]
         solver = make_solver(name, w["grid"], **kw)
-        chain = PMChain(parts, w["grid"], dx, solver)
+        chain = PMChain(parts, w["grid"], dx, solver, deposit_mode=w.get("deposit", "sorted"))
         chain.prime()
         for _ in range(3):
             chain.step(w["dt"])
@@ -267,6 +268,8 @@ def main() -> int:
     ap.add_argument("--reorder-every", type=int, default=None,
                     help="steps between moves of the particles into cell order (0: never; "
                          "default: the workload's)")
+    ap.add_argument("--deposit-mode", default=None, choices=["sorted", "fixed", "bricked"],
+                    help="the chain's deposit mode (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of timed CPU steps")
     ap.add_argument("--cpu-threads", type=int, default=None)
@@ -315,6 +318,8 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
     w = dict(WORKLOADS[args.workload])
+    if args.deposit_mode:
+        w["deposit"] = args.deposit_mode
     solver_name = args.solver or w["solver"]
     grav = 1.0
     t_ic = time.perf_counter()
@@ -342,7 +347,8 @@ def main() -> int:
     make = lambda: make_solver(solver_name, w["grid"], **kw)  # noqa: E731
     solver = make()
     reorder = args.reorder_every if args.reorder_every is not None else w.get("reorder_every", 0)
-    chain = PMChain(parts, w["grid"], dx, solver, reorder_every=reorder)
+    chain = PMChain(parts, w["grid"], dx, solver, reorder_every=reorder,
+                    deposit_mode=w.get("deposit", "sorted"))
     chain.prime()
     chain.run(args.warmup, w["dt"])
     torch.cuda.synchronize()

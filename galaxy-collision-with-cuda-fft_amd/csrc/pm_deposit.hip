@@ -20,7 +20,9 @@
 
 #include "pm_cic.h"
 #include "pm_common.h"
+#include "pm_deposit_bricked.h"
 #include "pm_deposit_fixed.h"
+#include "pm_radix.h"
 #include "pm_timing.h"
 
 namespace {
@@ -29,7 +31,8 @@ constexpr int kChunk = 16;           // positions per piece chunk (one thread ea
 constexpr int kSortWaves = 4;        // waves per radix workgroup
 constexpr int kSortItems = 16;       // keys per lane
 constexpr int kTile = kSortWaves * 64 * kSortItems;  // 4096 keys per radix tile
-constexpr int kMaxDigitBits = 9;     // 25-bit keys of a 256^3 grid sort in 3 passes
+constexpr int kMaxDigitBits = pm::radix::kMaxDigitBits;  // 25-bit cell keys: 3 passes
+static_assert(kTile == pm::radix::kTileKeys, "radix tile size");
 constexpr int kSmallPieces = 8;      // open cells with <= this many pieces finish in one thread
 constexpr int kWavePieces = 256;     // ... in one wave; larger ones in a workgroup
 constexpr int kBigBatch = 1024;      // pieces per batch for large cells
@@ -967,6 +970,57 @@ __global__ void __launch_bounds__(256) reset_map_kernel(int32_t* __restrict__ ma
 
 }  // namespace
 
+namespace pm {
+namespace radix {
+
+int plan_digits(uint64_t limit, int bits[4]) {
+  int key_bits = 1;
+  while ((uint64_t(1) << key_bits) <= limit) ++key_bits;
+  const int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  for (int s = 0; s < 4; ++s)
+    bits[s] = s < passes ? key_bits / passes + (s < key_bits % passes ? 1 : 0) : 0;
+  return passes;
+}
+
+pm_status sort(const uint32_t* keys_in, uint32_t* const keys[2], int32_t* const vals[2],
+               uint32_t* hist, uint32_t* digit, int64_t np, int shift, const int* bits,
+               int passes, int* out, hipStream_t stream) {
+  PM_REQUIRE(passes >= 1 && passes <= 4, "radix sort: 1 to 4 passes");
+  const int tiles = static_cast<int>(tiles_of(np));
+  int src = 0;
+  for (int s = 0; s < passes; ++s) {
+    const int b = bits[s];
+    PM_REQUIRE(b >= 1 && b <= kMaxDigitBits, "radix sort: digit width out of range");
+    const int buckets = 1 << b;
+    const uint32_t* kin = s == 0 ? keys_in : keys[src];
+    {
+      PM_TIMED("deposit.radix_hist", 4.0 * np, stream);
+      radix_hist_kernel<<<tiles, 256, 0, stream>>>(kin, np, tiles, shift, b, hist);
+      PM_CHECK_LAUNCH("radix_hist_kernel");
+    }
+    {
+      PM_TIMED("deposit.radix_scan", 8.0 * buckets * tiles, stream);
+      radix_scan_kernel<<<buckets, 256, 0, stream>>>(hist, tiles, digit);
+      PM_CHECK_LAUNCH("radix_scan_kernel");
+    }
+    {
+      // read key (+ index after the first pass), write key + index
+      PM_LAUNCH_TIMED("deposit.radix_scatter", (s == 0 ? 12.0 : 16.0) * np,
+                      radix_scatter_kernel, tiles, 256, 0, stream, kin,
+                      s == 0 ? nullptr : vals[src], keys[1 - src], vals[1 - src], hist, digit,
+                      np, tiles, shift, b);
+      PM_CHECK_LAUNCH("radix_scatter_kernel");
+    }
+    src = 1 - src;
+    shift += b;
+  }
+  *out = src;
+  return PM_OK;
+}
+
+}  // namespace radix
+}  // namespace pm
+
 struct pm_deposit_plan {
   int n = 0;
   int64_t cap = 0;
@@ -999,6 +1053,7 @@ struct pm_deposit_plan {
   int mode = PM_DEPOSIT_SORTED;
   unsigned long long* acc64 = nullptr;  // PM_DEPOSIT_FIXED: n^3 accumulators (kept zeroed)
   unsigned int* maxbits = nullptr;
+  pm::depbrick::State* brick = nullptr;  // PM_DEPOSIT_BRICKED
 };
 
 namespace {
@@ -1012,6 +1067,7 @@ void free_plan(pm_deposit_plan* p) {
     if (b) (void)hipFree(b);
   for (auto& e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  pm::depbrick::destroy(p->brick);
   delete p;
 }
 
@@ -1033,11 +1089,7 @@ void init_plan_shape(pm_deposit_plan* p, int n, int64_t cap) {
   p->cap = cap;
   const int64_t m = static_cast<int64_t>(n) + 1;
   p->sentinel = static_cast<uint32_t>(m * m * m);
-  int key_bits = 1;
-  while ((int64_t(1) << key_bits) <= static_cast<int64_t>(p->sentinel)) ++key_bits;
-  p->passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
-  for (int s = 0; s < p->passes; ++s)
-    p->digit_bits[s] = key_bits / p->passes + (s < key_bits % p->passes ? 1 : 0);
+  p->passes = pm::radix::plan_digits(p->sentinel, p->digit_bits);
 }
 
 // Calls f(buffer pointer, element count, name) for every sorted-mode scratch buffer of a
@@ -1112,8 +1164,14 @@ pm_status pm_deposit_plan_destroy(pm_deposit_plan* plan) {
 pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode) {
   pm::clear_error();
   PM_REQUIRE(plan, "pm_deposit_plan_set_mode: null plan");
-  PM_REQUIRE(mode == PM_DEPOSIT_SORTED || mode == PM_DEPOSIT_FIXED,
+  PM_REQUIRE(mode == PM_DEPOSIT_SORTED || mode == PM_DEPOSIT_FIXED || mode == PM_DEPOSIT_BRICKED,
              "pm_deposit_plan_set_mode: unknown mode");
+  if (mode == PM_DEPOSIT_BRICKED && !plan->brick) {
+    pm_status st = pm::depbrick::create(plan->n, plan->cap, &plan->brick);
+    if (st != PM_OK) return st;
+  }
+  // a sorted-mode deposit overwrites the sort buffers that hold the binning's order
+  if (plan->brick && mode != plan->mode) pm::depbrick::invalidate(plan->brick);
   if (mode == PM_DEPOSIT_FIXED && !plan->acc64) {
     const int64_t nodes = static_cast<int64_t>(plan->n) * plan->n * plan->n;
     pm_status st;
@@ -1130,6 +1188,10 @@ pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode) {
 pm_status pm_deposit_plan_brick_list(pm_deposit_plan* plan, const int32_t** list,
                                      const uint32_t** count, int64_t* max_count) {
   PM_REQUIRE(plan && list && count, "pm_deposit_plan_brick_list: null pointer");
+  if (plan->mode == PM_DEPOSIT_BRICKED) {
+    pm::depbrick::node_list(plan->brick, list, count, max_count);
+    return PM_OK;
+  }
   *list = plan->bricks_valid ? plan->brick_list : nullptr;
   *count = plan->bricks_valid ? plan->brick_count : nullptr;
   if (max_count) *max_count = nbricks_of(plan->n);
@@ -1138,8 +1200,16 @@ pm_status pm_deposit_plan_brick_list(pm_deposit_plan* plan, const int32_t** list
 
 pm_status pm_deposit_plan_order(pm_deposit_plan* plan, const int32_t** order) {
   PM_REQUIRE(plan && order, "pm_deposit_plan_order: null pointer");
-  *order = plan->order;
+  *order = plan->mode == PM_DEPOSIT_BRICKED ? pm::depbrick::order(plan->brick) : plan->order;
   return PM_OK;
+}
+
+pm_status pm_deposit_plan_mark_ordered(pm_deposit_plan* plan) {
+  pm::clear_error();
+  PM_REQUIRE(plan, "pm_deposit_plan_mark_ordered: null plan");
+  PM_REQUIRE(plan->mode == PM_DEPOSIT_BRICKED,
+             "pm_deposit_plan_mark_ordered: needs PM_DEPOSIT_BRICKED mode");
+  return pm::depbrick::mark_ordered(plan->brick);
 }
 
 pm_status pm_deposit_plan_node_bricks(pm_deposit_plan* plan, const uint8_t** bricks,
@@ -1173,10 +1243,23 @@ namespace {
 
 // The deposit; keys_in (optional) holds the particles' keys already, so the key pass is
 // skipped and the first sort pass reads them where they lie.
+pm::depbrick::SortBuffers sort_buffers(pm_deposit_plan* p) {
+  pm::depbrick::SortBuffers sb;
+  sb.keys[0] = p->keys[0];
+  sb.keys[1] = p->keys[1];
+  sb.vals[0] = p->vals[0];
+  sb.vals[1] = p->vals[1];
+  sb.hist = p->hist;
+  sb.digit = p->digit;
+  return sb;
+}
+
+// bin_always: a PM_DEPOSIT_BRICKED deposit bins the particles again (pm_deposit_cic) rather
+// than reusing the last binning (pm_deposit_cic_step without PM_STEP_BIN).
 pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, const float* pz,
                       const float* mass, int64_t np, float dx, float* rho,
                       const uint32_t* keys_in, uint32_t* keys_out, int32_t* perm_out,
-                      hipStream_t stream) {
+                      hipStream_t stream, bool bin_always = true, bool rho_kept = false) {
   PM_REQUIRE(plan, "pm_deposit_cic: null plan");
   PM_REQUIRE(rho, "pm_deposit_cic: null rho");
   PM_REQUIRE(np >= 0, "pm_deposit_cic: negative particle count");
@@ -1189,6 +1272,12 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
   PM_REQUIRE(np == 0 || (px && py && pz && mass), "pm_deposit_cic: null particle array");
   plan->bricks_valid = plan->mode == PM_DEPOSIT_SORTED;
   plan->order = nullptr;
+  if (plan->mode == PM_DEPOSIT_BRICKED) {
+    PM_REQUIRE(!keys_in && !keys_out && !perm_out,
+               "pm_deposit_cic: PM_DEPOSIT_BRICKED mode takes no keys or permutation");
+    return pm::depbrick::deposit(plan->brick, sort_buffers(plan), px, py, pz, mass, np, dx, rho,
+                                 bin_always, rho_kept, stream);
+  }
   if (plan->mode == PM_DEPOSIT_FIXED) {
     PM_REQUIRE(!perm_out, "pm_deposit_cic: the particle order needs PM_DEPOSIT_SORTED mode");
     return pm::depfix::deposit(px, py, pz, mass, np, plan->n, dx, rho, keys_out, plan->acc64,
@@ -1247,32 +1336,12 @@ pm_status deposit_cic(pm_deposit_plan* plan, const float* px, const float* py, c
   if (timing) PM_TRY_HIP(hipEventRecord(plan->ev[1], stream));
 
   // 2. sort (the first pass sorts particle indices: no index array is read)
-  const int tiles = static_cast<int>((np + kTile - 1) / kTile);
-  int src = 0, shift = 0;
-  for (int s = 0; s < plan->passes; ++s) {
-    const int bits = plan->digit_bits[s];
-    const int buckets = 1 << bits;
-    const uint32_t* kin = (s == 0 && keys_in) ? keys_in : plan->keys[src];
-    {
-      PM_TIMED("deposit.radix_hist", 4.0 * np, stream);
-      radix_hist_kernel<<<tiles, 256, 0, stream>>>(kin, np, tiles, shift, bits, plan->hist);
-      PM_CHECK_LAUNCH("radix_hist_kernel");
-    }
-    {
-      PM_TIMED("deposit.radix_scan", 8.0 * buckets * tiles, stream);
-      radix_scan_kernel<<<buckets, 256, 0, stream>>>(plan->hist, tiles, plan->digit);
-      PM_CHECK_LAUNCH("radix_scan_kernel");
-    }
-    {
-      // read key (+ index after the first pass), write key + index
-      PM_LAUNCH_TIMED("deposit.radix_scatter", (s == 0 ? 12.0 : 16.0) * np,
-                      radix_scatter_kernel, tiles, 256, 0, stream, kin,
-                      s == 0 ? nullptr : plan->vals[src], plan->keys[1 - src],
-                      plan->vals[1 - src], plan->hist, plan->digit, np, tiles, shift, bits);
-      PM_CHECK_LAUNCH("radix_scatter_kernel");
-    }
-    src = 1 - src;
-    shift += bits;
+  int src = 0;
+  {
+    pm_status st = pm::radix::sort(keys_in ? keys_in : plan->keys[0], plan->keys, plan->vals,
+                                   plan->hist, plan->digit, np, 0, plan->digit_bits,
+                                   plan->passes, &src, stream);
+    if (st != PM_OK) return st;
   }
   const uint32_t* skey = plan->keys[src];
   const int32_t* sidx = plan->vals[src];
@@ -1354,6 +1423,16 @@ pm_status pm_deposit_cic_keyed(pm_deposit_plan* plan, const float* px, const flo
                                hipStream_t stream) {
   PM_REQUIRE(np == 0 || keys, "pm_deposit_cic_keyed: null keys");
   return deposit_cic(plan, px, py, pz, mass, np, dx, rho, keys, nullptr, perm_out, stream);
+}
+
+pm_status pm_deposit_cic_step(pm_deposit_plan* plan, const float* px, const float* py,
+                              const float* pz, const float* mass, int64_t np, float dx,
+                              float* rho, int32_t flags, hipStream_t stream) {
+  PM_REQUIRE(plan, "pm_deposit_cic_step: null plan");
+  PM_REQUIRE(plan->mode == PM_DEPOSIT_BRICKED, "pm_deposit_cic_step: needs PM_DEPOSIT_BRICKED mode");
+  PM_REQUIRE((flags & ~(PM_STEP_BIN | PM_STEP_RHO_KEPT)) == 0, "pm_deposit_cic_step: unknown flags");
+  return deposit_cic(plan, px, py, pz, mass, np, dx, rho, nullptr, nullptr, nullptr, stream,
+                     (flags & PM_STEP_BIN) != 0, (flags & PM_STEP_RHO_KEPT) != 0);
 }
 
 }  // extern "C"

@@ -16,6 +16,10 @@ pm_status deposit(const float* px, const float* py, const float* pz, const float
                   int64_t np, int n, float dx, float* rho, uint32_t* keys_out,
                   unsigned long long* acc, unsigned int* maxbits, hipStream_t s);
 pm_status zero(unsigned long long* acc, int64_t count, hipStream_t s);
+// maxbits[0] = float bits of the largest finite positive mass*inv_vol (0 if none);
+// maxbits[1 .. kMaxBlocks] are scratch.
+pm_status max_mprime(const float* mass, int64_t np, float inv_vol, unsigned int* maxbits,
+                     hipStream_t s);
 
 }  // namespace depfix
 }  // namespace pm

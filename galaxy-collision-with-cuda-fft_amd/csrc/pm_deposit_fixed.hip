@@ -28,6 +28,7 @@
 #include "pm_cic.h"
 #include "pm_common.h"
 #include "pm_deposit_fixed.h"
+#include "pm_fixed.h"
 #include "pm_timing.h"
 
 namespace {
@@ -71,10 +72,7 @@ __global__ void __launch_bounds__(256) max_fold_kernel(const unsigned int* __res
 }
 
 __device__ __forceinline__ int scale_exp(unsigned int maxbits, int b) {
-  if (maxbits == 0u) return 0;
-  int e;
-  (void)frexp(static_cast<double>(__uint_as_float(maxbits)), &e);  // max in [2^(e-1), 2^e)
-  return 61 - (e - 1) - b;
+  return pm::fixed_scale(maxbits, b);
 }
 
 __device__ __forceinline__ void add_u64(uint32_t& lo, uint32_t& hi, uint32_t lo2, uint32_t hi2) {
@@ -213,15 +211,6 @@ __global__ void __launch_bounds__(256) zero_u64_kernel(unsigned long long* __res
   if (i < count) p[i] = 0ull;
 }
 
-int bit_length(int64_t v) {
-  int b = 0;
-  while (v > 0) {
-    ++b;
-    v >>= 1;
-  }
-  return b;
-}
-
 }  // namespace
 
 namespace pm {
@@ -233,21 +222,28 @@ pm_status zero(unsigned long long* acc, int64_t count, hipStream_t s) {
   return PM_OK;
 }
 
+pm_status max_mprime(const float* mass, int64_t np, float inv_vol, unsigned int* maxbits,
+                     hipStream_t s) {
+  // maxbits[0] = the scale word, maxbits[1 .. kMaxBlocks] = per-block maxima
+  const unsigned blocks = np > 0 ? std::min<unsigned>(pm::stream_blocks(np), kMaxBlocks) : 1u;
+  PM_TIMED("deposit.fixed_max", 4.0 * np, s);
+  max_mprime_kernel<<<blocks, 256, 0, s>>>(mass, np, inv_vol, maxbits + 1);
+  PM_CHECK_LAUNCH("max_mprime_kernel");
+  max_fold_kernel<<<1, 256, 0, s>>>(maxbits + 1, static_cast<int>(blocks), maxbits);
+  PM_CHECK_LAUNCH("max_fold_kernel");
+  return PM_OK;
+}
+
 pm_status deposit(const float* px, const float* py, const float* pz, const float* mass,
                   int64_t np, int n, float dx, float* rho, uint32_t* keys_out,
                   unsigned long long* acc, unsigned int* maxbits, hipStream_t s) {
   PM_REQUIRE(np < (int64_t(1) << 31), "pm_deposit_cic: at most 2^31 - 1 particles");
   const float inv_vol = 1.0f / ((dx * dx) * dx);
-  const int b = bit_length(np);
+  const int b = pm::bit_length(np);
   const int64_t nodes = static_cast<int64_t>(n) * n * n;
   {
-    // maxbits[0] = the scale word, maxbits[1 .. kMaxBlocks] = per-block maxima
-    const unsigned blocks = np > 0 ? std::min<unsigned>(pm::stream_blocks(np), kMaxBlocks) : 1u;
-    PM_TIMED("deposit.fixed_max", 4.0 * np, s);
-    max_mprime_kernel<<<blocks, 256, 0, s>>>(mass, np, inv_vol, maxbits + 1);
-    PM_CHECK_LAUNCH("max_mprime_kernel");
-    max_fold_kernel<<<1, 256, 0, s>>>(maxbits + 1, static_cast<int>(blocks), maxbits);
-    PM_CHECK_LAUNCH("max_fold_kernel");
+    pm_status st = max_mprime(mass, np, inv_vol, maxbits, s);
+    if (st != PM_OK) return st;
   }
   if (np > 0) {
     const int64_t waves = (np + 64 * kGroups - 1) / (64 * kGroups);

@@ -24,6 +24,9 @@ PM_MG_OPT_BOTTOM = 2
 PM_MG_OPT_SMALL_SWEEP = 3
 PM_DEPOSIT_SORTED = 0
 PM_DEPOSIT_FIXED = 1
+PM_DEPOSIT_BRICKED = 2
+PM_STEP_BIN = 1
+PM_STEP_RHO_KEPT = 2
 PM_WARM_PREVIOUS = 0
 PM_WARM_EXTRAPOLATE = 1
 PM_WARM_LEARNED = 2
@@ -147,6 +150,8 @@ _SIGNATURES = {
     "pm_deposit_plan_destroy": [_vp],
     "pm_deposit_cic": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "pm_deposit_cic_keyed": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
+    "pm_deposit_cic_step": [_vp, _vp, _vp, _vp, _vp, _i64, _f, _vp, _i32, _vp],
+    "pm_deposit_plan_mark_ordered": [_vp],
     "pm_deposit_cic_noalloc": [_i32, _vp, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp, _vp],
     "pm_deposit_plan_set_timing": [_vp, _i32],
     "pm_deposit_plan_set_mode": [_vp, _i32],

@@ -33,10 +33,14 @@ from galaxy_collision_amd.deposit import DepositPlan
 
 
 class PMChain:
-    """deposit_mode: "sorted" (default: cell-key sort + ordered segmented reduction) or
-    "fixed" (64-bit fixed-point integer atomics, order independent); both deterministic.
-    At 256^3 / 10M particles "fixed" is slower (its 64-bit device-scope atomics), see the
-    deposit mode notes in pmchain.h.
+    """deposit_mode: "sorted" (default: cell-key sort + ordered segmented reduction),
+    "fixed" (64-bit fixed-point integer atomics, order independent) or "bricked" (the
+    "fixed" result bit for bit, summed per 8^3-cell brick in LDS from a binning of the
+    particles that is redone only when the particles are reordered, or every 4 deposits
+    without reordering); all deterministic. At 256^3 / 10M particles "fixed" is slower (its
+    64-bit device-scope atomics), see the deposit mode notes in pmchain.h. The fixed-point
+    modes do not depend on the particles' order, so reordering leaves their runs bitwise
+    unchanged (up to the slots).
 
     reorder_every: K > 0 moves the particles into the last deposit's cell order in the first
     drift of a run that follows K or more deposits since the last move
@@ -74,6 +78,11 @@ class PMChain:
         self.reorder_every = int(reorder_every)
         self.steps_done = 0
         self._since_order = 0  # deposits since the particles were last put in cell order
+        # "bricked" mode: deposits since the last binning, binning cadence without reorders,
+        # and a request to bin at the next deposit (set when a reorder follows it)
+        self._since_bin = 0
+        self._bin_every = self.reorder_every if self.reorder_every > 0 else 4
+        self._bin_next = False
         self.warm = False
         self.last_cycles: list[int] = []
 
@@ -83,6 +92,17 @@ class PMChain:
     def deposit(self) -> None:
         p = self.parts
         self.plan.set_mode(self.deposit_mode)
+        if self.deposit_mode == "bricked":
+            # bin again when the drift after this deposit moves the particles into the
+            # binning's order, or when the binning has aged without such a move
+            due = self._bin_next or self._since_bin + 1 >= self._bin_every
+            # the grid's rho is this chain's own: only the chain's deposits write it
+            self.plan.deposit_step(p.pos_x, p.pos_y, p.pos_z, p.mass, p.n, self.dx,
+                                   self.grid.rho, bin=due, rho_kept=True)
+            self._since_bin = 0 if due else self._since_bin + 1
+            self._bin_next = False
+            self.keys_valid = False
+            return
         if self.keys_valid:
             self.plan.deposit_keyed(p.pos_x, p.pos_y, p.pos_z, p.mass, p.n, self.dx,
                                     self.grid.rho, self.keys)
@@ -131,7 +151,10 @@ class PMChain:
         vel = (self._p(p.vel_x), self._p(p.vel_y), self._p(p.vel_z))
         nat.call("pm_kick_drift", *pos, *vel, *(nat.ptr(a) for a in self.acc), p.n, h, dt,
                  nat.stream())
+        bricked = self.deposit_mode == "bricked"
         for s in range(steps):
+            if bricked:
+                self._bin_next = self._reorder_due(s, steps)
             self.deposit()
             self.solve()
             listed = self.plan.brick_list()
@@ -150,21 +173,29 @@ class PMChain:
                          nat.ptr(self.g4), p.n, self.n, self.dx, h, h, dt,
                          nat.ptr(self.keys, torch.int32), nat.stream())
                 p.swap_shadow()
+                if bricked:
+                    self.plan.mark_ordered()
                 self._since_order = -1  # this step's deposit is the order's own
                 pos = (self._p(p.pos_x), self._p(p.pos_y), self._p(p.pos_z))
                 vel = (self._p(p.vel_x), self._p(p.vel_y), self._p(p.vel_z))
                 self.keys_valid = True
             elif s + 1 < steps:
-                # also writes the drifted particles' cell keys for the next deposit
+                # also writes the drifted particles' cell keys for the next sorted deposit
+                keys = None if bricked else nat.ptr(self.keys, torch.int32)
                 nat.call("pm_gather_kick_drift4", *pos, nat.ptr(self.g4), *vel, p.n, self.n,
-                         self.dx, h, h, dt, nat.ptr(self.keys, torch.int32), nat.stream())
-                self.keys_valid = True
+                         self.dx, h, h, dt, keys, nat.stream())
+                self.keys_valid = not bricked
             else:
                 nat.call("pm_gather_kick4", *pos, nat.ptr(self.g4),
                          *(nat.ptr(a) for a in self.acc), *vel, p.n, self.n, self.dx, h,
                          nat.stream())
             self.steps_done += 1
             self._since_order += 1
+
+    def _reorder_due(self, s: int, steps: int) -> bool:
+        """Whether step s of a run of ``steps`` moves the particles into cell order."""
+        return (s + 1 < steps and self.reorder_every > 0
+                and self._since_order + 1 >= self.reorder_every)
 
     def _reorder_order(self) -> int | None:
         """Device address of the particle order to move to in this step's drift, or None."""

@@ -1,9 +1,11 @@
 """The particle<->grid stages of the PM force chain on libpmchain (reference deposit.py).
 
 - ``deposit_density(parts, rho, dx, mode="sorted")``: CIC deposit of mass density
-  (M/dx^3), deterministic either way: "sorted" (cell-key sort + segmented reduction in a
-  defined float32 order, also yields the keys and particle order; the default here and in
-  the fused step) or "fixed" (64-bit fixed-point integer sums, order independent).
+  (M/dx^3), deterministic in every mode: "sorted" (cell-key sort + segmented reduction in a
+  defined float32 order, also yields the keys and particle order; the default here),
+  "fixed" (64-bit fixed-point integer sums with device-scope atomics, order independent) or
+  "bricked" (the "fixed" result bit for bit, summed per 8^3-cell brick in LDS; the fused
+  step's default, which reuses one binning over several steps).
   Reference deposit.py:177-182.
 - ``potential_to_accel(phi, ax, ay, az, dx)``: g = -grad(phi), central inside, one-sided on
   the faces. Reference deposit.py:204-207.
@@ -25,7 +27,8 @@ from galaxy_collision_amd import _native as nat
 from galaxy_collision_amd.data import as_tensor
 
 
-_MODES = {"sorted": nat.PM_DEPOSIT_SORTED, "fixed": nat.PM_DEPOSIT_FIXED}
+_MODES = {"sorted": nat.PM_DEPOSIT_SORTED, "fixed": nat.PM_DEPOSIT_FIXED,
+          "bricked": nat.PM_DEPOSIT_BRICKED}
 
 
 class DepositPlan:
@@ -66,6 +69,20 @@ class DepositPlan:
                  int(n), float(dx), nat.ptr(as_tensor(rho)), nat.ptr(keys, torch.int32),
                  nat.ptr(perm_out, torch.int32) if perm_out is not None else None,
                  nat.stream())
+
+    def deposit_step(self, pos_x, pos_y, pos_z, mass, n: int, dx: float, rho,
+                     bin: bool = False, rho_kept: bool = False) -> None:
+        """"bricked" mode: the deposit of one step of a run, reusing the last binning
+        unless ``bin`` (same result as ``deposit``, bit for bit). ``rho_kept``: ``rho`` is
+        the array of this plan's last deposit, unchanged since (its far-away zeros are kept)."""
+        flags = (nat.PM_STEP_BIN if bin else 0) | (nat.PM_STEP_RHO_KEPT if rho_kept else 0)
+        nat.call("pm_deposit_cic_step", self.handle, nat.ptr(as_tensor(pos_x)),
+                 nat.ptr(as_tensor(pos_y)), nat.ptr(as_tensor(pos_z)), nat.ptr(as_tensor(mass)),
+                 int(n), float(dx), nat.ptr(as_tensor(rho)), flags, nat.stream())
+
+    def mark_ordered(self) -> None:
+        """"bricked" mode: the particles were moved into ``order()`` of the last binning."""
+        nat.call("pm_deposit_plan_mark_ordered", self.handle)
 
     def brick_list(self) -> tuple[int, int, int] | None:
         """(list address, device count address, max count) of the last deposit's marked node

@@ -60,9 +60,18 @@ pm_status pm_deposit_plan_destroy(pm_deposit_plan* plan);
  * keys_out is available, perm_out is not. Non-finite contributions are dropped. The first
  * switch to PM_DEPOSIT_FIXED allocates n^3 64-bit accumulators and synchronizes. Measured at
  * 256^3 with 10M cell-ordered particles: 5.2 ms (the 64-bit device-scope atomics, ~0.4 G/s
- * across the XCDs) vs about 0.35 ms for the PM_DEPOSIT_SORTED deposit (the whole PM step
- * takes about 0.95 ms), which stays the default everywhere. */
-enum { PM_DEPOSIT_SORTED = 0, PM_DEPOSIT_FIXED = 1 };
+ * across the XCDs) vs about 0.35 ms for the PM_DEPOSIT_SORTED deposit.
+ * PM_DEPOSIT_BRICKED: the PM_DEPOSIT_FIXED result, bit for bit (same terms, same scale,
+ * exact integer sums), computed without device-scope atomics: the particles are binned by
+ * 8^3-cell bricks (a stable radix sort of brick keys), each unit of at most 4096 binned
+ * particles sums its brick's region (the brick's nodes plus one cell of slack) in LDS, and a
+ * combine pass adds the units' region sums per node. pm_deposit_cic bins on every call;
+ * pm_deposit_cic_step reuses the last binning, which stays exact however far the particles
+ * have moved since (a particle outside its unit's region takes a global-atomic path). The
+ * first switch to it allocates the binning tables, about 10.6 KB per possible unit
+ * (max_particles/4096 + the number of cell bricks) and n^3 64-bit sums, and synchronizes.
+ * keys_out and perm_out are not available in this mode. */
+enum { PM_DEPOSIT_SORTED = 0, PM_DEPOSIT_FIXED = 1, PM_DEPOSIT_BRICKED = 2 };
 pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode);
 
 /* keys_out (optional, n_particles uint32): extended cell key per particle in input order,
@@ -101,6 +110,28 @@ pm_status pm_deposit_cic_noalloc(int32_t grid_size, const float* pos_x, const fl
                                  float dx, float* rho, uint32_t* keys_out, int32_t* perm_out,
                                  hipStream_t stream);
 
+/* The deposit of one step of a particle run (PM_DEPOSIT_BRICKED plans only): rho as
+ * pm_deposit_cic gives it, bit for bit, reusing the plan's last binning unless
+ * flags & PM_STEP_BIN (or the plan has no binning for n_particles particles at this dx),
+ * which bins again first; then pm_deposit_plan_order gives the new binning's particle order.
+ * The masses must be those of the last binning (their maximum sets the fixed-point scale).
+ * flags & PM_STEP_RHO_KEPT: rho is the array of this plan's last deposit, unchanged since;
+ * the node bricks far from every particle then keep their zeros instead of being stored
+ * again (rho is still written whole after a new binning, and around particles that were
+ * outside their unit's region in this or the last deposit).
+ * Replaces the per-step deposit call (reference deposit.py:47-82 via sim.py:318-325). */
+enum { PM_STEP_BIN = 1, PM_STEP_RHO_KEPT = 2 };
+pm_status pm_deposit_cic_step(pm_deposit_plan* plan, const float* pos_x, const float* pos_y,
+                              const float* pos_z, const float* mass, int64_t n_particles,
+                              float dx, float* rho, int32_t flags, hipStream_t stream);
+
+/* PM_DEPOSIT_BRICKED: the caller has moved the particles into the order of the plan's last
+ * binning (pm_deposit_plan_order, e.g. by pm_gather_kick_drift4_reorder), so slot i now
+ * holds binned position i: later pm_deposit_cic_step calls read the particles in place.
+ * Part of the same per-step deposit (reference deposit.py:47-82); no counterpart of its
+ * own. */
+pm_status pm_deposit_plan_mark_ordered(pm_deposit_plan* plan);
+
 /* Timing hook: per-phase device times (ms) of the last pm_deposit_cic on this plan when
  * pm_deposit_plan_set_timing(plan, 1) was set before it; phases = key, sort, reduce, assemble. */
 pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable);
@@ -108,13 +139,17 @@ pm_status pm_deposit_plan_set_timing(pm_deposit_plan* plan, int32_t enable);
 /* The node bricks of pm_deposit_plan_node_bricks that are marked, compacted: list[0..*count)
  * (device memory; *count is a device value, at most *max_count = the number of bricks), in
  * no particular order. Owned by the plan, valid until its next deposit; NULL after a
- * fixed-mode deposit. */
+ * fixed-mode deposit. In PM_DEPOSIT_BRICKED mode: the node bricks within one brick of a
+ * binned brick (every node brick when a particle was outside its unit's region), which
+ * hold every CIC corner node of the deposited particles. */
 pm_status pm_deposit_plan_brick_list(pm_deposit_plan* plan, const int32_t** list,
                                      const uint32_t** count, int64_t* max_count);
 
 /* Particle order of the last pm_deposit_cic / pm_deposit_cic_keyed with this plan: order[i]
  * is the particle at sorted position i (the perm_out array), or NULL after a fixed-mode or
- * empty deposit. Device memory owned by the plan, valid until its next deposit. */
+ * empty deposit. Device memory owned by the plan, valid until its next deposit. In
+ * PM_DEPOSIT_BRICKED mode: the binning's order (particles stably sorted by cell brick) when
+ * the last deposit binned, else NULL. */
 pm_status pm_deposit_plan_order(pm_deposit_plan* plan, const int32_t** order);
 
 /* Node bricks of the last deposit with this plan: bricks[b] != 0 when some deposited
