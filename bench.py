@@ -54,12 +54,12 @@ REFERENCE_STAGE_MS = {"deposit": 2.54, "solve": 27.8, "grad": 1.17, "gather": 5.
 WORKLOADS = {
     "collision10m": dict(kind="two_galaxy", n_per=5_000_000, grid=256, box=40.0, dt=0.01,
                          solver="multigrid_mp", warm="extrapolate", tol=2e-6, sweeps=(1, 1),
-                         reorder_every=4),
+                         deposit="bricked", reorder_every=8),
     "collision2m": dict(kind="two_galaxy", n_per=1_048_576, grid=128, box=40.0, dt=0.01,
                         solver="multigrid"),
     "collision100m": dict(kind="two_galaxy", n_per=50_000_000, grid=256, box=40.0, dt=0.01,
                           solver="multigrid_mp", warm="extrapolate", masses=(1.0, 0.5),
-                          tol=2e-6, sweeps=(1, 1), reorder_every=8),
+                          tol=2e-6, sweeps=(1, 1), deposit="bricked", reorder_every=8),
     "plummer": dict(kind="plummer", n=131_072, grid=64, box=20.0, dt=0.01, solver="multigrid"),
 }
 

@@ -55,7 +55,7 @@ constexpr int kUnit = 8192;              // binned positions per unit (at most)
 constexpr int kF64Bits = 13 + 62 - 53;
 static_assert(kUnit == 1 << 13, "kF64Bits assumes 2^13 positions per unit");
 constexpr unsigned kPersistent = 2048;   // workgroups of the persistent passes
-constexpr int kUnitGroups = 1024;        // workgroups of the unit pass (4 per CU stay resident)
+constexpr int kUnitGroups = 768;         // workgroups of the unit pass (3 per CU stay resident)
 static_assert(kCB == 8 && pm::kNodeBrick == 8, "bricks of 8 cells / 8 nodes per axis");
 
 struct Unit {
@@ -450,12 +450,60 @@ __device__ __forceinline__ int corner_off(int c) {
   return ((c >> 2) * kR + ((c >> 1) & 1)) * kR + (c & 1);
 }
 
+// Adds the 8 corner sums of the lanes with `flushing` set to their cell `cur`. Lanes walk
+// the binned positions side by side, so where a wave crosses from one cell to the next many
+// lanes flush the same cell at once: up to two such groups of 16 or more lanes are summed
+// over the wave first (one add per row and corner), the rest add per lane. Every lane of
+// the wave must call this.
+template <typename T>
+__device__ __forceinline__ void flush_sums(T* __restrict__ acc, int cur, const T (&sum)[8],
+                                           bool flushing, int lane) {
+  uint64_t fl = __ballot(flushing);
+#pragma unroll 1
+  for (int it = 0; it < 2 && __popcll(fl) >= 16; ++it) {
+    const int ka = __builtin_amdgcn_readlane(cur, __builtin_ctzll(fl));
+    const int kz = __builtin_amdgcn_readlane(cur, 63 - __builtin_clzll(fl));
+    const uint64_t ma = __ballot(cur == ka) & fl, mz = __ballot(cur == kz) & fl;
+    const int k0 = __popcll(mz) > __popcll(ma) ? kz : ka;
+    const uint64_t same = __popcll(mz) > __popcll(ma) ? mz : ma;
+    if (__popcll(same) < 16) break;
+    const bool me = (same >> lane) & 1ull;
+    T w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = me ? sum[c] : T(0);
+    T z = group_sums(w, lane);
+    z = z + dpp64<kDppRor8>(z);
+    if ((lane & 8) == 0) atomicAdd(&acc[k0 + corner_off(lane & 7)], z);
+    fl &= ~same;
+  }
+  if ((fl >> lane) & 1ull) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) atomicAdd(&acc[cur + corner_off(c)], sum[c]);
+  }
+}
+
+// A particle outside its unit's region: exact global adds of its 8 terms, and the node
+// bricks it touched flagged for the combine pass. Rare, so kept out of the unit pass's body.
+__device__ __noinline__ void add_outside(const unsigned long long (&q)[8], int ix, int iy, int iz,
+                                         int n, int nbn, unsigned long long* __restrict__ esc,
+                                         uint8_t* __restrict__ eflag,
+                                         uint32_t* __restrict__ esc_flag) {
+  const int64_t n2 = static_cast<int64_t>(n) * n;
+  for (int c = 0; c < 8; ++c) {
+    const int jx = ix + (c >> 2), jy = iy + ((c >> 1) & 1), jz = iz + (c & 1);
+    if (!q[c] || jx < 0 || jx >= n || jy < 0 || jy >= n || jz < 0 || jz >= n) continue;
+    atomicAdd(&esc[jx * n2 + static_cast<int64_t>(jy) * n + jz], q[c]);
+    eflag[((jx >> 3) * nbn + (jy >> 3)) * static_cast<int64_t>(nbn) + (jz >> 3)] = 1;
+    *esc_flag = 1u;
+  }
+}
+
 // Each lane walks kRun binned positions 64 apart (wave w's positions of a round are
 // w*kWavePos + 64*i + lane, so every load is coalesced; cell-ordered particles keep a lane
 // in one cell for many steps where cells are dense, with the few that drifted into a
-// neighbouring cell since the binning in between), keeps the current cell's 8 corner sums
-// in registers and adds them to the LDS sums when the cell changes and at the end of the
-// round (summed over the wave first when the whole wave ends in one cell).
+// neighbouring cell since the binning in between) and keeps the 8 corner sums of up to
+// three cells in registers; they go to the LDS sums at the end of the round, those that
+// many lanes share summed over the wave first (flush_sums).
 constexpr int kRun = 16;                  // positions per lane and round
 constexpr int kWavePos = 64 * kRun;       // positions per wave and round
 constexpr int kRoundPos = 4 * kWavePos;   // positions per workgroup and round
@@ -506,13 +554,17 @@ __global__ void __launch_bounds__(256) unit_deposit_kernel(
     };
     for (int base = 0; base < un.count; base += kRoundPos) {
       const int p0 = base + wave * kWavePos + lane;
-      int cur = -1;  // cell (region index) whose sums `sum` holds
-      T sum[8];
+      // three cells (region indices, -1 when free) with their 8 corner sums: the lane's
+      // cell, and the cells of particles that drifted into a neighbouring cell or lie
+      // past a cell boundary; a fourth cell takes slot 2 after adding its sums
+      int k0 = -1, k1 = -1, k2 = -1;
+      T s0[8], s1[8], s2[8];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) sum[c] = T(0);
+      for (int c = 0; c < 8; ++c) s0[c] = s1[c] = s2[c] = T(0);
       float4 grp[kGroup];
       load(p0, grp);
-#pragma unroll
+      // (not unrolled: the unrolled body of all kRun particles outgrew the instruction cache)
+#pragma unroll 1
       for (int g = 0; g < kRun; g += kGroup) {
         float4 nxt[kGroup];
         if (g + kGroup < kRun) load(p0 + 64 * (g + kGroup), nxt);
@@ -540,30 +592,31 @@ __global__ void __launch_bounds__(256) unit_deposit_kernel(
           for (int c = 0; c < 8; ++c) v[c] = S::term(bxy[c >> 1] * ((c & 1) ? fz : gz0), s);
           if (reg) {
             const int key = (rx * kR + ry) * kR + rz;
-            if (key != cur) {
-              if (cur >= 0) {
+            if (key == k0 || k0 < 0) {
+              k0 = key;
 #pragma unroll
-                for (int c = 0; c < 8; ++c) atomicAdd(&acc[cur + corner_off(c)], sum[c]);
-              }
-              cur = key;
+              for (int c = 0; c < 8; ++c) s0[c] = s0[c] + v[c];
+            } else if (key == k1 || k1 < 0) {
+              k1 = key;
 #pragma unroll
-              for (int c = 0; c < 8; ++c) sum[c] = v[c];
+              for (int c = 0; c < 8; ++c) s1[c] = s1[c] + v[c];
+            } else if (key == k2 || k2 < 0) {
+              k2 = key;
+#pragma unroll
+              for (int c = 0; c < 8; ++c) s2[c] = s2[c] + v[c];
             } else {
 #pragma unroll
-              for (int c = 0; c < 8; ++c) sum[c] = sum[c] + v[c];
+              for (int c = 0; c < 8; ++c) {
+                atomicAdd(&acc[k2 + corner_off(c)], s2[c]);
+                s2[c] = v[c];
+              }
+              k2 = key;
             }
           } else {
-            // outside the unit's region: exact global adds, node bricks flagged for the
-            // combine pass
+            unsigned long long q[8];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-              const int jx = ix + (c >> 2), jy = iy + ((c >> 1) & 1), jz = iz + (c & 1);
-              const unsigned long long qv = S::u64(v[c]);
-              if (!qv || jx < 0 || jx >= n || jy < 0 || jy >= n || jz < 0 || jz >= n) continue;
-              atomicAdd(&esc[jx * n2 + static_cast<int64_t>(jy) * n + jz], qv);
-              eflag[((jx >> 3) * nbn + (jy >> 3)) * static_cast<int64_t>(nbn) + (jz >> 3)] = 1;
-              esc_any[parity] = 1u;
-            }
+            for (int c = 0; c < 8; ++c) q[c] = S::u64(v[c]);
+            add_outside(q, ix, iy, iz, n, nbn, esc, eflag, esc_any + parity);
           }
         }
         if (g + kGroup < kRun) {
@@ -571,17 +624,9 @@ __global__ void __launch_bounds__(256) unit_deposit_kernel(
           for (int i = 0; i < kGroup; ++i) grp[i] = nxt[i];
         }
       }
-      // end of the round: where the whole wave holds one cell (a dense cell), its sums
-      // meet over the wave first, one add per row and corner
-      const int cur0 = __builtin_amdgcn_readfirstlane(cur);
-      if (cur0 >= 0 && __ballot(cur == cur0) == ~0ull) {
-        T z = group_sums(sum, lane);
-        z = z + dpp64<kDppRor8>(z);
-        if ((lane & 8) == 0) atomicAdd(&acc[cur0 + corner_off(lane & 7)], z);
-      } else if (cur >= 0) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) atomicAdd(&acc[cur + corner_off(c)], sum[c]);
-      }
+      flush_sums(acc, k0, s0, k0 >= 0, lane);
+      flush_sums(acc, k1, s1, k1 >= 0, lane);
+      flush_sums(acc, k2, s2, k2 >= 0, lane);
     }
     __syncthreads();
     if (home) {

@@ -2,11 +2,11 @@
 
 - collision at 256^3 (2 x 1,048,576 particles, box 40): the deposit bitwise against the oracle,
   the mixed-precision solve within 1e-5 of the exact float64 solution of the same discrete
-  problem (DST-I), and a fused multi-step run with particle reordering whose last deposit is
-  bitwise the oracle's and whose phi meets the same 1e-5;
+  problem (DST-I), and fused multi-step runs with particle reordering (sorted and bricked
+  deposits) whose last deposit is bitwise the oracle's and whose phi meets the same 1e-5;
 - the learned warm start with the packaged res-64 network on a 256^3 step sequence: every
   solve within 1e-5 of DST-I;
-- 100M particles (2 x 50M, 2:1 mass ratio) on 256^3: mass on the grid equals the in-box CIC
+- 100M particles (2 x 50M, 2:1 mass ratio) on 256^3, both deposits: mass on the grid equals the in-box CIC
   mass computed independently in float64, the state stays finite, momentum drifts by less
   than 1e-4 of the momentum scale, and phi meets 1e-5;
 - two chains on one grid size keep working when the second one is larger.
@@ -96,16 +96,20 @@ def test_collision256_deposit_and_mixed_solve(device, collision2m):
     torch.cuda.empty_cache()
 
 
-def test_collision256_fused_run_with_reordering(device, collision2m):
+@pytest.mark.parametrize("mode,reorder", [("sorted", 4), ("bricked", 8)])
+def test_collision256_fused_run_with_reordering(device, collision2m, mode, reorder):
     """Nine fused steps with the benchmark's settings (particles moved into cell order every
-    4 steps): the last deposit equals the oracle's deposit of the particles in their final
-    slots bit for bit, phi meets 1e-5, and every particle is still there (by load index)."""
-    announce("collision 256^3 / 2M: fused 9-step run vs oracle")
+    4 steps with the sorted deposit, into brick order every 8 with the bricked one, which
+    reuses one binning until then): the last deposit equals the oracle's deposit of the
+    particles in their final slots bit for bit (oracle_deposit_cic / oracle_deposit_fixed),
+    phi meets 1e-5, and every particle is still there (by load index)."""
+    announce(f"collision 256^3 / 2M: fused 9-step run vs oracle, {mode} deposit")
     from galaxy_collision_amd.chain import PMChain
 
     icr = collision2m
     parts = _state(icr)
-    chain = PMChain(parts, N_GRID, DX, _production_solver(), reorder_every=4)
+    chain = PMChain(parts, N_GRID, DX, _production_solver(), reorder_every=reorder,
+                    deposit_mode=mode)
     chain.prime()
     chain.run(9, DT)
     torch.cuda.synchronize()
@@ -116,7 +120,10 @@ def test_collision256_fused_run_with_reordering(device, collision2m):
     assert_bitwise(mass, icr.mass[pid], "mass follows pid")
     assert_bitwise(parts.gid.to_numpy(), icr.gid[pid], "gid follows pid")
     pos = parts.positions()
-    rho_ref, _, _ = O.deposit_cic(pos, mass, N_GRID, DX)
+    if mode == "sorted":
+        rho_ref, _, _ = O.deposit_cic(pos, mass, N_GRID, DX)
+    else:
+        rho_ref, _ = O.deposit_cic_fixed(pos, mass, N_GRID, DX)
     rho = chain.grid.rho.to_numpy()
     assert_bitwise(rho, rho_ref, "last deposit")
     rel = _rel_to_exact(rho, chain.grid.phi.to_numpy())
@@ -180,8 +187,9 @@ def _momentum(parts) -> tuple[np.ndarray, float]:
     return p, float((m * speed).sum())
 
 
-def test_collision100m_properties(device):
-    announce("collision 100M on 256^3: property run")
+@pytest.mark.parametrize("mode", ["sorted", "bricked"])
+def test_collision100m_properties(device, mode):
+    announce(f"collision 100M on 256^3: property run, {mode} deposit")
     from galaxy_collision_amd import ic
     from galaxy_collision_amd.chain import PMChain
     from galaxy_collision_amd.data import ParticleState
@@ -196,7 +204,8 @@ def test_collision100m_properties(device):
     del tic
     torch.cuda.empty_cache()
     p0, scale = _momentum(parts)
-    chain = PMChain(parts, N_GRID, DX, _production_solver(), reorder_every=8)
+    chain = PMChain(parts, N_GRID, DX, _production_solver(), reorder_every=8,
+                    deposit_mode=mode)
     chain.prime()
     mass_in = _in_box_cic_mass(parts, N_GRID, DX)
     on_grid = float(chain.grid.rho.tensor.double().sum()) * DX ** 3
