@@ -99,7 +99,10 @@ _PROFILE_KERNELS = {"mg.rb_pass": "rb_pass_kernel", "gather_kick": "gather_kick4
                     "mg.rbs": "rbs4_kernel<false>", "mg.residual": "residual_kernel",
                     "mg.prolong": "prolong_add_kernel", "deposit.piece": "piece_kernel",
                     "deposit.key": "key_kernel", "deposit.radix_hist": "radix_hist_kernel",
-                    "deposit.radix_scatter": "radix_scatter_kernel"}
+                    "deposit.radix_scatter": "radix_scatter_kernel",
+                    "deposit.unit_sums": "unit_deposit_kernel<true, false>",
+                    "deposit.combine": "brick_combine_kernel",
+                    "deposit.brick_reduce": "brick_reduce_kernel"}
 
 
 def measured_traffic(workload: str, grid: int, timer_name: str):
