@@ -14,25 +14,29 @@
 // region adds one cell of slack on each side: nodes [8b-2, 8b+8], 11 per axis, 1331
 // 64-bit sums.
 //
-// Binning (when asked for, or when there is none): key = cell brick (or a sentinel past
-// the last brick for particles with no corner in the box), a stable radix sort of the keys,
-// brick bounds in the sorted order, and a unit table: each brick's positions cut into units
-// of at most kUnit, then the positions of the sentinel particles as units of no brick.
+// Binning (when asked for, or when there is none): key = (cell brick, cell in the brick)
+// (or a sentinel past the last brick for particles with no corner in the box), a stable
+// radix sort of the keys (so a brick's particles are in cell order), brick bounds in the
+// sorted order, and a unit table: each brick's positions cut into units of at most kUnit,
+// then the positions of the sentinel particles as units of no brick.
 //
-// Every deposit (two launches):
-//   unit pass: one workgroup per unit zeroes the 1331 LDS sums, reads its particles (through
-//     the binning's order, or in place once the caller has moved the particles into that
-//     order), adds the 8 corner q of every particle whose cell lies in the region with LDS
-//     atomics (a wave whose 64 particles share one cell sums them with shuffles first and
-//     adds once), and stores the region's sums for its unit. A particle outside the
+// Every deposit (three launches):
+//   unit pass: each workgroup takes a balanced range of units; per unit it zeroes the 1331
+//     LDS sums, reads the unit's particles (through the binning's order, or in place once
+//     the caller has moved the particles into that order) with each lane walking a run of
+//     positions 64 apart and keeping the corner q of up to three cells in registers, adds
+//     them to the LDS sums at the end of a round (those many lanes share summed over the
+//     wave first), and stores the region's sums for its unit. A particle outside the
 //     region (it moved more than one cell out of its brick since the binning, or has no
 //     brick) adds its q to a zeroed n^3 64-bit array with global atomics and flags the node
 //     bricks it touched: exact, only slower, so the result never depends on how fresh the
 //     binning is;
+//   brick reduce: the region sums of a brick with several units are added into its first;
 //   combine pass: every 8^3 node brick sums, per node, the unit sums of the cell bricks whose
 //     region holds the node (the brick itself; the one below for the first layer; the one
 //     above for the last two), adds and clears the flagged 64-bit array entries, converts,
-//     and stores rho; node bricks with no unit nearby store zeros.
+//     and stores rho; node bricks with no unit nearby store zeros (or keep the zeros they
+//     hold when rho is the last deposit's array).
 #include <algorithm>
 #include <climits>
 
@@ -47,8 +51,10 @@
 namespace {
 
 constexpr int kCB = 8;                   // extended cells per brick edge
+constexpr int kCellBits = 9;             // bits of a cell's index within its brick
 constexpr int kR = kCB + 1 + 2;          // region nodes per axis (one cell of slack a side)
 constexpr int kRN = kR * kR * kR;        // 1331 sums per region
+constexpr int kRegion = kRN + 1;         // stride of the stored regions (16-byte aligned)
 constexpr int kUnit = 8192;              // binned positions per unit (at most)
 // With N >= 2^(kF64Bits-1) particles a unit's node sums stay below kUnit * 2^(62-b) <= 2^53
 // (b = bit length of N), so float64 LDS sums of the integer terms are exact.
@@ -71,10 +77,29 @@ __device__ __forceinline__ void decompose(uint32_t b, uint32_t nb, int& x, int& 
   x = static_cast<int>(b / (nb * nb));
 }
 
+// Layout of a unit's 11^3 region sums in memory: per axis the region coordinate r falls in
+// class 0 (r < 2: nodes 8b-2, 8b-1, read by the node brick below), 1 (2 <= r < 10: the
+// brick's own 8 node layers) or 2 (r = 10: node 8b+8, read by the node brick above); the
+// 27 class blocks are stored one after another, each contiguous, so the node brick that
+// reads a block reads whole cache lines of it.
+__host__ __device__ constexpr int cls_lo(int c) { return c == 0 ? 0 : (c == 1 ? 2 : 10); }
+__host__ __device__ constexpr int cls_n(int c) { return c == 0 ? 2 : (c == 1 ? 8 : 1); }
+__host__ __device__ constexpr int cls_of(int r) { return r < 2 ? 0 : (r < 10 ? 1 : 2); }
+__host__ __device__ constexpr int block_off(int cx, int cy, int cz) {
+  return kR * kR * cls_lo(cx) + kR * cls_n(cx) * cls_lo(cy) + cls_n(cx) * cls_n(cy) * cls_lo(cz);
+}
+__device__ __forceinline__ int region_slot(int rx, int ry, int rz) {
+  const int cx = cls_of(rx), cy = cls_of(ry), cz = cls_of(rz);
+  return block_off(cx, cy, cz) +
+         ((rx - cls_lo(cx)) * cls_n(cy) + (ry - cls_lo(cy))) * cls_n(cz) + (rz - cls_lo(cz));
+}
+static_assert(block_off(2, 2, 2) + 1 == kRN, "class blocks tile the region");
+
 // ---------------------------------------------------------------------------------------
 // binning
 // ---------------------------------------------------------------------------------------
-// Cell brick of every particle (grid-stride), and this workgroup's largest finite positive
+// Binning key of every particle, (cell brick << kCellBits) | cell in the brick (grid-stride),
+// and this workgroup's largest finite positive
 // m' = mass*inv_vol as float bits in maxparts[blockIdx.x] (positive floats order like their
 // bits), folded by unit_build_kernel.
 __global__ void __launch_bounds__(256) brick_key_kernel(
@@ -92,7 +117,10 @@ __global__ void __launch_bounds__(256) brick_key_kernel(
                     pm::cic_axis(z, dx, n, iz, fz);
     const uint32_t bx = static_cast<uint32_t>(ix + 1) >> 3, by = static_cast<uint32_t>(iy + 1) >> 3,
                    bz = static_cast<uint32_t>(iz + 1) >> 3;
-    keys[p] = in ? (bx * nbc + by) * nbc + bz : sentinel;
+    const uint32_t cell = ((static_cast<uint32_t>(ix + 1) & 7u) << 6) |
+                          ((static_cast<uint32_t>(iy + 1) & 7u) << 3) |
+                          (static_cast<uint32_t>(iz + 1) & 7u);
+    keys[p] = in ? (((bx * nbc + by) * nbc + bz) << kCellBits) | cell : sentinel;
     const float v = m * inv_vol;
     const unsigned int bits = (v > 0.0f && v < __builtin_huge_valf()) ? __float_as_uint(v) : 0u;
     mx = bits > mx ? bits : mx;
@@ -119,9 +147,9 @@ __global__ void __launch_bounds__(256) brick_bounds_kernel(const uint32_t* __res
                                                            int32_t* __restrict__ end) {
   const int64_t j = pm::xcd_block() * blockDim.x + threadIdx.x;
   if (j >= np) return;
-  const uint32_t b = skeys[j];
-  const uint32_t prev = j > 0 ? skeys[j - 1] : 0xFFFFFFFFu;
-  const uint32_t next = j + 1 < np ? skeys[j + 1] : 0xFFFFFFFFu;
+  const uint32_t b = skeys[j] >> kCellBits;
+  const uint32_t prev = j > 0 ? skeys[j - 1] >> kCellBits : 0xFFFFFFFFu;
+  const uint32_t next = j + 1 < np ? skeys[j + 1] >> kCellBits : 0xFFFFFFFFu;
   if (b >= ncb) return;
   if (prev != b) first[b] = static_cast<int32_t>(j);
   if (next != b) end[b] = static_cast<int32_t>(j + 1);
@@ -149,8 +177,9 @@ __device__ __forceinline__ uint32_t scan_1024(uint32_t v, uint32_t& total, uint3
   return before + incl - v;
 }
 
-// Per-thread span of the one-workgroup table passes: 32 consecutive entries read as eight
-// 16-byte loads issued together (the tables are padded to whole rounds of 1024 spans).
+// Entries per thread and round of the one-workgroup table passes: thread t takes entries
+// round + 1024*q + t (q < kSpan), so every load and store of a wave is coalesced; all of a
+// thread's loads are issued together (the tables are padded to whole rounds).
 constexpr int kSpan = 32;
 constexpr int64_t kRound = int64_t(1024) * kSpan;
 inline int64_t padded(int64_t count) { return (count + kRound - 1) / kRound * kRound; }
@@ -190,44 +219,40 @@ __global__ void __launch_bounds__(1024) unit_build_kernel(int32_t* __restrict__ 
   }
   uint32_t units_before = 0, binned = 0, multi_before = 0;  // totals of the earlier rounds
   for (uint32_t round = 0; round < ncb; round += static_cast<uint32_t>(kRound)) {
-    const uint32_t b0 = round + threadIdx.x * kSpan;
-    int4 f4[kSpan / 4], e4[kSpan / 4];
-#pragma unroll
-    for (int q = 0; q < kSpan / 4; ++q) {
-      f4[q] = reinterpret_cast<const int4*>(first + b0)[q];
-      e4[q] = reinterpret_cast<const int4*>(end + b0)[q];
-    }
-    int32_t f[kSpan], c[kSpan];
-#pragma unroll
-    for (int q = 0; q < kSpan / 4; ++q) {
-      f[4 * q] = f4[q].x, f[4 * q + 1] = f4[q].y, f[4 * q + 2] = f4[q].z, f[4 * q + 3] = f4[q].w;
-      c[4 * q] = e4[q].x - f4[q].x, c[4 * q + 1] = e4[q].y - f4[q].y;
-      c[4 * q + 2] = e4[q].z - f4[q].z, c[4 * q + 3] = e4[q].w - f4[q].w;
-    }
+    const uint32_t b0 = round + threadIdx.x;
     uint32_t nu = 0, nc = 0, nm = 0;
-#pragma unroll
+#pragma unroll 8
     for (int q = 0; q < kSpan; ++q) {
-      const uint32_t k = (static_cast<uint32_t>(c[q]) + unit - 1) / unit;
+      const uint32_t cnt = static_cast<uint32_t>(end[b0 + 1024u * q] - first[b0 + 1024u * q]);
+      const uint32_t k = (cnt + unit - 1) / unit;
       nu += k;
-      nc += static_cast<uint32_t>(c[q]);
+      nc += cnt;
       nm += k >= 2 ? 1u : 0u;
     }
+    // a thread's bricks take consecutive units (any order of the bricks serves)
     uint32_t round_units, round_binned, round_multi;
     uint32_t base = units_before + scan_1024(nu, round_units, s_wave);
     (void)scan_1024(nc, round_binned, s_wave);
     uint32_t mbase = multi_before + scan_1024(nm, round_multi, s_wave);
-#pragma unroll
+    // the next brick's bounds are loaded while this one's units are written
+    int32_t fn = first[b0], en = end[b0];
+#pragma unroll 1
     for (int q = 0; q < kSpan; ++q) {
-      const uint32_t b = b0 + q;
+      const uint32_t b = b0 + 1024u * q;
       if (b >= ncb) break;
-      const uint32_t cnt = static_cast<uint32_t>(c[q]);
+      const int32_t f = fn;
+      const uint32_t cnt = static_cast<uint32_t>(en - fn);
+      if (q + 1 < kSpan) {
+        fn = first[b + 1024u];
+        en = end[b + 1024u];
+      }
       const uint32_t k = (cnt + unit - 1) / unit;
       unit_first[b] = static_cast<int32_t>(base);
       unit_num[b] = static_cast<int32_t>(k);
       for (uint32_t w = 0; w < k; ++w) {
         Unit un;
         un.brick = static_cast<int32_t>(b);
-        un.begin = f[q] + static_cast<int32_t>(w * unit);
+        un.begin = f + static_cast<int32_t>(w * unit);
         un.count = static_cast<int32_t>(min(static_cast<uint32_t>(unit), cnt - w * unit));
         un.units = w == 0 ? static_cast<int32_t>(k) : 0;
         units[base + w] = un;
@@ -294,19 +319,17 @@ __global__ void __launch_bounds__(1024) node_list_kernel(const uint32_t* __restr
   if (threadIdx.x == 0) s_act = s_idle = 0u;
   __syncthreads();
   for (int64_t round = 0; round < nnb; round += kRound) {
-    const int64_t lo = round + int64_t(threadIdx.x) * kSpan;
-    uint4 m4[kSpan / 4];
+    const int64_t lo = round + threadIdx.x;
+    uint32_t m[kSpan];
 #pragma unroll
-    for (int q = 0; q < kSpan / 4; ++q) m4[q] = reinterpret_cast<const uint4*>(node_src + lo)[q];
+    for (int q = 0; q < kSpan; ++q) m[q] = node_src[lo + 1024 * q];
     uint32_t bits = 0, act = 0, idle = 0;
 #pragma unroll
     for (int q = 0; q < kSpan; ++q) {
-      const uint4 w = m4[q / 4];
-      const uint32_t v = (q & 3) == 0 ? w.x : (q & 3) == 1 ? w.y : (q & 3) == 2 ? w.z : w.w;
-      const bool valid = lo + q < nnb;
-      bits |= (valid && v) ? (1u << q) : 0u;
-      act += (valid && v) ? 1u : 0u;
-      idle += (valid && !v) ? 1u : 0u;
+      const bool valid = lo + 1024 * q < nnb;
+      bits |= (valid && m[q]) ? (1u << q) : 0u;
+      act += (valid && m[q]) ? 1u : 0u;
+      idle += (valid && !m[q]) ? 1u : 0u;
     }
     uint32_t ia = act, ii = idle;
 #pragma unroll
@@ -323,9 +346,10 @@ __global__ void __launch_bounds__(1024) node_list_kernel(const uint32_t* __restr
       wi = atomicAdd(&s_idle, ii);
     }
     uint32_t pa = __shfl(wa, 63, 64) + ia - act, pi = __shfl(wi, 63, 64) + ii - idle;
+#pragma unroll
     for (int q = 0; q < kSpan; ++q) {
-      const int64_t B = lo + q;
-      if (B >= nnb) break;
+      const int64_t B = lo + 1024 * q;
+      if (B >= nnb) continue;
       if ((bits >> q) & 1u) {
         list[pa++] = static_cast<int32_t>(B);
       } else {
@@ -378,9 +402,11 @@ struct Sum;
 template <>
 struct Sum<true> {
   using T = double;
-  // t * 2^s is exact in float32 (a power-of-two scaling that stays far from overflow and
-  // from the subnormals), so rounding it there gives llrint(ldexp((double)t, s))
-  static __device__ __forceinline__ T term(float t, int s) {
+  // T: a term already scaled by 2^s (exactly, see unit_deposit_kernel): round it there
+  static __device__ __forceinline__ T folded(float t) { return static_cast<double>(rintf(t)); }
+  // t * 2^s is exact in float32 for s > 0 (a power-of-two scaling that stays below 2^62),
+  // so rounding it there gives llrint(ldexp((double)t, s))
+  static __device__ __forceinline__ T scaled(float t, int s) {
     return static_cast<double>(rintf(__builtin_amdgcn_ldexpf(t, s)));
   }
   static __device__ __forceinline__ unsigned long long u64(T v) {
@@ -390,7 +416,10 @@ struct Sum<true> {
 template <>
 struct Sum<false> {
   using T = unsigned long long;
-  static __device__ __forceinline__ T term(float t, int s) { return pm::fixed_q(t, s); }
+  static __device__ __forceinline__ T folded(float t) {
+    return static_cast<unsigned long long>(rintf(t));
+  }
+  static __device__ __forceinline__ T scaled(float t, int s) { return pm::fixed_q(t, s); }
   static __device__ __forceinline__ unsigned long long u64(T v) { return v; }
 };
 
@@ -498,36 +527,109 @@ __device__ __noinline__ void add_outside(const unsigned long long (&q)[8], int i
   }
 }
 
-// Each lane walks kRun binned positions 64 apart (wave w's positions of a round are
-// w*kWavePos + 64*i + lane, so every load is coalesced; cell-ordered particles keep a lane
-// in one cell for many steps where cells are dense, with the few that drifted into a
-// neighbouring cell since the binning in between) and keeps the 8 corner sums of up to
-// three cells in registers; they go to the LDS sums at the end of the round, those that
-// many lanes share summed over the wave first (flush_sums).
-constexpr int kRun = 16;                  // positions per lane and round
-constexpr int kWavePos = 64 * kRun;       // positions per wave and round
-constexpr int kRoundPos = 4 * kWavePos;   // positions per workgroup and round
-constexpr int kGroup = 4;                 // particles per lane whose loads are in flight
+// g = x / dx, the IEEE quotient, with the reciprocal of dx refined once per thread: the
+// steps of the compiler's float32 division (reciprocal, one Newton step, quotient and two
+// remainder corrections by fma) without its operand range scaling, which never applies
+// when dx lies in [2^-30, 2^30] and 2^-60 <= |x| < 2^60 (no operand, remainder or quotient
+// near the under- or overflow range), so the result is the IEEE quotient there. Larger |x|
+// give |g| >= 2^29, infinity or NaN, outside the grid like the IEEE quotient; smaller |x|
+// (and dx outside that range) take the division itself (`slow`).
+__device__ __forceinline__ float quot(float x, float dx, float r1) {
+  const float q0 = x * r1;
+  const float e1 = fmaf(-dx, q0, x);
+  const float q1 = fmaf(e1, r1, q0);
+  const float e2 = fmaf(-dx, q1, x);
+  return fmaf(e2, r1, q1);
+}
 
-// IDX: the particles are read through the binning's order idx (else in place).
-// Workgroup w takes units [wg_first[w], wg_first[w+1]) (unit_assign_kernel).
-template <bool F64, bool IDX>
-__global__ void __launch_bounds__(256) unit_deposit_kernel(
-    const Unit* __restrict__ units, const int32_t* __restrict__ wg_first,
-    const int32_t* __restrict__ idx, const float* __restrict__ px, const float* __restrict__ py,
-    const float* __restrict__ pz, const float* __restrict__ mass, int n, float dx,
-    float inv_vol, int nbc, int nbn, const unsigned int* __restrict__ maxbits, int bl,
+// Buffer resource over `count` 4-byte values at p (binned positions stay below 2^30, so
+// byte offsets fit 32 bits): loads through it take one 32-bit offset register and return 0
+// past the end instead of reading beyond it. Dword 3 is the gfx9 raw-buffer format.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, int64_t count) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), static_cast<short>(0),
+                                           static_cast<int>(static_cast<uint32_t>(count * 4)),
+                                           0x00020000);
+}
+__device__ __forceinline__ uint32_t ldb(__amdgpu_buffer_rsrc_t r, uint32_t index) {
+  return static_cast<uint32_t>(__builtin_amdgcn_raw_buffer_load_b32(r, index * 4u, 0, 0));
+}
+__device__ __forceinline__ float ldbf(__amdgpu_buffer_rsrc_t r, uint32_t index) {
+  return __uint_as_float(ldb(r, index));
+}
+
+// A wave takes its positions a phase of kPhasePos at a time: loaded coalesced (lane l reads
+// positions l, l+64, ... of the phase), stored to the wave's LDS stage, and read back so
+// that lane l takes kPhaseRun consecutive positions, which in a brick's cell order keeps a
+// lane in one cell wherever cells hold more than a few particles. A lane keeps the 8 corner
+// sums of up to three cells in registers over a round of kPhases phases; they go to the LDS
+// sums at the end of the round, and a slot taken over by a fourth cell goes there at once,
+// those that many lanes share summed over the wave first (flush_sums).
+constexpr int kPhaseRun = 4;                   // consecutive positions per lane and phase
+constexpr int kPhasePos = 64 * kPhaseRun;      // positions per wave and phase
+constexpr int kPhases = 4;                     // phases per round
+constexpr int kWavePos = kPhases * kPhasePos;  // positions per wave and round
+constexpr int kRoundPos = 4 * kWavePos;        // positions per workgroup and round
+constexpr int kGroup = 4;                      // positions per lane processed together
+// Stage layout: position p of a phase at p + p/32 (one pad word per 32), so the 32 lanes of
+// each half-wave hit 32 distinct banks both when storing (lanes l + 64j) and when reading
+// their runs (lanes 4l + i).
+static_assert(kPhaseRun == 4, "the stage padding is laid out for runs of 4");
+constexpr int kStageStride = kPhasePos + kPhasePos / 32;  // floats per array and wave
+__device__ __forceinline__ int stage_slot(int p) { return p + (p >> 5); }
+constexpr int kFoldMax = 64;              // largest scale exponent folded into m'
+
+// The unit loop of unit_deposit_kernel. FOLD (s <= kFoldMax): the scale 2^s is applied to
+// m' once, M = m' * 2^s, and the terms ((M*wx)*wy)*wz are rounded directly. For s <= 64
+// this gives the rounded ((m'*wx)*wy)*wz * 2^s of the definition bit for bit: scaling by a
+// power of two commutes with every rounding while no value of either product chain is
+// below 2^-126, and once one is, both chains' terms stay below 2^(s-126) <= 2^-62 (or below
+// 2^-126 for the scaled chain when s < 0) and round to 0.
+template <bool F64, bool IDX, bool FOLD>
+__device__ __forceinline__ void unit_loop(
+    typename Sum<F64>::T* __restrict__ acc, float* __restrict__ stage,
+    const Unit* __restrict__ units, int32_t u_begin,
+    int32_t u_end, const int32_t* __restrict__ idx, const float* __restrict__ px,
+    const float* __restrict__ py, const float* __restrict__ pz, const float* __restrict__ mass,
+    int64_t np, int n, float dx, float inv_vol, int nbc, int nbn, int s,
     unsigned long long* __restrict__ partial, unsigned long long* __restrict__ esc,
-    uint8_t* __restrict__ eflag, uint32_t* __restrict__ esc_any, int parity) {
+    uint8_t* __restrict__ eflag, uint32_t* __restrict__ esc_flag) {
   using S = Sum<F64>;
   using T = typename S::T;
-  __shared__ T acc[kRN];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int s = pm::fixed_scale(*maxbits, bl);
-  const int64_t n2 = static_cast<int64_t>(n) * n;
-  const int32_t u_end = wg_first[blockIdx.x + 1];
-  for (int32_t u = wg_first[blockIdx.x]; u < u_end; ++u) {
-    const Unit un = units[u];
+  const float nf = static_cast<float>(n);
+  const float r0 = __builtin_amdgcn_rcpf(dx);
+  const float r1 = fmaf(fmaf(-dx, r0, 1.0f), r0, r0);
+  const bool fast = dx >= 0x1p-30f && dx <= 0x1p30f;
+  const float scale = FOLD ? __builtin_amdgcn_ldexpf(1.0f, s) : 1.0f;  // 2^s, normal for s >= -97
+  // a phase's particles of unit U as the coalesced loads see them (positions past the
+  // unit's end read its last particle, unused)
+  const __amdgpu_buffer_rsrc_t bx = rsrc(px, np), by = rsrc(py, np), bz = rsrc(pz, np),
+                               bm = rsrc(mass, np), bi = rsrc(idx, IDX ? np : 0);
+  auto load = [&](const Unit& U, int off0, float4 (&p)[kPhaseRun]) {
+    uint32_t sl[kPhaseRun];
+#pragma unroll
+    for (int j = 0; j < kPhaseRun; ++j) {
+      const uint32_t q = static_cast<uint32_t>(U.begin + min(off0 + 64 * j, U.count - 1));
+      sl[j] = IDX ? ldb(bi, q) : q;
+    }
+#pragma unroll
+    for (int j = 0; j < kPhaseRun; ++j)
+      p[j] = make_float4(ldbf(bx, sl[j]), ldbf(by, sl[j]), ldbf(bz, sl[j]), ldbf(bm, sl[j]));
+  };
+  float* const sx = stage + wave * 4 * kStageStride;
+  float* const sy = sx + kStageStride;
+  float* const sz = sy + kStageStride;
+  float* const sm = sz + kStageStride;
+  // a unit's table entry is loaded one unit ahead, and a wave's first phase of a unit while
+  // it finishes the unit before, so neither load's latency adds to the unit's time
+  Unit un_next = u_begin < u_end ? units[u_begin] : Unit{};
+  float4 ph[kPhaseRun];
+  if (u_begin < u_end) load(un_next, wave * kWavePos + lane, ph);
+  for (int32_t u = u_begin; u < u_end; ++u) {
+    const Unit un = un_next;
+    const bool more = u + 1 < u_end;
+    if (more) un_next = units[u + 1];
+    bool pref = false;  // ph holds the next unit's first phase
     const bool home = un.brick >= 0;
     int ox = INT_MIN / 2, oy = INT_MIN / 2, oz = INT_MIN / 2;  // region origin (nodes)
     if (home) {
@@ -539,109 +641,174 @@ __global__ void __launch_bounds__(256) unit_deposit_kernel(
     }
     for (int q = tid; q < kRN; q += blockDim.x) acc[q] = T(0);
     __syncthreads();
-    // a group's particles (positions past the unit's end read its last particle, unused);
-    // the next group's loads are issued before this group's work
-    auto load = [&](int off0, float4 (&p)[kGroup]) {
-      int32_t sl[kGroup];
-#pragma unroll
-      for (int i = 0; i < kGroup; ++i) {
-        const int32_t j = un.begin + min(off0 + 64 * i, un.count - 1);
-        sl[i] = IDX ? idx[j] : j;
-      }
-#pragma unroll
-      for (int i = 0; i < kGroup; ++i)
-        p[i] = make_float4(px[sl[i]], py[sl[i]], pz[sl[i]], mass[sl[i]]);
-    };
     for (int base = 0; base < un.count; base += kRoundPos) {
-      const int p0 = base + wave * kWavePos + lane;
+      if (base + wave * kWavePos >= un.count) continue;  // nothing of this round for the wave
       // three cells (region indices, -1 when free) with their 8 corner sums: the lane's
-      // cell, and the cells of particles that drifted into a neighbouring cell or lie
-      // past a cell boundary; a fourth cell takes slot 2 after adding its sums
+      // cell, and the cells past a cell boundary of its runs; a fourth cell takes slot 2
+      // after adding its sums
       int k0 = -1, k1 = -1, k2 = -1;
       T s0[8], s1[8], s2[8];
 #pragma unroll
       for (int c = 0; c < 8; ++c) s0[c] = s1[c] = s2[c] = T(0);
-      float4 grp[kGroup];
-      load(p0, grp);
-      // (not unrolled: the unrolled body of all kRun particles outgrew the instruction cache)
+      // (not unrolled: the unrolled body of all kPhases phases outgrew the instruction cache)
 #pragma unroll 1
-      for (int g = 0; g < kRun; g += kGroup) {
-        float4 nxt[kGroup];
-        if (g + kGroup < kRun) load(p0 + 64 * (g + kGroup), nxt);
+      for (int h = 0; h < kPhases; ++h) {
+        const int w0 = base + wave * kWavePos + h * kPhasePos;  // the phase's first offset
+        if (w0 >= un.count) break;
+        // the stage is free: the wave's reads of the last phase were issued before these
+        // writes, and LDS operations of one wave complete in order
 #pragma unroll
-        for (int i = 0; i < kGroup; ++i) {
-          const bool valid = p0 + 64 * (g + i) < un.count;
-          int ix = 0, iy = 0, iz = 0;
-          float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-          const bool in = valid && pm::cic_axis(grp[i].x, dx, n, ix, fx) &&
-                          pm::cic_axis(grp[i].y, dx, n, iy, fy) &&
-                          pm::cic_axis(grp[i].z, dx, n, iz, fz);
-          const float mp = grp[i].w * inv_vol;
-          // m' <= 0 or not finite: every term is dropped (t <= 0 or not finite)
-          const bool good = in && mp > 0.0f && mp < __builtin_huge_valf();
-          const int rx = ix - ox, ry = iy - oy, rz = iz - oz;
-          const bool reg = good && home && static_cast<unsigned>(rx) <= 9u &&
-                           static_cast<unsigned>(ry) <= 9u && static_cast<unsigned>(rz) <= 9u;
-          if (!good) continue;
-          // t_c = ((m'*wx)*wy)*wz, the products shared between corners
-          const float gx0 = 1.0f - fx, gy0 = 1.0f - fy, gz0 = 1.0f - fz;
-          const float a0 = mp * gx0, a1 = mp * fx;
-          const float bxy[4] = {a0 * gy0, a0 * fy, a1 * gy0, a1 * fy};
-          T v[8];
+        for (int j = 0; j < kPhaseRun; ++j) {
+          const int t = stage_slot(64 * j + lane);
+          sx[t] = ph[j].x;
+          sy[t] = ph[j].y;
+          sz[t] = ph[j].z;
+          sm[t] = ph[j].w;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the next phase's loads (this round's next phase, the next round's first, or the
+        // next unit's first) are issued before this phase's work
+        const int next = h + 1 < kPhases ? w0 + kPhasePos : base + kRoundPos + wave * kWavePos;
+        if (next < un.count) {
+          load(un, next + lane, ph);
+        } else if (more && !pref) {
+          load(un_next, wave * kWavePos + lane, ph);
+          pref = true;
+        }
+#pragma unroll 1
+        for (int g = 0; g < kPhaseRun; g += kGroup) {
+          float4 grp[kGroup];
 #pragma unroll
-          for (int c = 0; c < 8; ++c) v[c] = S::term(bxy[c >> 1] * ((c & 1) ? fz : gz0), s);
-          if (reg) {
+          for (int i = 0; i < kGroup; ++i) {
+            const int t = stage_slot(kPhaseRun * lane + g + i);
+            grp[i] = make_float4(sx[t], sy[t], sz[t], sm[t]);
+          }
+#pragma unroll
+          for (int i = 0; i < kGroup; ++i) {
+            const bool valid = w0 + kPhaseRun * lane + g + i < un.count;
+            const float x = grp[i].x, y = grp[i].y, z = grp[i].z;
+            float gx = quot(x, dx, r1), gy = quot(y, dx, r1), gz = quot(z, dx, r1);
+            const bool slow = !fast || !(fabsf(x) >= 0x1p-60f) || !(fabsf(y) >= 0x1p-60f) ||
+                              !(fabsf(z) >= 0x1p-60f);
+            if (slow) {
+              gx = x / dx;
+              gy = y / dx;
+              gz = z / dx;
+            }
+            const bool in = valid && gx >= -1.0f && gx < nf && gy >= -1.0f && gy < nf &&
+                            gz >= -1.0f && gz < nf;
+            const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+            const int ix = in ? static_cast<int>(flx) : 0, iy = in ? static_cast<int>(fly) : 0,
+                      iz = in ? static_cast<int>(flz) : 0;
+            const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
+            const float mp = grp[i].w * inv_vol;
+            // m' <= 0 or not finite: every term is dropped (t <= 0 or not finite)
+            const bool good = in && mp > 0.0f && mp < __builtin_huge_valf();
+            const int rx = ix - ox, ry = iy - oy, rz = iz - oz;
+            const bool reg = good && home && static_cast<unsigned>(rx) <= 9u &&
+                             static_cast<unsigned>(ry) <= 9u && static_cast<unsigned>(rz) <= 9u;
+            const float gx0 = 1.0f - fx, gy0 = 1.0f - fy, gz0 = 1.0f - fz;
+            if (good && !reg) {
+              // t_c = ((m'*wx)*wy)*wz, the products shared between corners
+              const float a0 = mp * gx0, a1 = mp * fx;
+              const float bxy[4] = {a0 * gy0, a0 * fy, a1 * gy0, a1 * fy};
+              unsigned long long q[8];
+#pragma unroll
+              for (int c = 0; c < 8; ++c) q[c] = pm::fixed_q(bxy[c >> 1] * ((c & 1) ? fz : gz0), s);
+              add_outside(q, ix, iy, iz, n, nbn, esc, eflag, esc_flag);
+            }
+            // slot of the particle's cell: the cell of slot 0 or 1 (or a free one of them), else
+            // slot 2, whose cell, when another, first goes to the LDS sums; the lanes doing so
+            // at one step mostly share that cell, so it is summed over the wave (every lane
+            // takes part in flush_sums)
             const int key = (rx * kR + ry) * kR + rz;
-            if (key == k0 || k0 < 0) {
-              k0 = key;
+            const bool h0 = reg && (key == k0 || k0 < 0);
+            const bool h1 = reg && !h0 && (key == k1 || k1 < 0);
+            const bool h2 = reg && !h0 && !h1;
+            const bool evict = h2 && k2 >= 0 && key != k2;
+            if (__builtin_amdgcn_ballot_w64(evict) != 0ull) {
+              flush_sums(acc, k2, s2, evict, lane);
+              if (evict) {
 #pragma unroll
-              for (int c = 0; c < 8; ++c) s0[c] = s0[c] + v[c];
-            } else if (key == k1 || k1 < 0) {
-              k1 = key;
-#pragma unroll
-              for (int c = 0; c < 8; ++c) s1[c] = s1[c] + v[c];
-            } else if (key == k2 || k2 < 0) {
-              k2 = key;
-#pragma unroll
-              for (int c = 0; c < 8; ++c) s2[c] = s2[c] + v[c];
-            } else {
+                for (int c = 0; c < 8; ++c) s2[c] = T(0);
+              }
+            }
+            k0 = h0 ? key : k0;
+            k1 = h1 ? key : k1;
+            k2 = h2 ? key : k2;
+            if (reg) {
+              const float m2 = FOLD ? mp * scale : mp;
+              const float a0 = m2 * gx0, a1 = m2 * fx;
+              const float bxy[4] = {a0 * gy0, a0 * fy, a1 * gy0, a1 * fy};
+              T v[8];
 #pragma unroll
               for (int c = 0; c < 8; ++c) {
-                atomicAdd(&acc[k2 + corner_off(c)], s2[c]);
-                s2[c] = v[c];
+                const float t = bxy[c >> 1] * ((c & 1) ? fz : gz0);
+                v[c] = FOLD ? S::folded(t) : S::scaled(t, s);
               }
-              k2 = key;
-            }
-          } else {
-            unsigned long long q[8];
+              if (h0) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) q[c] = S::u64(v[c]);
-            add_outside(q, ix, iy, iz, n, nbn, esc, eflag, esc_any + parity);
+                for (int c = 0; c < 8; ++c) s0[c] = s0[c] + v[c];
+              }
+              if (h1) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) s1[c] = s1[c] + v[c];
+              }
+              if (h2) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) s2[c] = s2[c] + v[c];
+              }
+            }
           }
         }
-        if (g + kGroup < kRun) {
-#pragma unroll
-          for (int i = 0; i < kGroup; ++i) grp[i] = nxt[i];
-        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
       flush_sums(acc, k0, s0, k0 >= 0, lane);
       flush_sums(acc, k1, s1, k1 >= 0, lane);
       flush_sums(acc, k2, s2, k2 >= 0, lane);
     }
+    if (more && !pref) load(un_next, wave * kWavePos + lane, ph);
     __syncthreads();
     if (home) {
-      unsigned long long* dst = partial + static_cast<int64_t>(u) * kRN;
-      for (int q = tid; q < kRN; q += blockDim.x) dst[q] = S::u64(acc[q]);
+      unsigned long long* dst = partial + static_cast<int64_t>(u) * kRegion;
+      for (int q = tid; q < kRN; q += blockDim.x)
+        dst[region_slot(q / (kR * kR), (q / kR) % kR, q % kR)] = S::u64(acc[q]);
     }
     __syncthreads();  // the next unit zeroes acc
   }
 }
 
+// IDX: the particles are read through the binning's order idx (else in place).
+// Workgroup w takes units [wg_first[w], wg_first[w+1]) (unit_assign_kernel).
+template <bool F64, bool IDX>
+__global__ void __launch_bounds__(256, 3) unit_deposit_kernel(
+    const Unit* __restrict__ units, const int32_t* __restrict__ wg_first,
+    const int32_t* __restrict__ idx, const float* __restrict__ px, const float* __restrict__ py,
+    const float* __restrict__ pz, const float* __restrict__ mass, int64_t np, int n, float dx,
+    float inv_vol, int nbc, int nbn, const unsigned int* __restrict__ maxbits, int bl,
+    unsigned long long* __restrict__ partial, unsigned long long* __restrict__ esc,
+    uint8_t* __restrict__ eflag, uint32_t* __restrict__ esc_any, int parity) {
+  __shared__ typename Sum<F64>::T acc[kRN];
+  __shared__ float stage[4 * 4 * kStageStride];
+  const int s = pm::fixed_scale(*maxbits, bl);
+  const int32_t u_begin = wg_first[blockIdx.x], u_end = wg_first[blockIdx.x + 1];
+  if (s <= kFoldMax)
+    unit_loop<F64, IDX, true>(acc, stage, units, u_begin, u_end, idx, px, py, pz, mass, np, n,
+                              dx, inv_vol, nbc, nbn, s, partial, esc, eflag, esc_any + parity);
+  else
+    unit_loop<F64, IDX, false>(acc, stage, units, u_begin, u_end, idx, px, py, pz, mass, np, n,
+                               dx, inv_vol, nbc, nbn, s, partial, esc, eflag, esc_any + parity);
+}
+
 // One workgroup: splits the units into `groups` consecutive ranges of about equal cost
-// (positions plus kUnitCost per unit, for its setup and its stored region), so the unit
-// pass's workgroups finish together: wg_first[w] = the first unit of range w,
-// wg_first[groups] = the number of units.
-constexpr int kUnitCost = 1024;
+// (positions plus kUnitCost per unit, for its setup, its cleared and stored region and its
+// partly filled rounds), so the unit pass's workgroups finish together: wg_first[w] = the
+// first unit of range w, wg_first[groups] = the number of units. kUnitCost: a unit costs
+// about as much time as 1,750 positions (per-workgroup timings at 10M particles / 256^3).
+constexpr int kUnitCost = 1750;
 
 __global__ void __launch_bounds__(1024) unit_assign_kernel(const Unit* __restrict__ units,
                                                            const uint32_t* __restrict__ unit_count,
@@ -699,14 +866,14 @@ __global__ void __launch_bounds__(256) brick_reduce_kernel(const Unit* __restric
     const int k = units[u0].units;
     unsigned long long acc = 0ull;
     if (r < kRN) {
-      const unsigned long long* p = partial + static_cast<int64_t>(u0) * kRN + r;
+      const unsigned long long* p = partial + static_cast<int64_t>(u0) * kRegion + r;
 #pragma unroll 1
       for (int q = wave; q < k; q += 4 * kCombineBatchR) {
         unsigned long long v[kCombineBatchR];
 #pragma unroll
         for (int w = 0; w < kCombineBatchR; ++w) {
           const int qq = q + 4 * w;
-          v[w] = qq < k ? p[static_cast<int64_t>(qq) * kRN] : 0ull;
+          v[w] = qq < k ? p[static_cast<int64_t>(qq) * kRegion] : 0ull;
         }
 #pragma unroll
         for (int w = 0; w < kCombineBatchR; ++w) acc += v[w];
@@ -715,7 +882,7 @@ __global__ void __launch_bounds__(256) brick_reduce_kernel(const Unit* __restric
     s_sum[wave][lane] = acc;
     __syncthreads();
     if (wave == 0 && r < kRN)
-      partial[static_cast<int64_t>(u0) * kRN + r] = s_sum[0][lane] + s_sum[1][lane] +
+      partial[static_cast<int64_t>(u0) * kRegion + r] = s_sum[0][lane] + s_sum[1][lane] +
                                                      s_sum[2][lane] + s_sum[3][lane];
     __syncthreads();
   }
@@ -726,82 +893,101 @@ __global__ void __launch_bounds__(256) brick_reduce_kernel(const Unit* __restric
 // particles outside their regions (esc_any[parity], dirty[parity]), whose sums may lie
 // there; otherwise they still hold the zeros stored before. The flags for the next deposit
 // go to the other parity, which no workgroup of this launch reads.
+//
+// One wave per node brick B; lane (a, b) sums the node row (a, b, 0..7). Node brick B reads
+// from the region of cell brick B + e (e in {-1, 0, 1}^3) the class block 1 - e: its own
+// core, one node layer of the brick below (class 2 = node 8B) and two of the brick above
+// (class 0 = nodes 8B+6, 8B+7) per axis, each block contiguous; all of a lane's loads are
+// independent, so a wave has them in flight together. vec_rho: rho rows of 8 nodes are
+// 16-byte aligned (n a multiple of 4, rho aligned).
 __global__ void __launch_bounds__(256) brick_combine_kernel(
     const int32_t* __restrict__ node_list, uint32_t nnb, const int2* __restrict__ desc,
     const unsigned long long* __restrict__ partial, unsigned long long* __restrict__ esc,
     uint8_t* __restrict__ eflag, uint32_t* __restrict__ esc_any, uint32_t* __restrict__ dirty,
     int parity, bool full, uint32_t* __restrict__ node_count,
     const unsigned int* __restrict__ maxbits, int bl, int n, int nbn,
-    float* __restrict__ rho) {
-  __shared__ int2 s_desc[27];
+    float* __restrict__ rho, bool vec_rho) {
   const int s = pm::fixed_scale(*maxbits, bl);
-  const int tid = threadIdx.x;
-  // lane: node row (a, bb) of the brick and nodes 2h, 2h+1 of the row
-  const int a = tid >> 5, bb = (tid >> 2) & 7, h = tid & 3;
+  const int lane = threadIdx.x & 63;
+  const int a = lane >> 3, b = lane & 7;
+  const uint32_t wave0 =
+      static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const uint32_t waves = gridDim.x * 4;
   const int64_t n2 = static_cast<int64_t>(n) * n;
   const bool outside = esc_any[parity] != 0u;
   const uint32_t active = node_count[1];
   const uint32_t limit = (full || outside || dirty[parity]) ? nnb : active;
-  for (uint32_t e = blockIdx.x; e < limit; e += gridDim.x) {
+  for (uint32_t e = wave0; e < limit; e += waves) {
     const uint32_t B = static_cast<uint32_t>(node_list[e]);
-    const bool sourced = e < active;
-    if (sourced && tid < 27) s_desc[tid] = desc[static_cast<int64_t>(e) * 27 + tid];
     const bool flagged = eflag[B] != 0;
     int bi, bj, bk;
     decompose(B, static_cast<uint32_t>(nbn), bi, bj, bk);
-    const int i = kCB * bi + a, j = kCB * bj + bb, k = kCB * bk + 2 * h;
-    __syncthreads();
-    // a node takes sums from the brick itself and, on the brick's first or last layers
-    // along an axis, from the neighbour below or above: at most 8 sources, whose loads all
-    // leave before any is added
-    unsigned long long sum0 = 0ull, sum1 = 0ull;
-    if (sourced) {
-      const int ddx = a == 0 ? -1 : (a >= 6 ? 1 : 0);
-      const int ddy = bb == 0 ? -1 : (bb >= 6 ? 1 : 0);
-      unsigned long long v[2][8];
+    const int i = kCB * bi + a, j = kCB * bj + b, k0 = kCB * bk;
+    unsigned long long sum[kCB];
 #pragma unroll
-      for (int o = 0; o < 2; ++o) {
-        const int l = 2 * h + o;
-        const int ddz = l == 0 ? -1 : (l >= 6 ? 1 : 0);
+    for (int c = 0; c < kCB; ++c) sum[c] = 0ull;
+    if (e < active) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int ex = (c >> 2) ? ddx : 0, ey = ((c >> 1) & 1) ? ddy : 0, ez = (c & 1) ? ddz : 0;
-          // the secondary source exists only where the neighbour's region reaches
-          const bool used = ((c >> 2) == 0 || ddx != 0) && (((c >> 1) & 1) == 0 || ddy != 0) &&
-                            ((c & 1) == 0 || ddz != 0);
-          const int2 dd = s_desc[(ex + 1) * 9 + (ey + 1) * 3 + (ez + 1)];
-          const int r = ((a + 2 - kCB * ex) * kR + (bb + 2 - kCB * ey)) * kR + (l + 2 - kCB * ez);
-          v[o][c] = (used && dd.y > 0) ? partial[static_cast<int64_t>(dd.x) * kRN + r] : 0ull;
+      for (int d = 0; d < 27; ++d) {
+        const int ex = d / 9 - 1, ey = (d / 3) % 3 - 1, ez = d % 3 - 1;
+        const int2 dd = desc[static_cast<int64_t>(e) * 27 + d];
+        // the lane's row lies in this block when its a (b) is a layer the block covers
+        const bool row = (ex == 0 || (ex < 0 ? a == 0 : a >= 6)) &&
+                         (ey == 0 || (ey < 0 ? b == 0 : b >= 6));
+        if (dd.y > 0 && row) {
+          const int ny = cls_n(1 - ey), nz = cls_n(1 - ez);
+          const int ox = ex > 0 ? a - 6 : (ex == 0 ? a : 0);
+          const int oy = ey > 0 ? b - 6 : (ey == 0 ? b : 0);
+          // rows of 8 and of 2 start at even entries of 16-byte aligned regions
+          const unsigned long long* src = partial + static_cast<int64_t>(dd.x) * kRegion +
+                                          block_off(1 - ex, 1 - ey, 1 - ez) + (ox * ny + oy) * nz;
+          // nodes c of the row in this block: all 8 (ez = 0), c = 0 (ez = -1), c = 6, 7
+          if (ez == 0) {
+#pragma unroll
+            for (int c = 0; c < kCB; c += 2) {
+              const ulonglong2 w = *reinterpret_cast<const ulonglong2*>(src + c);
+              sum[c] += w.x;
+              sum[c + 1] += w.y;
+            }
+          } else if (ez < 0) {
+            sum[0] += src[0];
+          } else {
+            const ulonglong2 w = *reinterpret_cast<const ulonglong2*>(src);
+            sum[6] += w.x;
+            sum[7] += w.y;
+          }
         }
-      }
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        sum0 += v[0][c];
-        sum1 += v[1][c];
       }
     }
     const bool in_ij = i < n && j < n;
-    unsigned long long sum[2] = {sum0, sum1};
-    if (flagged) {
+    if (flagged && in_ij) {
 #pragma unroll
-      for (int o = 0; o < 2; ++o) {
-        if (in_ij && k + o < n) {
-          const int64_t node = i * n2 + static_cast<int64_t>(j) * n + (k + o);
-          sum[o] += esc[node];
-          esc[node] = 0ull;
+      for (int c = 0; c < kCB; ++c) {
+        if (k0 + c < n) {
+          const int64_t nd = i * n2 + static_cast<int64_t>(j) * n + (k0 + c);
+          sum[c] += esc[nd];
+          esc[nd] = 0ull;
         }
       }
     }
     if (in_ij) {
-      float* dst = rho + i * n2 + static_cast<int64_t>(j) * n + k;
+      float* dst = rho + i * n2 + static_cast<int64_t>(j) * n + k0;
+      float r[kCB];
 #pragma unroll
-      for (int o = 0; o < 2; ++o)
-        if (k + o < n) dst[o] = static_cast<float>(ldexp(static_cast<double>(sum[o]), -s));
+      for (int c = 0; c < kCB; ++c)
+        r[c] = static_cast<float>(ldexp(static_cast<double>(sum[c]), -s));
+      if (vec_rho && k0 + kCB <= n) {
+        reinterpret_cast<float4*>(dst)[0] = make_float4(r[0], r[1], r[2], r[3]);
+        reinterpret_cast<float4*>(dst)[1] = make_float4(r[4], r[5], r[6], r[7]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < kCB; ++c)
+          if (k0 + c < n) dst[c] = r[c];
+      }
     }
-    __syncthreads();  // s_desc and the flag are read by every lane
-    if (flagged && tid == 0) eflag[B] = 0;
+    if (flagged && lane == 0) eflag[B] = 0;  // every lane read it above
   }
-  if (blockIdx.x == 0 && tid == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     // the gradient covers every node brick when a particle deposited outside the regions
     node_count[0] = outside ? nnb : active;
     dirty[parity ^ 1] = outside ? 1u : 0u;
@@ -876,6 +1062,9 @@ void destroy(State* st) {
 
 pm_status create(int n, int64_t cap, State** out) {
   *out = nullptr;
+  // the unit pass addresses particles by 32-bit byte offsets
+  if (cap >= (int64_t(1) << 30))
+    return pm::invalid("PM_DEPOSIT_BRICKED: at most 2^30 - 1 particles per plan");
   auto* st = new State();
   st->n = n;
   st->cap = cap;
@@ -886,7 +1075,7 @@ pm_status create(int n, int64_t cap, State** out) {
   const int64_t full = (cap + kUnit - 1) / kUnit;
   st->max_home_units = full + std::min<int64_t>(st->ncb, cap);
   st->max_units = st->max_home_units + full + 1;
-  st->passes = pm::radix::plan_digits(static_cast<uint64_t>(st->ncb), st->bits);
+  st->passes = pm::radix::plan_digits(static_cast<uint64_t>(st->ncb) << kCellBits, st->bits);
   const int64_t nodes = static_cast<int64_t>(n) * n * n;
   pm_status e = PM_OK;
   // the one-workgroup table passes read whole rounds: first/end/node_src are padded
@@ -903,7 +1092,7 @@ pm_status create(int n, int64_t cap, State** out) {
       (e = alloc(&st->multi, st->max_units, "hipMalloc (brick units)")) ||
       (e = alloc(&st->multi_count, 1, "hipMalloc (brick units)")) ||
       (e = alloc(&st->wg_first, kUnitGroups + 1, "hipMalloc (unit ranges)")) ||
-      (e = alloc(&st->partial, st->max_home_units * kRN, "hipMalloc (unit sums)")) ||
+      (e = alloc(&st->partial, st->max_home_units * kRegion, "hipMalloc (unit sums)")) ||
       (e = alloc(&st->esc, nodes, "hipMalloc (outside sums)")) ||
       (e = alloc(&st->eflag, st->nnb, "hipMalloc (outside flags)")) ||
       (e = alloc(&st->esc_any, 2, "hipMalloc (outside flag)")) ||
@@ -956,7 +1145,7 @@ pm_status deposit(State* st, const SortBuffers& sb, const float* px, const float
       PM_TIMED("deposit.brick_key", 20.0 * np, s);
       brick_key_kernel<<<key_blocks, 256, 0, s>>>(
           px, py, pz, mass, np, n, dx, inv_vol, static_cast<uint32_t>(st->nbc),
-          static_cast<uint32_t>(st->ncb), sb.keys[0], st->maxbits + 1);
+          static_cast<uint32_t>(st->ncb) << kCellBits, sb.keys[0], st->maxbits + 1);
       PM_CHECK_LAUNCH("brick_key_kernel");
     }
     int src = 0;
@@ -1005,7 +1194,7 @@ pm_status deposit(State* st, const SortBuffers& sb, const float* px, const float
     const double bytes = (idx ? 20.0 : 16.0) * np;
 #define PM_UNIT_PASS(F64, IDX)                                                                 \
   PM_LAUNCH_TIMED("deposit.unit_sums", bytes, (unit_deposit_kernel<F64, IDX>), kUnitGroups, 256, \
-                  0, s, st->units, st->wg_first, idx, px, py, pz, mass, n, dx, inv_vol,        \
+                  0, s, st->units, st->wg_first, idx, px, py, pz, mass, np, n, dx, inv_vol,    \
                   st->nbc, st->nbn, st->maxbits, bl, st->partial, st->esc, st->eflag,          \
                   st->esc_any, st->parity)
     if (bl >= kF64Bits) {
@@ -1027,7 +1216,8 @@ pm_status deposit(State* st, const SortBuffers& sb, const float* px, const float
                     static_cast<unsigned>(std::min<int64_t>(st->nnb, kPersistent)), 256, 0, s,
                     st->node_list, static_cast<uint32_t>(st->nnb), st->desc, st->partial, st->esc,
                     st->eflag, st->esc_any, st->dirty, st->parity, full, st->node_count,
-                    st->maxbits, bl, n, st->nbn, rho);
+                    st->maxbits, bl, n, st->nbn, rho,
+                    (n & 3) == 0 && (reinterpret_cast<uintptr_t>(rho) & 15u) == 0u);
     PM_CHECK_LAUNCH("brick_combine_kernel");
   }
   st->parity ^= 1;

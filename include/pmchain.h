@@ -63,13 +63,13 @@ pm_status pm_deposit_plan_destroy(pm_deposit_plan* plan);
  * across the XCDs) vs about 0.35 ms for the PM_DEPOSIT_SORTED deposit.
  * PM_DEPOSIT_BRICKED: the PM_DEPOSIT_FIXED result, bit for bit (same terms, same scale,
  * exact integer sums), computed without device-scope atomics: the particles are binned by
- * 8^3-cell bricks (a stable radix sort of brick keys), each unit of at most 4096 binned
+ * 8^3-cell bricks (a stable radix sort of brick keys), each unit of at most 8192 binned
  * particles sums its brick's region (the brick's nodes plus one cell of slack) in LDS, and a
  * combine pass adds the units' region sums per node. pm_deposit_cic bins on every call;
  * pm_deposit_cic_step reuses the last binning, which stays exact however far the particles
  * have moved since (a particle outside its unit's region takes a global-atomic path). The
  * first switch to it allocates the binning tables, about 10.6 KB per possible unit
- * (max_particles/4096 + the number of cell bricks) and n^3 64-bit sums, and synchronizes.
+ * (max_particles/8192 + the number of cell bricks) and n^3 64-bit sums, and synchronizes.
  * keys_out and perm_out are not available in this mode. */
 enum { PM_DEPOSIT_SORTED = 0, PM_DEPOSIT_FIXED = 1, PM_DEPOSIT_BRICKED = 2 };
 pm_status pm_deposit_plan_set_mode(pm_deposit_plan* plan, int32_t mode);

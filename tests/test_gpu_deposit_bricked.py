@@ -9,8 +9,8 @@ of the same terms in tests/test_oracle.py):
 - reusing one binning while the particles move (pm_deposit_cic_step): within a cell of their
   brick, beyond it (the global-atomic path), out of and into the box;
 - after the particles are moved into the binning's order (pm_deposit_plan_mark_ordered);
-- the binning order is a stable argsort of the brick keys, and the node-brick list holds
-  every CIC corner node of the particles (all node bricks when one fell outside);
+- the binning order is a stable argsort of the (brick, cell) keys, and the node-brick list
+  holds every CIC corner node of the particles (all node bricks when one fell outside);
 - PMChain runs in "bricked" mode, with reordering, equal "fixed" mode runs bit for bit.
 """
 
@@ -66,16 +66,18 @@ def _check(device, pos, mass, n, dx):
     return rho
 
 
-def _brick_keys(pos, n, dx):
-    """Cell brick of each particle ((n+1)/8 bricks per axis of the extended cells), or the
-    sentinel past the last brick when no corner lies in the box."""
+def _bin_keys(pos, n, dx):
+    """Binning key of each particle: (cell brick << 9) | cell within the brick, over the
+    extended cells (ceil((n+1)/8) bricks per axis), or the sentinel (bricks << 9) when no
+    corner lies in the box."""
     m = n + 1
     nbc = (m + 7) // 8
     key = O.cell_keys(pos, n, dx).astype(np.int64)
     out = key == m ** 3
     ex, ey, ez = key // (m * m), (key // m) % m, key % m
     b = ((ex >> 3) * nbc + (ey >> 3)) * nbc + (ez >> 3)
-    return np.where(out, nbc ** 3, b)
+    cell = ((ex & 7) << 6) | ((ey & 7) << 3) | (ez & 7)
+    return np.where(out, nbc ** 3 << 9, (b << 9) | cell)
 
 
 def test_bricked_closed_forms_and_edges(device):
@@ -115,6 +117,67 @@ def test_bricked_random_bitwise(device, npart, n, dx, dense, sort):
         pos = pos[np.argsort(O.cell_keys(pos, n, dx), kind="stable")]
     mass = rng.uniform(0.5, 2.0, npart).astype(F32)
     _check(device, pos, mass, n, dx)
+
+
+@pytest.mark.parametrize("case", ["tiny_mass", "mass_range", "tiny_dx", "huge_dx",
+                                  "tiny_pos", "tiny_mass_f64"])
+def test_bricked_scales_and_quotient_ranges(device, case):
+    """The unit pass's shortcuts stay bitwise the oracle's arithmetic at the edges of their
+    ranges: fixed-point scales above 2^64 (terms scaled one by one) and far below (m' * 2^s
+    folded once, with terms that underflow), masses over 33 decades, dx outside [2^-30, 2^30]
+    and coordinates below 2^-60 (the IEEE division itself), zeros and signed zeros."""
+    announce(f"bricked deposit scale / quotient range case {case}")
+    rng = np.random.default_rng(7)
+    n, dx, npart = 32, 0.5, 20_000
+    if case == "tiny_mass_f64":
+        npart = 2_100_000  # float64 LDS sums
+    if case == "tiny_dx":
+        dx = 2.0 ** -34
+    elif case == "huge_dx":
+        dx = 2.0 ** 31
+    pos = rng.normal(n * dx / 2, n * dx / 5, size=(npart, 3)).astype(F32)
+    mass = rng.uniform(0.5, 2.0, npart).astype(F32)
+    if case in ("tiny_mass", "tiny_mass_f64"):
+        mass *= F32(1e-9)  # m' near 2^-27: scale 2^s with s > 64
+    elif case == "mass_range":
+        mass = (10.0 ** rng.uniform(-30, 3, npart)).astype(F32)
+    elif case == "tiny_pos":
+        k = npart // 4
+        pos[:k, 0] = rng.choice(np.array([0.0, -0.0, 1e-25, -1e-25, 3e-39, -3e-39, 1e-45],
+                                         F32), k)
+        pos[k:2 * k, 1] = rng.uniform(-1e-19, 1e-19, k).astype(F32)
+        pos[2 * k:2 * k + 50, 2] = np.array([1e30, -1e30, np.inf, -np.inf, 2.0 ** 60] * 10, F32)
+    _check(device, pos, mass, n, dx)
+
+
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_bricked_unaligned_arrays(device, shift):
+    """Particle arrays that start 4, 8 or 12 bytes past a 16-byte boundary (read one value
+    at a time instead of four) give the same bitwise result, binned anew and reused."""
+    announce(f"bricked deposit, arrays offset by {shift} floats")
+    rng = np.random.default_rng(11 + shift)
+    n, dx, npart = 40, 0.5, 50_001
+    pos = rng.normal(n * dx / 2, n * dx / 6, size=(npart, 3)).astype(F32)
+    pos = pos[np.argsort(O.cell_keys(pos, n, dx), kind="stable")]
+    mass = rng.uniform(0.5, 2.0, npart).astype(F32)
+    ref, _ = O.deposit_cic_fixed(pos, mass, n, dx)
+
+    def shifted(a):
+        t = torch.empty(npart + shift, dtype=torch.float32, device=device)
+        t[shift:] = torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        return t[shift:]
+
+    cols = [shifted(pos[:, a]) for a in range(3)] + [shifted(mass)]
+    assert all(c.data_ptr() % 16 == 4 * shift for c in cols)
+    plan = _plan(n, npart)
+    for step in (None, True, False):
+        rho = torch.full((n, n, n), 7.0, device=device)
+        if step is None:
+            plan.deposit(*cols, npart, dx, rho)
+        else:
+            plan.deposit_step(*cols, npart, dx, rho, bin=step)
+        assert_bitwise(rho.cpu().numpy(), ref)
+    plan.close()
 
 
 def test_bricked_step_reuses_binning_exactly(device):
@@ -175,8 +238,8 @@ def _read_u32(addr: int) -> int:
 
 
 def test_bricked_order_and_in_place_reads(device):
-    """The binning order is the stable argsort of the brick keys; once the particles are
-    moved into it (mark_ordered) later step deposits read them in place, still exact."""
+    """The binning order is the stable argsort of the (brick, cell) keys; once the particles
+    are moved into it (mark_ordered) later step deposits read them in place, still exact."""
     announce("bricked binning order / in-place reads")
     n, dx = 48, 0.5
     rng = np.random.default_rng(12)
@@ -196,7 +259,7 @@ def test_bricked_order_and_in_place_reads(device):
              nat.stream())
     torch.cuda.synchronize()
     perm = order.cpu().numpy()
-    assert_bitwise(perm, np.argsort(_brick_keys(pos, n, dx), kind="stable").astype(np.int32))
+    assert_bitwise(perm, np.argsort(_bin_keys(pos, n, dx), kind="stable").astype(np.int32))
     # move the particles into the binning's order, as the reordering drift does
     pos2, mass2 = pos[perm], mass[perm]
     parts2 = _parts(pos2, mass2)
