@@ -21,16 +21,17 @@
 // then the positions of the sentinel particles as units of no brick.
 //
 // Every deposit (three launches):
-//   unit pass: each workgroup takes a balanced range of units; per unit it zeroes the 1331
-//     LDS sums, reads the unit's particles (through the binning's order, or in place once
-//     the caller has moved the particles into that order) with each lane walking a run of
-//     positions 64 apart and keeping the corner q of up to three cells in registers, adds
+//   unit pass: workgroups claim units from a queue, the larger first; per unit a workgroup
+//     zeroes the 1331 LDS sums, reads the unit's particles (through the binning's order,
+//     or in place once the caller has moved the particles into that order), loaded
+//     coalesced and passed through LDS so that each lane takes runs of consecutive
+//     (cell-ordered) positions, keeps the corner q of up to two cells in registers, adds
 //     them to the LDS sums at the end of a round (those many lanes share summed over the
-//     wave first), and stores the region's sums for its unit. A particle outside the
-//     region (it moved more than one cell out of its brick since the binning, or has no
-//     brick) adds its q to a zeroed n^3 64-bit array with global atomics and flags the node
-//     bricks it touched: exact, only slower, so the result never depends on how fresh the
-//     binning is;
+//     wave first), and stores the region's sums for its unit. A particle outside the region
+//     (it moved more than one cell out of its brick since the binning, or has no brick)
+//     adds its q to a zeroed n^3 64-bit array with global atomics and flags the node bricks
+//     it touched: exact, only slower, so the result never depends on how fresh the binning
+//     is;
 //   brick reduce: the region sums of a brick with several units are added into its first;
 //   combine pass: every 8^3 node brick sums, per node, the unit sums of the cell bricks whose
 //     region holds the node (the brick itself; the one below for the first layer; the one
@@ -451,6 +452,19 @@ __device__ __forceinline__ T dpp64(T v) {
   return out;
 }
 
+// v from lane src (all lanes take part).
+template <typename T>
+__device__ __forceinline__ T shfl64(T v, int src) {
+  uint64_t bits;
+  __builtin_memcpy(&bits, &v, 8);
+  const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(bits), src, 64));
+  const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(bits >> 32), src, 64));
+  bits = (static_cast<uint64_t>(hi) << 32) | lo;
+  T out;
+  __builtin_memcpy(&out, &bits, 8);
+  return out;
+}
+
 // Sums of the 8 corner values over each group of 8 lanes, reduce-scattered: lane 8g + j
 // returns group g's sum for corner j (partners 7-j, then j^2, then j^1 inside the group).
 // Every lane of the wave must take part.
@@ -479,30 +493,35 @@ __device__ __forceinline__ int corner_off(int c) {
   return ((c >> 2) * kR + ((c >> 1) & 1)) * kR + (c & 1);
 }
 
-// Adds the 8 corner sums of the lanes with `flushing` set to their cell `cur`. Lanes walk
-// the binned positions side by side, so where a wave crosses from one cell to the next many
-// lanes flush the same cell at once: up to two such groups of 16 or more lanes are summed
-// over the wave first (one add per row and corner), the rest add per lane. Every lane of
-// the wave must call this.
+// Adds the 8 corner sums of the lanes with `flushing` set to their cell `cur`. Lanes take
+// the binned positions in cell order side by side, so many lanes flush the same cell at
+// once: up to kFlushGroups such groups of kFlushMin or more lanes (the cells of the lowest
+// and the highest flushing lane first) are summed over the wave (one add per corner), the
+// rest add per lane. Every lane of the wave must call this.
+constexpr int kFlushGroups = 6;
+constexpr int kFlushMin = 8;
 template <typename T>
 __device__ __forceinline__ void flush_sums(T* __restrict__ acc, int cur, const T (&sum)[8],
                                            bool flushing, int lane) {
   uint64_t fl = __ballot(flushing);
 #pragma unroll 1
-  for (int it = 0; it < 2 && __popcll(fl) >= 16; ++it) {
+  for (int it = 0; it < kFlushGroups && __popcll(fl) >= kFlushMin; ++it) {
     const int ka = __builtin_amdgcn_readlane(cur, __builtin_ctzll(fl));
     const int kz = __builtin_amdgcn_readlane(cur, 63 - __builtin_clzll(fl));
     const uint64_t ma = __ballot(cur == ka) & fl, mz = __ballot(cur == kz) & fl;
     const int k0 = __popcll(mz) > __popcll(ma) ? kz : ka;
     const uint64_t same = __popcll(mz) > __popcll(ma) ? mz : ma;
-    if (__popcll(same) < 16) break;
+    if (__popcll(same) < kFlushMin) break;
     const bool me = (same >> lane) & 1ull;
     T w[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) w[c] = me ? sum[c] : T(0);
     T z = group_sums(w, lane);
     z = z + dpp64<kDppRor8>(z);
-    if ((lane & 8) == 0) atomicAdd(&acc[k0 + corner_off(lane & 7)], z);
+    // rows 0..3 hold their sums in lanes 0..7: add the rows pairwise across the wave
+    z = z + shfl64(z, lane ^ 16);
+    z = z + shfl64(z, lane ^ 32);
+    if (lane < 8) atomicAdd(&acc[k0 + corner_off(lane)], z);
     fl &= ~same;
   }
   if ((fl >> lane) & 1ull) {
@@ -561,21 +580,21 @@ __device__ __forceinline__ float ldbf(__amdgpu_buffer_rsrc_t r, uint32_t index) 
 // positions l, l+64, ... of the phase), stored to the wave's LDS stage, and read back so
 // that lane l takes kPhaseRun consecutive positions, which in a brick's cell order keeps a
 // lane in one cell wherever cells hold more than a few particles. A lane keeps the 8 corner
-// sums of up to three cells in registers over a round of kPhases phases; they go to the LDS
-// sums at the end of the round, and a slot taken over by a fourth cell goes there at once,
+// sums of up to two cells in registers over a round of kPhases phases; they go to the LDS
+// sums at the end of the round, and a slot taken over by a third cell goes there at once,
 // those that many lanes share summed over the wave first (flush_sums).
-constexpr int kPhaseRun = 4;                   // consecutive positions per lane and phase
+constexpr int kPhaseRun = 8;                   // consecutive positions per lane and phase
 constexpr int kPhasePos = 64 * kPhaseRun;      // positions per wave and phase
-constexpr int kPhases = 4;                     // phases per round
+constexpr int kPhases = 2;                     // phases per round
 constexpr int kWavePos = kPhases * kPhasePos;  // positions per wave and round
 constexpr int kRoundPos = 4 * kWavePos;        // positions per workgroup and round
 constexpr int kGroup = 4;                      // positions per lane processed together
-// Stage layout: position p of a phase at p + p/32 (one pad word per 32), so the 32 lanes of
-// each half-wave hit 32 distinct banks both when storing (lanes l + 64j) and when reading
-// their runs (lanes 4l + i).
-static_assert(kPhaseRun == 4, "the stage padding is laid out for runs of 4");
-constexpr int kStageStride = kPhasePos + kPhasePos / 32;  // floats per array and wave
-__device__ __forceinline__ int stage_slot(int p) { return p + (p >> 5); }
+// Stage layout: position p of a phase at p + p/8 (one pad word per 8), so the 32 lanes of
+// each half-wave read their runs (lanes 8l + i, slots 9l + i) from 32 distinct banks; the
+// coalesced stores (lanes l + 64j) meet at most two to a bank.
+static_assert(kPhaseRun == 8, "the stage padding is laid out for runs of 8");
+constexpr int kStageStride = kPhasePos + kPhasePos / 8;  // floats per array and wave
+__device__ __forceinline__ int stage_slot(int p) { return p + (p >> 3); }
 constexpr int kFoldMax = 64;              // largest scale exponent folded into m'
 
 // The unit loop of unit_deposit_kernel. FOLD (s <= kFoldMax): the scale 2^s is applied to
@@ -586,9 +605,9 @@ constexpr int kFoldMax = 64;              // largest scale exponent folded into 
 // 2^-126 for the scaled chain when s < 0) and round to 0.
 template <bool F64, bool IDX, bool FOLD>
 __device__ __forceinline__ void unit_loop(
-    typename Sum<F64>::T* __restrict__ acc, float* __restrict__ stage,
-    const Unit* __restrict__ units, int32_t u_begin,
-    int32_t u_end, const int32_t* __restrict__ idx, const float* __restrict__ px,
+    typename Sum<F64>::T* __restrict__ acc, float* __restrict__ stage, int32_t* __restrict__ claim,
+    const Unit* __restrict__ units, const int32_t* __restrict__ work, int32_t nu,
+    uint32_t* __restrict__ queue, const int32_t* __restrict__ idx, const float* __restrict__ px,
     const float* __restrict__ py, const float* __restrict__ pz, const float* __restrict__ mass,
     int64_t np, int n, float dx, float inv_vol, int nbc, int nbn, int s,
     unsigned long long* __restrict__ partial, unsigned long long* __restrict__ esc,
@@ -620,15 +639,24 @@ __device__ __forceinline__ void unit_loop(
   float* const sy = sx + kStageStride;
   float* const sz = sy + kStageStride;
   float* const sm = sz + kStageStride;
-  // a unit's table entry is loaded one unit ahead, and a wave's first phase of a unit while
-  // it finishes the unit before, so neither load's latency adds to the unit's time
-  Unit un_next = u_begin < u_end ? units[u_begin] : Unit{};
+  // Units in the work order: workgroup w takes positions w and w + G (G workgroups) of it,
+  // then claims positions from 2G on through the queue, at each unit's start the one after
+  // the next (the claim's result is only needed at the unit's end; claims spread over the
+  // launch instead of all workgroups meeting on one counter at its start). A unit's table
+  // entry is loaded one unit ahead, and a wave's first phase of a unit while it finishes the
+  // unit before, so none of these loads' latency adds to a unit's time.
+  const int32_t G = static_cast<int32_t>(gridDim.x);
+  int32_t cur = static_cast<int32_t>(blockIdx.x), nxt = cur + G;
+  Unit un_next = cur < nu ? units[work[cur]] : Unit{};
   float4 ph[kPhaseRun];
-  if (u_begin < u_end) load(un_next, wave * kWavePos + lane, ph);
-  for (int32_t u = u_begin; u < u_end; ++u) {
+  if (cur < nu) load(un_next, wave * kWavePos + lane, ph);
+  while (cur < nu) {
+    const int32_t u = work[cur];
     const Unit un = un_next;
-    const bool more = u + 1 < u_end;
-    if (more) un_next = units[u + 1];
+    const bool more = nxt < nu;
+    if (more) un_next = units[work[nxt]];
+    uint32_t after = 0u;
+    if (tid == 0) after = atomicAdd(queue, 1u) + 2u * static_cast<uint32_t>(G);
     bool pref = false;  // ph holds the next unit's first phase
     const bool home = un.brick >= 0;
     int ox = INT_MIN / 2, oy = INT_MIN / 2, oz = INT_MIN / 2;  // region origin (nodes)
@@ -643,13 +671,13 @@ __device__ __forceinline__ void unit_loop(
     __syncthreads();
     for (int base = 0; base < un.count; base += kRoundPos) {
       if (base + wave * kWavePos >= un.count) continue;  // nothing of this round for the wave
-      // three cells (region indices, -1 when free) with their 8 corner sums: the lane's
-      // cell, and the cells past a cell boundary of its runs; a fourth cell takes slot 2
-      // after adding its sums
-      int k0 = -1, k1 = -1, k2 = -1;
-      T s0[8], s1[8], s2[8];
+      // two cells (region indices, -1 when free) with their 8 corner sums: the lane's cell,
+      // and the cell past a cell boundary of its runs; a third cell takes slot 1 after
+      // adding its sums
+      int k0 = -1, k1 = -1;
+      T s0[8], s1[8];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) s0[c] = s1[c] = s2[c] = T(0);
+      for (int c = 0; c < 8; ++c) s0[c] = s1[c] = T(0);
       // (not unrolled: the unrolled body of all kPhases phases outgrew the instruction cache)
 #pragma unroll 1
       for (int h = 0; h < kPhases; ++h) {
@@ -719,25 +747,23 @@ __device__ __forceinline__ void unit_loop(
               for (int c = 0; c < 8; ++c) q[c] = pm::fixed_q(bxy[c >> 1] * ((c & 1) ? fz : gz0), s);
               add_outside(q, ix, iy, iz, n, nbn, esc, eflag, esc_flag);
             }
-            // slot of the particle's cell: the cell of slot 0 or 1 (or a free one of them), else
-            // slot 2, whose cell, when another, first goes to the LDS sums; the lanes doing so
+            // slot of the particle's cell: slot 0 when it holds that cell (or is free), else
+            // slot 1, whose cell, when another, first goes to the LDS sums; the lanes doing so
             // at one step mostly share that cell, so it is summed over the wave (every lane
             // takes part in flush_sums)
             const int key = (rx * kR + ry) * kR + rz;
             const bool h0 = reg && (key == k0 || k0 < 0);
-            const bool h1 = reg && !h0 && (key == k1 || k1 < 0);
-            const bool h2 = reg && !h0 && !h1;
-            const bool evict = h2 && k2 >= 0 && key != k2;
+            const bool h1 = reg && !h0;
+            const bool evict = h1 && k1 >= 0 && key != k1;
             if (__builtin_amdgcn_ballot_w64(evict) != 0ull) {
-              flush_sums(acc, k2, s2, evict, lane);
+              flush_sums(acc, k1, s1, evict, lane);
               if (evict) {
 #pragma unroll
-                for (int c = 0; c < 8; ++c) s2[c] = T(0);
+                for (int c = 0; c < 8; ++c) s1[c] = T(0);
               }
             }
             k0 = h0 ? key : k0;
             k1 = h1 ? key : k1;
-            k2 = h2 ? key : k2;
             if (reg) {
               const float m2 = FOLD ? mp * scale : mp;
               const float a0 = m2 * gx0, a1 = m2 * fx;
@@ -756,10 +782,6 @@ __device__ __forceinline__ void unit_loop(
 #pragma unroll
                 for (int c = 0; c < 8; ++c) s1[c] = s1[c] + v[c];
               }
-              if (h2) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) s2[c] = s2[c] + v[c];
-              }
             }
           }
         }
@@ -768,24 +790,28 @@ __device__ __forceinline__ void unit_loop(
       }
       flush_sums(acc, k0, s0, k0 >= 0, lane);
       flush_sums(acc, k1, s1, k1 >= 0, lane);
-      flush_sums(acc, k2, s2, k2 >= 0, lane);
     }
     if (more && !pref) load(un_next, wave * kWavePos + lane, ph);
+    if (tid == 0) claim[0] = static_cast<int32_t>(after);
     __syncthreads();
     if (home) {
       unsigned long long* dst = partial + static_cast<int64_t>(u) * kRegion;
       for (int q = tid; q < kRN; q += blockDim.x)
         dst[region_slot(q / (kR * kR), (q / kR) % kR, q % kR)] = S::u64(acc[q]);
     }
-    __syncthreads();  // the next unit zeroes acc
+    const int32_t claimed = claim[0];
+    __syncthreads();  // the next unit zeroes acc and rewrites claim[0]
+    cur = nxt;
+    nxt = claimed;
   }
 }
 
-// IDX: the particles are read through the binning's order idx (else in place).
-// Workgroup w takes units [wg_first[w], wg_first[w+1]) (unit_assign_kernel).
+// IDX: the particles are read through the binning's order idx (else in place). Workgroups
+// claim units in the order `work` from queue (zero at the launch) until all nu are taken.
 template <bool F64, bool IDX>
 __global__ void __launch_bounds__(256, 3) unit_deposit_kernel(
-    const Unit* __restrict__ units, const int32_t* __restrict__ wg_first,
+    const Unit* __restrict__ units, const int32_t* __restrict__ work,
+    const uint32_t* __restrict__ unit_count, uint32_t* __restrict__ queue,
     const int32_t* __restrict__ idx, const float* __restrict__ px, const float* __restrict__ py,
     const float* __restrict__ pz, const float* __restrict__ mass, int64_t np, int n, float dx,
     float inv_vol, int nbc, int nbn, const unsigned int* __restrict__ maxbits, int bl,
@@ -793,57 +819,37 @@ __global__ void __launch_bounds__(256, 3) unit_deposit_kernel(
     uint8_t* __restrict__ eflag, uint32_t* __restrict__ esc_any, int parity) {
   __shared__ typename Sum<F64>::T acc[kRN];
   __shared__ float stage[4 * 4 * kStageStride];
+  __shared__ int32_t claim[1];
   const int s = pm::fixed_scale(*maxbits, bl);
-  const int32_t u_begin = wg_first[blockIdx.x], u_end = wg_first[blockIdx.x + 1];
+  const int32_t nu = static_cast<int32_t>(*unit_count);
   if (s <= kFoldMax)
-    unit_loop<F64, IDX, true>(acc, stage, units, u_begin, u_end, idx, px, py, pz, mass, np, n,
-                              dx, inv_vol, nbc, nbn, s, partial, esc, eflag, esc_any + parity);
+    unit_loop<F64, IDX, true>(acc, stage, claim, units, work, nu, queue + parity, idx, px, py,
+                              pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc, eflag,
+                              esc_any + parity);
   else
-    unit_loop<F64, IDX, false>(acc, stage, units, u_begin, u_end, idx, px, py, pz, mass, np, n,
-                               dx, inv_vol, nbc, nbn, s, partial, esc, eflag, esc_any + parity);
+    unit_loop<F64, IDX, false>(acc, stage, claim, units, work, nu, queue + parity, idx, px, py,
+                               pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc, eflag,
+                               esc_any + parity);
 }
 
-// One workgroup: splits the units into `groups` consecutive ranges of about equal cost
-// (positions plus kUnitCost per unit, for its setup, its cleared and stored region and its
-// partly filled rounds), so the unit pass's workgroups finish together: wg_first[w] = the
-// first unit of range w, wg_first[groups] = the number of units. kUnitCost: a unit costs
-// about as much time as 1,750 positions (per-workgroup timings at 10M particles / 256^3).
-constexpr int kUnitCost = 1750;
-
-__global__ void __launch_bounds__(1024) unit_assign_kernel(const Unit* __restrict__ units,
-                                                           const uint32_t* __restrict__ unit_count,
-                                                           int groups,
-                                                           int32_t* __restrict__ wg_first) {
+// One workgroup: the order in which the unit pass claims units, work[0, nu): the units of
+// more than kUnit/2 positions first (each takes about as long as a few dozen small ones),
+// then the others, each part in table order, so that the last claims are short.
+__global__ void __launch_bounds__(1024) unit_order_kernel(const Unit* __restrict__ units,
+                                                          const uint32_t* __restrict__ unit_count,
+                                                          int32_t* __restrict__ work) {
   __shared__ uint32_t s_wave[16];
   const uint32_t nu = *unit_count;
   const uint32_t span = (nu + 1023u) / 1024u;
   const uint32_t u0 = min(threadIdx.x * span, nu), u1 = min(u0 + span, nu);
-  uint32_t cost = 0;
-  for (uint32_t u = u0; u < u1; ++u) cost += static_cast<uint32_t>(units[u].count) + kUnitCost;
-  uint32_t total;
-  uint32_t before = scan_1024(cost, total, s_wave);
-  // unit u starts range floor(P_u * groups / total) (P_u: cost of the units before u); the
-  // ranges between the previous unit's and u's take u as their first unit
-  int64_t prev_w = -1;
-  if (u0 > 0 && u0 < u1) {
-    const uint32_t pc = static_cast<uint32_t>(units[u0 - 1].count) + kUnitCost;
-    prev_w = static_cast<int64_t>(before - pc) * groups / max(total, 1u);
-  }
+  uint32_t big = 0;
+  for (uint32_t u = u0; u < u1; ++u) big += units[u].count > kUnit / 2 ? 1u : 0u;
+  uint32_t nbig;
+  uint32_t b = scan_1024(big, nbig, s_wave);
+  uint32_t small = nbig + (u0 - b);  // small units before u0 = units before u0 - big ones
   for (uint32_t u = u0; u < u1; ++u) {
-    const int64_t w = static_cast<int64_t>(before) * groups / max(total, 1u);
-    for (int64_t q = prev_w + 1; q <= w; ++q) wg_first[q] = static_cast<int32_t>(u);
-    prev_w = w;
-    before += static_cast<uint32_t>(units[u].count) + kUnitCost;
-  }
-  __syncthreads();
-  // ranges after the last unit's (and all of them when there are no units) are empty
-  if (threadIdx.x == 0) {
-    int64_t last_w = -1;
-    if (nu > 0) {
-      const uint32_t pc = static_cast<uint32_t>(units[nu - 1].count) + kUnitCost;
-      last_w = static_cast<int64_t>(total - pc) * groups / max(total, 1u);
-    }
-    for (int64_t q = last_w + 1; q <= groups; ++q) wg_first[q] = static_cast<int32_t>(nu);
+    if (units[u].count > kUnit / 2) work[b++] = static_cast<int32_t>(u);
+    else work[small++] = static_cast<int32_t>(u);
   }
 }
 
@@ -892,7 +898,8 @@ __global__ void __launch_bounds__(256) brick_reduce_kernel(const Unit* __restric
 // only when `full` (a new binning, another rho) or when this or the previous deposit had
 // particles outside their regions (esc_any[parity], dirty[parity]), whose sums may lie
 // there; otherwise they still hold the zeros stored before. The flags for the next deposit
-// go to the other parity, which no workgroup of this launch reads.
+// (and its unit pass's claim counter, zeroed) go to the other parity, which no workgroup of
+// this launch reads.
 //
 // One wave per node brick B; lane (a, b) sums the node row (a, b, 0..7). Node brick B reads
 // from the region of cell brick B + e (e in {-1, 0, 1}^3) the class block 1 - e: its own
@@ -904,7 +911,7 @@ __global__ void __launch_bounds__(256) brick_combine_kernel(
     const int32_t* __restrict__ node_list, uint32_t nnb, const int2* __restrict__ desc,
     const unsigned long long* __restrict__ partial, unsigned long long* __restrict__ esc,
     uint8_t* __restrict__ eflag, uint32_t* __restrict__ esc_any, uint32_t* __restrict__ dirty,
-    int parity, bool full, uint32_t* __restrict__ node_count,
+    uint32_t* __restrict__ queue, int parity, bool full, uint32_t* __restrict__ node_count,
     const unsigned int* __restrict__ maxbits, int bl, int n, int nbn,
     float* __restrict__ rho, bool vec_rho) {
   const int s = pm::fixed_scale(*maxbits, bl);
@@ -992,6 +999,7 @@ __global__ void __launch_bounds__(256) brick_combine_kernel(
     node_count[0] = outside ? nnb : active;
     dirty[parity ^ 1] = outside ? 1u : 0u;
     esc_any[parity ^ 1] = 0u;
+    queue[parity ^ 1] = 0u;  // the next deposit's unit pass claims from it
   }
 }
 
@@ -1031,7 +1039,8 @@ struct State {
   int2* desc = nullptr;            // 27 source descriptors per node brick with sources
   int32_t* multi = nullptr;        // first units of the bricks with several units
   uint32_t* multi_count = nullptr;
-  int32_t* wg_first = nullptr;     // unit ranges of the unit pass's workgroups
+  int32_t* work = nullptr;         // the order in which the unit pass claims units
+  uint32_t* queue = nullptr;       // claim counters of the unit pass, by parity
   unsigned long long* partial = nullptr;
   unsigned long long* esc = nullptr;
   uint8_t* eflag = nullptr;
@@ -1053,7 +1062,7 @@ void destroy(State* st) {
   if (!st) return;
   void* bufs[] = {st->first, st->end, st->unit_first, st->unit_num, st->units, st->unit_count,
                   st->node_src, st->node_list, st->node_count, st->desc, st->multi,
-                  st->multi_count, st->wg_first, st->partial, st->esc, st->eflag,
+                  st->multi_count, st->work, st->queue, st->partial, st->esc, st->eflag,
                   st->esc_any, st->dirty, st->maxbits};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -1091,7 +1100,8 @@ pm_status create(int n, int64_t cap, State** out) {
       (e = alloc(&st->desc, 27 * st->nnb, "hipMalloc (node sources)")) ||
       (e = alloc(&st->multi, st->max_units, "hipMalloc (brick units)")) ||
       (e = alloc(&st->multi_count, 1, "hipMalloc (brick units)")) ||
-      (e = alloc(&st->wg_first, kUnitGroups + 1, "hipMalloc (unit ranges)")) ||
+      (e = alloc(&st->work, st->max_units, "hipMalloc (unit order)")) ||
+      (e = alloc(&st->queue, 2, "hipMalloc (unit queue)")) ||
       (e = alloc(&st->partial, st->max_home_units * kRegion, "hipMalloc (unit sums)")) ||
       (e = alloc(&st->esc, nodes, "hipMalloc (outside sums)")) ||
       (e = alloc(&st->eflag, st->nnb, "hipMalloc (outside flags)")) ||
@@ -1111,6 +1121,7 @@ pm_status create(int n, int64_t cap, State** out) {
       (he = hipMemset(st->eflag, 0, st->nnb)) != hipSuccess ||
       (he = hipMemset(st->esc_any, 0, 2 * sizeof(uint32_t))) != hipSuccess ||
       (he = hipMemset(st->dirty, 0, 2 * sizeof(uint32_t))) != hipSuccess ||
+      (he = hipMemset(st->queue, 0, 2 * sizeof(uint32_t))) != hipSuccess ||
 
       (he = hipMemset(st->node_count, 0, 2 * sizeof(uint32_t))) != hipSuccess ||
       (he = hipDeviceSynchronize()) != hipSuccess) {
@@ -1166,9 +1177,8 @@ pm_status deposit(State* st, const SortBuffers& sb, const float* px, const float
                                            static_cast<int>(key_blocks), st->multi,
                                            st->multi_count);
       PM_CHECK_LAUNCH("unit_build_kernel");
-      unit_assign_kernel<<<1, 1024, 0, s>>>(st->units, st->unit_count, kUnitGroups,
-                                            st->wg_first);
-      PM_CHECK_LAUNCH("unit_assign_kernel");
+      unit_order_kernel<<<1, 1024, 0, s>>>(st->units, st->unit_count, st->work);
+      PM_CHECK_LAUNCH("unit_order_kernel");
     }
     {
       PM_TIMED("deposit.bin_nodes", 0.0, s);
@@ -1194,7 +1204,8 @@ pm_status deposit(State* st, const SortBuffers& sb, const float* px, const float
     const double bytes = (idx ? 20.0 : 16.0) * np;
 #define PM_UNIT_PASS(F64, IDX)                                                                 \
   PM_LAUNCH_TIMED("deposit.unit_sums", bytes, (unit_deposit_kernel<F64, IDX>), kUnitGroups, 256, \
-                  0, s, st->units, st->wg_first, idx, px, py, pz, mass, np, n, dx, inv_vol,    \
+                  0, s, st->units, st->work, st->unit_count, st->queue, idx, px, py, pz, mass, \
+                  np, n, dx, inv_vol,                                                          \
                   st->nbc, st->nbn, st->maxbits, bl, st->partial, st->esc, st->eflag,          \
                   st->esc_any, st->parity)
     if (bl >= kF64Bits) {
@@ -1215,8 +1226,8 @@ pm_status deposit(State* st, const SortBuffers& sb, const float* px, const float
     PM_LAUNCH_TIMED("deposit.combine", 4.0 * nodes, brick_combine_kernel,
                     static_cast<unsigned>(std::min<int64_t>(st->nnb, kPersistent)), 256, 0, s,
                     st->node_list, static_cast<uint32_t>(st->nnb), st->desc, st->partial, st->esc,
-                    st->eflag, st->esc_any, st->dirty, st->parity, full, st->node_count,
-                    st->maxbits, bl, n, st->nbn, rho,
+                    st->eflag, st->esc_any, st->dirty, st->queue, st->parity, full,
+                    st->node_count, st->maxbits, bl, n, st->nbn, rho,
                     (n & 3) == 0 && (reinterpret_cast<uintptr_t>(rho) & 15u) == 0u);
     PM_CHECK_LAUNCH("brick_combine_kernel");
   }
