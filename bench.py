@@ -356,7 +356,7 @@ def main() -> int:
     chain.run(args.warmup, w["dt"])
     torch.cuda.synchronize()
 
-    # pass 1: throughput. The radix scatter's own dispatches record HIP event pairs on its
+    # pass 1: throughput. The dominant kernel's own dispatches record HIP event pairs on its
     # stream (timing mode 2 adds no packets, so the steps run as untimed ones)
     chain.last_cycles.clear()
     nat.timing_reset()

@@ -569,8 +569,9 @@ typedef struct {
 } pm_kernel_time;
 
 /* on: 0 off; 1 every instrumented launch (multigrid V-cycles then run eagerly, not from
- * their graphs); 2 only the launches whose own dispatch records the pair (the radix scatter),
- * which adds nothing to the streams, so the timed steps run exactly as untimed ones. */
+ * their graphs); 2 only the launches whose own dispatch records the pair (the deposit's radix
+ * scatter, unit pass, brick reduce and combine), which adds nothing to the streams, so the
+ * timed steps run exactly as untimed ones. */
 pm_status pm_timing_enable(int32_t on);
 /* Drops accumulated totals (waits for pending events). */
 pm_status pm_timing_reset(void);
