@@ -969,12 +969,21 @@ struct Quad {
 };
 __device__ __forceinline__ Quad quad_idx(uint32_t u, int n, int hq) {
   // u = (i*n + j) * hq + kq, hq = hn / 4
-  const uint32_t row = u / static_cast<uint32_t>(hq);
   Quad q;
+  q.t0 = u * 4u;
+  if ((n & (n - 1)) == 0 && (hq & (hq - 1)) == 0) {
+    // power-of-two n and hq (uniform over the launch): shifts and masks instead of the
+    // integer divisions
+    const uint32_t row = u >> __builtin_ctz(static_cast<uint32_t>(hq));
+    q.kk0 = static_cast<int>(u & static_cast<uint32_t>(hq - 1)) * 4;
+    q.j = static_cast<int>(row & static_cast<uint32_t>(n - 1));
+    q.i = static_cast<int>(row >> __builtin_ctz(static_cast<uint32_t>(n)));
+    return q;
+  }
+  const uint32_t row = u / static_cast<uint32_t>(hq);
   q.kk0 = static_cast<int>(u - row * static_cast<uint32_t>(hq)) * 4;
   q.j = static_cast<int>(row % static_cast<uint32_t>(n));
   q.i = static_cast<int>(row / static_cast<uint32_t>(n));
-  q.t0 = u * 4u;
   return q;
 }
 __device__ __forceinline__ float4 ld4(const float* __restrict__ p, uint32_t off) {
@@ -1156,8 +1165,11 @@ constexpr int kProlongMaxLds = 4200;  // 2*(2048/n + 1)*(n/2 + 1) <= 4100 floats
 
 // Quads per thread of prolong_s4_kernel: quads u0 + tid + 256*m, m < kProlongQ.
 constexpr int kProlongQ = 2;
+constexpr int kProlongBlock = 256;  // threads of a prolong_s4_kernel workgroup
+constexpr int kProlongStageIt = (kProlongMaxLds + kProlongBlock - 1) / kProlongBlock;
+static_assert(kProlongBlock == pm::kBlock, "prolong_s4_kernel is launched with pm::kBlock");
 
-__global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict__ phi_c,
+__global__ void __launch_bounds__(kProlongBlock) prolong_s4_kernel(const float* __restrict__ phi_c,
                                                          float* __restrict__ e, int n, int hn,
                                                          int nc) {
   __shared__ float s_c[kProlongMaxLds];
@@ -1196,12 +1208,22 @@ __global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict
                       static_cast<int>(r1 / static_cast<uint32_t>(n)) == i0 &&
                       ci0 + 1 < nc && cj0 + ncj <= nc && 2 * ncj * nc <= kProlongMaxLds;
   if (staged) {
+    // rows cj0..cj0+ncj-1 of coarse plane ci0 + a are one contiguous run of per floats:
+    // the workgroup's values are two runs, loaded all at once (every load issued before
+    // the first LDS store)
     const int per = ncj * nc;
-    for (int t = threadIdx.x; t < 2 * per; t += blockDim.x) {
-      const int a = t / per, rem = t - a * per;
-      const int b = rem / nc, k = rem - b * nc;
-      s_c[t] = phi_c[static_cast<uint32_t>(ci0 + a) * nc2 +
-                     static_cast<uint32_t>(cj0 + b) * nc + k];
+    const uint32_t run0 = static_cast<uint32_t>(ci0) * nc2 + static_cast<uint32_t>(cj0) * nc;
+    float tmp[kProlongStageIt];
+#pragma unroll
+    for (int it = 0; it < kProlongStageIt; ++it) {
+      const int t = static_cast<int>(threadIdx.x) + it * kProlongBlock;
+      const uint32_t src = run0 + static_cast<uint32_t>(t < per ? t : t - per + nc2);
+      tmp[it] = t < 2 * per ? phi_c[src] : 0.0f;
+    }
+#pragma unroll
+    for (int it = 0; it < kProlongStageIt; ++it) {
+      const int t = static_cast<int>(threadIdx.x) + it * kProlongBlock;
+      if (t < 2 * per) s_c[t] = tmp[it];
     }
   }
   __syncthreads();
@@ -1257,7 +1279,10 @@ __global__ void __launch_bounds__(256) prolong_s4_kernel(const float* __restrict
 }
 
 // residual64_s_kernel, 8 consecutive natural nodes per thread (n % 8 == 0): the r values
-// are the same; the squared norm is summed in another order.
+// are the same; the squared norm is summed in another order. A thread takes kRes64Q such
+// runs (quads u, u + 256, ...) and issues the loads of all of them before any arithmetic
+// (2 measured slower than 1 at 256^3: 140 VGPRs halve the waves in flight).
+constexpr int kRes64Q = 1;
 __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restrict__ phi,
                                                             const float* __restrict__ rho,
                                                             float factor, int n,
@@ -1272,58 +1297,75 @@ __global__ void __launch_bounds__(256) residual64_s4_kernel(const float* __restr
   const uint32_t n2 = static_cast<uint32_t>(n) * n;
   int64_t r_begin, r_end;
   pm::block_range(quads, r_begin, r_end);
-  for (int64_t u = r_begin + threadIdx.x; u < r_end; u += blockDim.x) {
-    const Quad q = quad_idx(static_cast<uint32_t>(u), n, hq);
-    const uint32_t c0 = (static_cast<uint32_t>(q.i) * n + q.j) * n + 2 * q.kk0;
-    float r8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (q.i >= 1 && q.i <= n - 2 && q.j >= 1 && q.j <= n - 2) {
-      float ctr[10], ip[8], im[8], jp[8], jm[8], fv[8];
-      const float4 c_a = ld4(phi, c0), c_b = ld4(phi, c0 + 4);
-      ctr[0] = phi[c0 - 1];
-      ctr[1] = c_a.x; ctr[2] = c_a.y; ctr[3] = c_a.z; ctr[4] = c_a.w;
-      ctr[5] = c_b.x; ctr[6] = c_b.y; ctr[7] = c_b.z; ctr[8] = c_b.w;
-      ctr[9] = phi[c0 + 8];
-      auto load8 = [&](float* dst, const float* src, uint32_t off) {
-        const float4 a = ld4(src, off), b = ld4(src, off + 4);
-        dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
-        dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
-      };
-      load8(ip, phi, c0 + n2);
-      load8(im, phi, c0 - n2);
-      load8(jp, phi, c0 + n);
-      load8(jm, phi, c0 - n);
-      load8(fv, rho, c0);
-      // all fourteen loads leave before any arithmetic: the scheduler would otherwise hold
-      // some back to save registers, a second round trip for them
-      __builtin_amdgcn_sched_barrier(0);
+  const int64_t step = static_cast<int64_t>(blockDim.x);
+  for (int64_t u0 = r_begin + threadIdx.x; u0 < r_end; u0 += kRes64Q * step) {
+    Quad q[kRes64Q];
+    bool live[kRes64Q], in[kRes64Q];
+    uint32_t c0[kRes64Q];
+    float ctr[kRes64Q][10], ip[kRes64Q][8], im[kRes64Q][8], jp[kRes64Q][8], jm[kRes64Q][8],
+        fv[kRes64Q][8];
+    auto load8 = [&](float* dst, const float* src, uint32_t off) {
+      const float4 a = ld4(src, off), b = ld4(src, off + 4);
+      dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
+      dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
+    };
 #pragma unroll
-      for (int x = 0; x < 8; ++x) fv[x] = factor * fv[x];  // f = 4*pi*G*rho, as the setup
-      // every node is evaluated and the faces (k = 0, n-1) are masked afterwards: were the
-      // face terms skipped by a branch, the compiler would move the loads that only they
-      // use behind it, a second round trip to memory for most rows
-#pragma unroll
-      for (int x = 0; x < 8; ++x) {
-        const int k = 2 * q.kk0 + x;
-        const bool inner = k >= 1 && k <= n - 2;
-        const double sum = static_cast<double>(ip[x]) + static_cast<double>(im[x]) +
-                           static_cast<double>(jp[x]) + static_cast<double>(jm[x]) +
-                           static_cast<double>(ctr[x + 2]) + static_cast<double>(ctr[x]);
-        const double lap = (sum - 6.0 * static_cast<double>(ctr[x + 1])) * inv_h2;
-        const double r = static_cast<double>(fv[x]) - lap;
-        r8[x] = inner ? static_cast<float>(r) : 0.0f;
-        const double acc = v[0] + r * r;
-        v[0] = inner ? acc : v[0];
+    for (int m = 0; m < kRes64Q; ++m) {
+      const int64_t u = u0 + m * step;
+      live[m] = u < r_end;
+      q[m] = quad_idx(live[m] ? static_cast<uint32_t>(u) : 0u, n, hq);
+      c0[m] = (static_cast<uint32_t>(q[m].i) * n + q[m].j) * n + 2 * q[m].kk0;
+      in[m] = live[m] && q[m].i >= 1 && q[m].i <= n - 2 && q[m].j >= 1 && q[m].j <= n - 2;
+      if (in[m]) {
+        const uint32_t c = c0[m];
+        ctr[m][0] = phi[c - 1];
+        load8(&ctr[m][1], phi, c);
+        ctr[m][9] = phi[c + 8];
+        load8(ip[m], phi, c + n2);
+        load8(im[m], phi, c - n2);
+        load8(jp[m], phi, c + n);
+        load8(jm[m], phi, c - n);
+        load8(fv[m], rho, c);
       }
     }
-    // colour of natural k = 2*kk0 + x is (p0 + x) & 1; its position kk0 + (x >> 1)
-    const int p0 = (q.i + q.j) & 1;
-    float a0[4], a1[4];
+    // all loads leave before any arithmetic: the scheduler would otherwise hold some back
+    // to save registers, a second round trip for them
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int x = 0; x < 8; ++x) {
-      if (((p0 + x) & 1) == 0) a0[x >> 1] = r8[x]; else a1[x >> 1] = r8[x];
+    for (int m = 0; m < kRes64Q; ++m) {
+      if (!live[m]) continue;
+      float r8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (in[m]) {
+#pragma unroll
+        for (int x = 0; x < 8; ++x) fv[m][x] = factor * fv[m][x];  // f = 4*pi*G*rho, as the setup
+        // every node is evaluated and the faces (k = 0, n-1) are masked afterwards: were the
+        // face terms skipped by a branch, the compiler would move the loads that only they
+        // use behind it, a second round trip to memory for most rows
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+          const int k = 2 * q[m].kk0 + x;
+          const bool inner = k >= 1 && k <= n - 2;
+          const double sum = static_cast<double>(ip[m][x]) + static_cast<double>(im[m][x]) +
+                             static_cast<double>(jp[m][x]) + static_cast<double>(jm[m][x]) +
+                             static_cast<double>(ctr[m][x + 2]) + static_cast<double>(ctr[m][x]);
+          const double lap = (sum - 6.0 * static_cast<double>(ctr[m][x + 1])) * inv_h2;
+          const double r = static_cast<double>(fv[m][x]) - lap;
+          r8[x] = inner ? static_cast<float>(r) : 0.0f;
+          const double acc = v[0] + r * r;
+          v[0] = inner ? acc : v[0];
+        }
+      }
+      // colour of natural k = 2*kk0 + x is (p0 + x) & 1; its position kk0 + (x >> 1)
+      const int p0 = (q[m].i + q[m].j) & 1;
+      float a0[4], a1[4];
+#pragma unroll
+      for (int x = 0; x < 8; ++x) {
+        if (((p0 + x) & 1) == 0) a0[x >> 1] = r8[x]; else a1[x >> 1] = r8[x];
+      }
+      *reinterpret_cast<float4*>(r32s + q[m].t0) = make_float4(a0[0], a0[1], a0[2], a0[3]);
+      *reinterpret_cast<float4*>(r32s + plane + q[m].t0) =
+          make_float4(a1[0], a1[1], a1[2], a1[3]);
     }
-    *reinterpret_cast<float4*>(r32s + q.t0) = make_float4(a0[0], a0[1], a0[2], a0[3]);
-    *reinterpret_cast<float4*>(r32s + plane + q.t0) = make_float4(a1[0], a1[1], a1[2], a1[3]);
   }
   pm::block_sum<1>(v, s_red);
   if (threadIdx.x == 0) partials[blockIdx.x] = v[0];  // one column (pm::fold_columns_kernel)
@@ -1507,6 +1549,7 @@ struct pm_mg {
   // the 4-wide split reductions (float64 defect, fused update) run one quad per thread:
   // quad_blocks workgroups with their own partial sums
   unsigned quad_blocks = 0;
+  unsigned res_blocks = 0;  // workgroups (and partials) of residual64_s4_kernel
   double* partials_q = nullptr;
   // the float64 defect's partials when their fold is left to the cycle's own fold launch
   double* partials_r = nullptr;
@@ -1852,7 +1895,7 @@ pm_status mixed_residual(pm_mg* mg, const float* phi, hipStream_t s, bool split 
   // 8 nodes per thread on split grids with n % 8 == 0 and 16-byte aligned phi and rho
   const bool quad = split && mg->quad_blocks && (reinterpret_cast<uintptr_t>(phi) & 15u) == 0 &&
                     (reinterpret_cast<uintptr_t>(mg->cur_rho) & 15u) == 0;
-  const unsigned b = quad ? mg->quad_blocks : pm::red_blocks(L.nodes());
+  const unsigned b = quad ? mg->res_blocks : pm::red_blocks(L.nodes());
   double* partials = quad ? (defer ? mg->partials_r : mg->partials_q) : mg->partials;
   {
     PM_TIMED("mg.residual64", kRes64Bytes * L.nodes(), s);
@@ -1912,7 +1955,7 @@ pm_status enqueue_cycle(pm_mg* mg, int kind, float* phi, hipStream_t s) {
       PM_REQUIRE(partials == mg->partials_q, "multigrid: deferred defect fold off the 8-node path");
       pm::fold_columns_pair_kernel<2, 1><<<1, pm::kBlock, 0, s>>>(
           partials, static_cast<int>(blocks), mg->red, mg->partials_r,
-          static_cast<int>(mg->quad_blocks), mg->red + 2);
+          static_cast<int>(mg->res_blocks), mg->red + 2);
     } else if (partials == mg->partials_q)  // black_update4_kernel's columns
       pm::fold_columns_kernel<2><<<1, pm::kBlock, 0, s>>>(partials, static_cast<int>(blocks),
                                                            mg->red);
@@ -2200,6 +2243,8 @@ pm_status pm_mg_create(const pm_mg_desc* desc, pm_mg** out) {
       if (n % 8 == 0) {
         const int64_t quads = static_cast<int64_t>(n) * n * mg->hn / 4;
         mg->quad_blocks = static_cast<unsigned>((quads + pm::kBlock - 1) / pm::kBlock);
+        mg->res_blocks = static_cast<unsigned>((quads + pm::kBlock * kRes64Q - 1) /
+                                               (pm::kBlock * kRes64Q));
         if ((st = dalloc(mg, reinterpret_cast<void**>(&mg->partials_q),
                          sizeof(double) * pm::kRedMaxQ * mg->quad_blocks)) ||
             (st = dalloc(mg, reinterpret_cast<void**>(&mg->partials_r),
