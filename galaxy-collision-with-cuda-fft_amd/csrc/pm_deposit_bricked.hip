@@ -578,11 +578,15 @@ __device__ __forceinline__ float ldbf(__amdgpu_buffer_rsrc_t r, uint32_t index) 
 
 // A wave takes its positions a phase of kPhasePos at a time: loaded coalesced (lane l reads
 // positions l, l+64, ... of the phase), stored to the wave's LDS stage, and read back so
-// that lane l takes kPhaseRun consecutive positions, which in a brick's cell order keeps a
-// lane in one cell wherever cells hold more than a few particles. A lane keeps the 8 corner
-// sums of up to two cells in registers over a round of kPhases phases; they go to the LDS
-// sums at the end of the round, and a slot taken over by a third cell goes there at once,
-// those that many lanes share summed over the wave first (flush_sums).
+// that lane l takes R <= kPhaseRun consecutive positions (R = the phase's positions / 64,
+// rounded up), which in a brick's cell order keeps a lane in one cell wherever cells hold
+// more than a few particles. A round gives each wave kWavePos positions; the last round of a
+// unit (a small unit's only one) splits its remainder evenly over the 4 waves instead, so a
+// unit of a few hundred positions takes all waves with runs of one to four positions rather
+// than one wave with runs of eight. A lane keeps the 8 corner sums of up to two cells in
+// registers over a round of kPhases phases; they go to the LDS sums at the end of the round,
+// and a slot taken over by a third cell goes there at once, those that many lanes share
+// summed over the wave first (flush_sums).
 constexpr int kPhaseRun = 8;                   // consecutive positions per lane and phase
 constexpr int kPhasePos = 64 * kPhaseRun;      // positions per wave and phase
 constexpr int kPhases = 2;                     // phases per round
@@ -595,6 +599,11 @@ constexpr int kGroup = 4;                      // positions per lane processed t
 static_assert(kPhaseRun == 8, "the stage padding is laid out for runs of 8");
 constexpr int kStageStride = kPhasePos + kPhasePos / 8;  // floats per array and wave
 __device__ __forceinline__ int stage_slot(int p) { return p + (p >> 3); }
+// Positions per wave in a round with rem positions left: kWavePos in a full round, else the
+// remainder split over the 4 waves in multiples of 64 (4 * span >= rem).
+__device__ __forceinline__ int wave_span(int rem) {
+  return rem >= kRoundPos ? kWavePos : (((rem + 3) >> 2) + 63) & ~63;
+}
 constexpr int kFoldMax = 64;              // largest scale exponent folded into m'
 
 // The unit loop of unit_deposit_kernel. FOLD (s <= kFoldMax): the scale 2^s is applied to
@@ -606,6 +615,7 @@ constexpr int kFoldMax = 64;              // largest scale exponent folded into 
 template <bool F64, bool IDX, bool FOLD>
 __device__ __forceinline__ void unit_loop(
     typename Sum<F64>::T* __restrict__ acc, float* __restrict__ stage, int32_t* __restrict__ claim,
+    const uint16_t* __restrict__ inv_slot,
     const Unit* __restrict__ units, const int32_t* __restrict__ work, int32_t nu,
     uint32_t* __restrict__ queue, const int32_t* __restrict__ idx, const float* __restrict__ px,
     const float* __restrict__ py, const float* __restrict__ pz, const float* __restrict__ mass,
@@ -624,17 +634,19 @@ __device__ __forceinline__ void unit_loop(
   // unit's end read its last particle, unused)
   const __amdgpu_buffer_rsrc_t bx = rsrc(px, np), by = rsrc(py, np), bz = rsrc(pz, np),
                                bm = rsrc(mass, np), bi = rsrc(idx, IDX ? np : 0);
-  auto load = [&](const Unit& U, int off0, float4 (&p)[kPhaseRun]) {
+  auto load = [&](const Unit& U, int start, float4 (&p)[kPhaseRun]) {
     uint32_t sl[kPhaseRun];
 #pragma unroll
     for (int j = 0; j < kPhaseRun; ++j) {
-      const uint32_t q = static_cast<uint32_t>(U.begin + min(off0 + 64 * j, U.count - 1));
+      const uint32_t q = static_cast<uint32_t>(U.begin + min(start + lane + 64 * j, U.count - 1));
       sl[j] = IDX ? ldb(bi, q) : q;
     }
 #pragma unroll
     for (int j = 0; j < kPhaseRun; ++j)
       p[j] = make_float4(ldbf(bx, sl[j]), ldbf(by, sl[j]), ldbf(bz, sl[j]), ldbf(bm, sl[j]));
   };
+  // the first position of this wave in the round at `base` of a unit of `count` positions
+  auto first_pos = [&](int count, int base) { return base + wave * wave_span(count - base); };
   float* const sx = stage + wave * 4 * kStageStride;
   float* const sy = sx + kStageStride;
   float* const sz = sy + kStageStride;
@@ -649,7 +661,7 @@ __device__ __forceinline__ void unit_loop(
   int32_t cur = static_cast<int32_t>(blockIdx.x), nxt = cur + G;
   Unit un_next = cur < nu ? units[work[cur]] : Unit{};
   float4 ph[kPhaseRun];
-  if (cur < nu) load(un_next, wave * kWavePos + lane, ph);
+  if (cur < nu) load(un_next, first_pos(un_next.count, 0), ph);
   while (cur < nu) {
     const int32_t u = work[cur];
     const Unit un = un_next;
@@ -670,7 +682,10 @@ __device__ __forceinline__ void unit_loop(
     for (int q = tid; q < kRN; q += blockDim.x) acc[q] = T(0);
     __syncthreads();
     for (int base = 0; base < un.count; base += kRoundPos) {
-      if (base + wave * kWavePos >= un.count) continue;  // nothing of this round for the wave
+      // the wave's positions [lo, hi) of this round
+      const int span = wave_span(un.count - base);
+      const int lo = base + wave * span, hi = min(lo + span, un.count);
+      if (lo >= hi) continue;  // nothing of this round for the wave
       // two cells (region indices, -1 when free) with their 8 corner sums: the lane's cell,
       // and the cell past a cell boundary of its runs; a third cell takes slot 1 after
       // adding its sums
@@ -681,8 +696,10 @@ __device__ __forceinline__ void unit_loop(
       // (not unrolled: the unrolled body of all kPhases phases outgrew the instruction cache)
 #pragma unroll 1
       for (int h = 0; h < kPhases; ++h) {
-        const int w0 = base + wave * kWavePos + h * kPhasePos;  // the phase's first offset
-        if (w0 >= un.count) break;
+        const int w0 = lo + h * kPhasePos;  // the phase's first offset
+        if (w0 >= hi) break;
+        const int m = min(kPhasePos, hi - w0);  // the phase's positions
+        const int run = (m + 63) >> 6;          // positions per lane, 1..kPhaseRun
         // the stage is free: the wave's reads of the last phase were issued before these
         // writes, and LDS operations of one wave complete in order
 #pragma unroll
@@ -698,24 +715,24 @@ __device__ __forceinline__ void unit_loop(
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // the next phase's loads (this round's next phase, the next round's first, or the
         // next unit's first) are issued before this phase's work
-        const int next = h + 1 < kPhases ? w0 + kPhasePos : base + kRoundPos + wave * kWavePos;
-        if (next < un.count) {
-          load(un, next + lane, ph);
+        if (w0 + kPhasePos < hi || base + kRoundPos < un.count) {
+          load(un, w0 + kPhasePos < hi ? w0 + kPhasePos : first_pos(un.count, base + kRoundPos),
+               ph);
         } else if (more && !pref) {
-          load(un_next, wave * kWavePos + lane, ph);
+          load(un_next, first_pos(un_next.count, 0), ph);
           pref = true;
         }
 #pragma unroll 1
-        for (int g = 0; g < kPhaseRun; g += kGroup) {
+        for (int g = 0; g < run; g += kGroup) {
           float4 grp[kGroup];
 #pragma unroll
           for (int i = 0; i < kGroup; ++i) {
-            const int t = stage_slot(kPhaseRun * lane + g + i);
+            const int t = stage_slot(run * lane + g + i);  // run * lane + g + i < kPhasePos
             grp[i] = make_float4(sx[t], sy[t], sz[t], sm[t]);
           }
 #pragma unroll
           for (int i = 0; i < kGroup; ++i) {
-            const bool valid = w0 + kPhaseRun * lane + g + i < un.count;
+            const bool valid = g + i < run && run * lane + g + i < m;
             const float x = grp[i].x, y = grp[i].y, z = grp[i].z;
             float gx = quot(x, dx, r1), gy = quot(y, dx, r1), gz = quot(z, dx, r1);
             const bool slow = !fast || !(fabsf(x) >= 0x1p-60f) || !(fabsf(y) >= 0x1p-60f) ||
@@ -791,13 +808,13 @@ __device__ __forceinline__ void unit_loop(
       flush_sums(acc, k0, s0, k0 >= 0, lane);
       flush_sums(acc, k1, s1, k1 >= 0, lane);
     }
-    if (more && !pref) load(un_next, wave * kWavePos + lane, ph);
+    if (more && !pref) load(un_next, first_pos(un_next.count, 0), ph);
     if (tid == 0) claim[0] = static_cast<int32_t>(after);
     __syncthreads();
     if (home) {
+      // in stored order: consecutive lanes write consecutive words
       unsigned long long* dst = partial + static_cast<int64_t>(u) * kRegion;
-      for (int q = tid; q < kRN; q += blockDim.x)
-        dst[region_slot(q / (kR * kR), (q / kR) % kR, q % kR)] = S::u64(acc[q]);
+      for (int t = tid; t < kRN; t += blockDim.x) dst[t] = S::u64(acc[inv_slot[t]]);
     }
     const int32_t claimed = claim[0];
     __syncthreads();  // the next unit zeroes acc and rewrites claim[0]
@@ -820,16 +837,20 @@ __global__ void __launch_bounds__(256, 3) unit_deposit_kernel(
   __shared__ typename Sum<F64>::T acc[kRN];
   __shared__ float stage[4 * 4 * kStageStride];
   __shared__ int32_t claim[1];
+  // region index of every stored word (visible after the first unit's barrier)
+  __shared__ uint16_t inv_slot[kRN + 1];
+  for (int q = threadIdx.x; q < kRN; q += blockDim.x)
+    inv_slot[region_slot(q / (kR * kR), (q / kR) % kR, q % kR)] = static_cast<uint16_t>(q);
   const int s = pm::fixed_scale(*maxbits, bl);
   const int32_t nu = static_cast<int32_t>(*unit_count);
   if (s <= kFoldMax)
-    unit_loop<F64, IDX, true>(acc, stage, claim, units, work, nu, queue + parity, idx, px, py,
-                              pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc, eflag,
-                              esc_any + parity);
+    unit_loop<F64, IDX, true>(acc, stage, claim, inv_slot, units, work, nu, queue + parity, idx,
+                              px, py, pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc,
+                              eflag, esc_any + parity);
   else
-    unit_loop<F64, IDX, false>(acc, stage, claim, units, work, nu, queue + parity, idx, px, py,
-                               pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc, eflag,
-                               esc_any + parity);
+    unit_loop<F64, IDX, false>(acc, stage, claim, inv_slot, units, work, nu, queue + parity, idx,
+                               px, py, pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc,
+                               eflag, esc_any + parity);
 }
 
 // One workgroup: the order in which the unit pass claims units, work[0, nu): the units of
