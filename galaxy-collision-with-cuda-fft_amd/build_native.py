@@ -39,6 +39,14 @@ COMMON_FLAGS = [
 ]
 
 
+# Per-source additions. The bricked deposit's unit pass claims its next unit with one
+# lane's atomic whose result is only read at the unit's end; the atomic optimizer would
+# broadcast that result at once (a wait on every outstanding load, prefetches included).
+FILE_FLAGS = {
+    "pm_deposit_bricked.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"],
+}
+
+
 def _hipcc() -> str:
     exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not Path(exe).exists():
@@ -48,7 +56,7 @@ def _hipcc() -> str:
 
 def _compile(src: Path, verbose: bool) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [_hipcc(), *COMMON_FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [_hipcc(), *COMMON_FLAGS, *FILE_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)

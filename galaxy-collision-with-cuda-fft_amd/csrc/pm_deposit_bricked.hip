@@ -69,7 +69,8 @@ struct Unit {
   int32_t brick;  // cell brick, or -1 for particles with no brick at binning
   int32_t begin;  // first binned position
   int32_t count;  // positions
-  int32_t units;  // first unit of a brick: the brick's number of units; others 0
+  int32_t units;  // first unit of a brick: the brick's number of units; others 0 (in the
+                  // claim order: the unit's index)
 };
 
 __device__ __forceinline__ void decompose(uint32_t b, uint32_t nb, int& x, int& y, int& z) {
@@ -605,6 +606,8 @@ __device__ __forceinline__ int wave_span(int rem) {
   return rem >= kRoundPos ? kWavePos : (((rem + 3) >> 2) + 63) & ~63;
 }
 constexpr int kFoldMax = 64;              // largest scale exponent folded into m'
+constexpr int kUnitThreads = 256;         // threads of a unit pass workgroup
+typedef uint32_t U32x2 __attribute__((__vector_size__(8)));
 
 // The unit loop of unit_deposit_kernel. FOLD (s <= kFoldMax): the scale 2^s is applied to
 // m' once, M = m' * 2^s, and the terms ((M*wx)*wy)*wz are rounded directly. For s <= 64
@@ -616,7 +619,7 @@ template <bool F64, bool IDX, bool FOLD>
 __device__ __forceinline__ void unit_loop(
     typename Sum<F64>::T* __restrict__ acc, float* __restrict__ stage, int32_t* __restrict__ claim,
     const uint16_t* __restrict__ inv_slot,
-    const Unit* __restrict__ units, const int32_t* __restrict__ work, int32_t nu,
+    const Unit* __restrict__ work, int32_t nu,
     uint32_t* __restrict__ queue, const int32_t* __restrict__ idx, const float* __restrict__ px,
     const float* __restrict__ py, const float* __restrict__ pz, const float* __restrict__ mass,
     int64_t np, int n, float dx, float inv_vol, int nbc, int nbn, int s,
@@ -654,21 +657,29 @@ __device__ __forceinline__ void unit_loop(
   // Units in the work order: workgroup w takes positions w and w + G (G workgroups) of it,
   // then claims positions from 2G on through the queue, at each unit's start the one after
   // the next (the claim's result is only needed at the unit's end; claims spread over the
-  // launch instead of all workgroups meeting on one counter at its start). A unit's table
-  // entry is loaded one unit ahead, and a wave's first phase of a unit while it finishes the
-  // unit before, so none of these loads' latency adds to a unit's time.
+  // launch instead of all workgroups meeting on one counter at its start). A unit's record
+  // (one load: the work order holds the records) is loaded one unit ahead, and a wave's
+  // first phase of a unit while it finishes the unit before, so none of these loads'
+  // latency adds to a unit's time.
   const int32_t G = static_cast<int32_t>(gridDim.x);
   int32_t cur = static_cast<int32_t>(blockIdx.x), nxt = cur + G;
-  Unit un_next = cur < nu ? units[work[cur]] : Unit{};
+  Unit un_next = cur < nu ? work[cur] : Unit{};
   float4 ph[kPhaseRun];
   if (cur < nu) load(un_next, first_pos(un_next.count, 0), ph);
   while (cur < nu) {
-    const int32_t u = work[cur];
-    const Unit un = un_next;
+    // the record is the same in every lane: held in scalar registers for the unit (its
+    // load was issued a unit ago)
+    Unit un;
+    un.brick = __builtin_amdgcn_readfirstlane(un_next.brick);
+    un.begin = __builtin_amdgcn_readfirstlane(un_next.begin);
+    un.count = __builtin_amdgcn_readfirstlane(un_next.count);
+    un.units = __builtin_amdgcn_readfirstlane(un_next.units);
+    const int32_t u = un.units;
     const bool more = nxt < nu;
-    if (more) un_next = units[work[nxt]];
+    if (more) un_next = work[nxt];
     uint32_t after = 0u;
-    if (tid == 0) after = atomicAdd(queue, 1u) + 2u * static_cast<uint32_t>(G);
+    // (the claim's result is first used at the unit's end, so no wait for it here)
+    if (tid == 0) after = atomicAdd(queue, 1u);
     bool pref = false;  // ph holds the next unit's first phase
     const bool home = un.brick >= 0;
     int ox = INT_MIN / 2, oy = INT_MIN / 2, oz = INT_MIN / 2;  // region origin (nodes)
@@ -809,12 +820,36 @@ __device__ __forceinline__ void unit_loop(
       flush_sums(acc, k1, s1, k1 >= 0, lane);
     }
     if (more && !pref) load(un_next, first_pos(un_next.count, 0), ph);
-    if (tid == 0) claim[0] = static_cast<int32_t>(after);
+    if (tid == 0) claim[0] = static_cast<int32_t>(after + 2u * static_cast<uint32_t>(G));
     __syncthreads();
     if (home) {
-      // in stored order: consecutive lanes write consecutive words
-      unsigned long long* dst = partial + static_cast<int64_t>(u) * kRegion;
-      for (int t = tid; t < kRN; t += blockDim.x) dst[t] = S::u64(acc[inv_slot[t]]);
+      // in stored order (consecutive lanes write consecutive words) through a buffer
+      // resource over the unit's stored region: 32-bit offsets, every LDS read issued
+      // before the first store
+      const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+          partial + static_cast<int64_t>(u) * kRegion, static_cast<short>(0), kRegion * 8,
+          0x00020000);
+      constexpr int kOut = (kRN + kUnitThreads - 1) / kUnitThreads;
+      int qi[kOut];
+#pragma unroll
+      for (int k = 0; k < kOut; ++k) {
+        const int t = tid + k * kUnitThreads;
+        qi[k] = t < kRN ? inv_slot[t] : 0;
+      }
+      T val[kOut];
+#pragma unroll
+      for (int k = 0; k < kOut; ++k) val[k] = acc[qi[k]];
+#pragma unroll
+      for (int k = 0; k < kOut; ++k) {
+        const int t = tid + k * kUnitThreads;
+        if (t < kRN) {
+          const unsigned long long w = S::u64(val[k]);
+          U32x2 w2;
+          w2[0] = static_cast<uint32_t>(w);
+          w2[1] = static_cast<uint32_t>(w >> 32);
+          __builtin_amdgcn_raw_buffer_store_b64(w2, rp, t * 8, 0, 0);
+        }
+      }
     }
     const int32_t claimed = claim[0];
     __syncthreads();  // the next unit zeroes acc and rewrites claim[0]
@@ -826,9 +861,9 @@ __device__ __forceinline__ void unit_loop(
 // IDX: the particles are read through the binning's order idx (else in place). Workgroups
 // claim units in the order `work` from queue (zero at the launch) until all nu are taken.
 template <bool F64, bool IDX>
-__global__ void __launch_bounds__(256, 3) unit_deposit_kernel(
-    const Unit* __restrict__ units, const int32_t* __restrict__ work,
-    const uint32_t* __restrict__ unit_count, uint32_t* __restrict__ queue,
+__global__ void __launch_bounds__(kUnitThreads, 3) unit_deposit_kernel(
+    const Unit* __restrict__ work, const uint32_t* __restrict__ unit_count,
+    uint32_t* __restrict__ queue,
     const int32_t* __restrict__ idx, const float* __restrict__ px, const float* __restrict__ py,
     const float* __restrict__ pz, const float* __restrict__ mass, int64_t np, int n, float dx,
     float inv_vol, int nbc, int nbn, const unsigned int* __restrict__ maxbits, int bl,
@@ -844,33 +879,76 @@ __global__ void __launch_bounds__(256, 3) unit_deposit_kernel(
   const int s = pm::fixed_scale(*maxbits, bl);
   const int32_t nu = static_cast<int32_t>(*unit_count);
   if (s <= kFoldMax)
-    unit_loop<F64, IDX, true>(acc, stage, claim, inv_slot, units, work, nu, queue + parity, idx,
+    unit_loop<F64, IDX, true>(acc, stage, claim, inv_slot, work, nu, queue + parity, idx,
                               px, py, pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc,
                               eflag, esc_any + parity);
   else
-    unit_loop<F64, IDX, false>(acc, stage, claim, inv_slot, units, work, nu, queue + parity, idx,
+    unit_loop<F64, IDX, false>(acc, stage, claim, inv_slot, work, nu, queue + parity, idx,
                                px, py, pz, mass, np, n, dx, inv_vol, nbc, nbn, s, partial, esc,
                                eflag, esc_any + parity);
 }
 
-// One workgroup: the order in which the unit pass claims units, work[0, nu): the units of
-// more than kUnit/2 positions first (each takes about as long as a few dozen small ones),
-// then the others, each part in table order, so that the last claims are short.
+// One workgroup: the order in which the unit pass claims units, work[0, nu), as the unit
+// records with their index in the 4th field: by size class floor(log2(count)), larger
+// classes first (a unit takes a fixed few microseconds plus a time proportional to its
+// positions), each class in table order, so that the last claims are the shortest. The
+// order only balances the workgroups: every unit's sums are its own.
+constexpr int kOrderClasses = 14;  // floor(log2(count)) for counts up to kUnit = 2^13
+static_assert(kUnit == 1 << (kOrderClasses - 1), "one class per power of two up to kUnit");
+__device__ __forceinline__ int order_class(int32_t count) {
+  return count > 0 ? 31 - __builtin_clz(static_cast<uint32_t>(count)) : 0;
+}
 __global__ void __launch_bounds__(1024) unit_order_kernel(const Unit* __restrict__ units,
                                                           const uint32_t* __restrict__ unit_count,
-                                                          int32_t* __restrict__ work) {
-  __shared__ uint32_t s_wave[16];
+                                                          Unit* __restrict__ work) {
+  __shared__ uint32_t s_start[kOrderClasses];       // next free position of each class
+  __shared__ uint32_t s_wcnt[16][kOrderClasses];    // units of each class per wave (a chunk)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t nu = *unit_count;
-  const uint32_t span = (nu + 1023u) / 1024u;
-  const uint32_t u0 = min(threadIdx.x * span, nu), u1 = min(u0 + span, nu);
-  uint32_t big = 0;
-  for (uint32_t u = u0; u < u1; ++u) big += units[u].count > kUnit / 2 ? 1u : 0u;
-  uint32_t nbig;
-  uint32_t b = scan_1024(big, nbig, s_wave);
-  uint32_t small = nbig + (u0 - b);  // small units before u0 = units before u0 - big ones
-  for (uint32_t u = u0; u < u1; ++u) {
-    if (units[u].count > kUnit / 2) work[b++] = static_cast<int32_t>(u);
-    else work[small++] = static_cast<int32_t>(u);
+  if (tid < kOrderClasses) s_start[tid] = 0;
+  __syncthreads();
+  // the class sizes (order-free counts)
+  for (uint32_t u = tid; u < nu; u += blockDim.x)
+    atomicAdd(&s_start[order_class(units[u].count)], 1u);
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t start = 0;
+    for (int c = kOrderClasses - 1; c >= 0; --c) {
+      const uint32_t size = s_start[c];
+      s_start[c] = start;
+      start += size;
+    }
+  }
+  __syncthreads();
+  // chunks of 1024 units in table order: a unit's place in its class = the class's units of
+  // the earlier chunks, of the earlier waves of this chunk and of the lower lanes of its wave
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t c0 = 0; c0 < nu; c0 += blockDim.x) {
+    const uint32_t u = c0 + tid;
+    const bool valid = u < nu;
+    Unit w = valid ? units[u] : Unit{};
+    const int c = order_class(w.count);
+    uint32_t rank = 0;
+#pragma unroll
+    for (int k = 0; k < kOrderClasses; ++k) {
+      const uint64_t m = __ballot(valid && c == k);
+      if (c == k) rank = __popcll(m & below);
+      if (lane == 0) s_wcnt[wave][k] = __popcll(m);
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t at = s_start[c] + rank;
+      for (int v = 0; v < wave; ++v) at += s_wcnt[v][c];
+      w.units = static_cast<int32_t>(u);
+      work[at] = w;
+    }
+    __syncthreads();
+    if (tid < kOrderClasses) {
+      uint32_t add = 0;
+      for (int v = 0; v < 16; ++v) add += s_wcnt[v][tid];
+      s_start[tid] += add;
+    }
+    __syncthreads();
   }
 }
 
@@ -1060,7 +1138,7 @@ struct State {
   int2* desc = nullptr;            // 27 source descriptors per node brick with sources
   int32_t* multi = nullptr;        // first units of the bricks with several units
   uint32_t* multi_count = nullptr;
-  int32_t* work = nullptr;         // the order in which the unit pass claims units
+  Unit* work = nullptr;  // the units in the order the unit pass claims them (4th field: index)
   uint32_t* queue = nullptr;       // claim counters of the unit pass, by parity
   unsigned long long* partial = nullptr;
   unsigned long long* esc = nullptr;
@@ -1225,7 +1303,7 @@ pm_status deposit(State* st, const SortBuffers& sb, const float* px, const float
     const double bytes = (idx ? 20.0 : 16.0) * np;
 #define PM_UNIT_PASS(F64, IDX)                                                                 \
   PM_LAUNCH_TIMED("deposit.unit_sums", bytes, (unit_deposit_kernel<F64, IDX>), kUnitGroups, 256, \
-                  0, s, st->units, st->work, st->unit_count, st->queue, idx, px, py, pz, mass, \
+                  0, s, st->work, st->unit_count, st->queue, idx, px, py, pz, mass,            \
                   np, n, dx, inv_vol,                                                          \
                   st->nbc, st->nbn, st->maxbits, bl, st->partial, st->esc, st->eflag,          \
                   st->esc_any, st->parity)
