@@ -22,16 +22,40 @@ struct Cic {
   int ix, iy, iz;
 };
 
+// g = x / dx, the IEEE quotient, with the reciprocal of dx refined once per thread: the
+// steps of the compiler's float32 division (reciprocal, one Newton step, quotient and two
+// remainder corrections by fma) without its operand range scaling, which never applies
+// when dx lies in [2^-30, 2^30] and 2^-60 <= |x| < 2^60 (no operand, remainder or quotient
+// near the under- or overflow range), so the result is the IEEE quotient there. Larger |x|
+// give |g| >= 2^29, infinity or NaN, outside the grid like the IEEE quotient; for smaller
+// |x| (and dx outside that range) callers take the division itself.
+__device__ __forceinline__ float quot(float x, float dx, float r1) {
+  const float q0 = x * r1;
+  const float e1 = fmaf(-dx, q0, x);
+  const float q1 = fmaf(e1, r1, q0);
+  const float e2 = fmaf(-dx, q1, x);
+  return fmaf(e2, r1, q1);
+}
+
+// The refined reciprocal quot() takes, and whether dx lies in its range.
+__device__ __forceinline__ float quot_recip(float dx) {
+  const float r0 = __builtin_amdgcn_rcpf(dx);
+  return fmaf(fmaf(-dx, r0, 1.0f), r0, r0);
+}
+__device__ __forceinline__ bool quot_dx_ok(float dx) { return dx >= 0x1p-30f && dx <= 0x1p30f; }
+
 // Base cell and offsets of one coordinate. Returns false when no corner of the cell can
 // lie in [0, n) along this axis, i.e. unless -1 <= g < n (NaN also returns false); this
 // keeps the float->int conversion in range for any input.
-__device__ __forceinline__ bool cic_axis(float x, float dx, int n, int& i, float& f) {
-  const float g = x / dx;
+__device__ __forceinline__ bool cic_axis_g(float g, int n, int& i, float& f) {
   if (!(g >= -1.0f && g < static_cast<float>(n))) return false;
   const float fl = floorf(g);
   i = static_cast<int>(fl);
   f = g - static_cast<float>(i);
   return true;
+}
+__device__ __forceinline__ bool cic_axis(float x, float dx, int n, int& i, float& f) {
+  return cic_axis_g(x / dx, n, i, f);
 }
 
 // Extended-cell key: cells with base index -1..n-1 per axis shifted by +1, C order.

@@ -547,21 +547,6 @@ __device__ __noinline__ void add_outside(const unsigned long long (&q)[8], int i
   }
 }
 
-// g = x / dx, the IEEE quotient, with the reciprocal of dx refined once per thread: the
-// steps of the compiler's float32 division (reciprocal, one Newton step, quotient and two
-// remainder corrections by fma) without its operand range scaling, which never applies
-// when dx lies in [2^-30, 2^30] and 2^-60 <= |x| < 2^60 (no operand, remainder or quotient
-// near the under- or overflow range), so the result is the IEEE quotient there. Larger |x|
-// give |g| >= 2^29, infinity or NaN, outside the grid like the IEEE quotient; smaller |x|
-// (and dx outside that range) take the division itself (`slow`).
-__device__ __forceinline__ float quot(float x, float dx, float r1) {
-  const float q0 = x * r1;
-  const float e1 = fmaf(-dx, q0, x);
-  const float q1 = fmaf(e1, r1, q0);
-  const float e2 = fmaf(-dx, q1, x);
-  return fmaf(e2, r1, q1);
-}
-
 // Buffer resource over `count` 4-byte values at p (binned positions stay below 2^30, so
 // byte offsets fit 32 bits): loads through it take one 32-bit offset register and return 0
 // past the end instead of reading beyond it. Dword 3 is the gfx9 raw-buffer format.
@@ -629,9 +614,8 @@ __device__ __forceinline__ void unit_loop(
   using T = typename S::T;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const float nf = static_cast<float>(n);
-  const float r0 = __builtin_amdgcn_rcpf(dx);
-  const float r1 = fmaf(fmaf(-dx, r0, 1.0f), r0, r0);
-  const bool fast = dx >= 0x1p-30f && dx <= 0x1p30f;
+  const float r1 = pm::quot_recip(dx);
+  const bool fast = pm::quot_dx_ok(dx);
   const float scale = FOLD ? __builtin_amdgcn_ldexpf(1.0f, s) : 1.0f;  // 2^s, normal for s >= -97
   // a phase's particles of unit U as the coalesced loads see them (positions past the
   // unit's end read its last particle, unused)
@@ -745,7 +729,7 @@ __device__ __forceinline__ void unit_loop(
           for (int i = 0; i < kGroup; ++i) {
             const bool valid = g + i < run && run * lane + g + i < m;
             const float x = grp[i].x, y = grp[i].y, z = grp[i].z;
-            float gx = quot(x, dx, r1), gy = quot(y, dx, r1), gz = quot(z, dx, r1);
+            float gx = pm::quot(x, dx, r1), gy = pm::quot(y, dx, r1), gz = pm::quot(z, dx, r1);
             const bool slow = !fast || !(fabsf(x) >= 0x1p-60f) || !(fabsf(y) >= 0x1p-60f) ||
                               !(fabsf(z) >= 0x1p-60f);
             if (slow) {

@@ -207,26 +207,47 @@ __global__ void __launch_bounds__(256) grad4_listed_kernel(const float* __restri
 }
 
 // CIC interpolation from the interleaved field (same weights, order and roundings as
-// cic_gather, so bitwise equal).
+// cic_gather, so bitwise equal). g = x/dx by pm::quot where that is the IEEE quotient (dx
+// and every |coordinate| in its range), else by the division; the corner nodes are
+// addressed from the base node by 32-bit offsets (n^3 < 2^32, and the sum wraps back into
+// range for a base index of -1 along an axis).
 __device__ __forceinline__ void cic_gather4(float x, float y, float z,
                                             const float4* __restrict__ g4, int n, float dx,
                                             float& ax, float& ay, float& az) {
   ax = 0.0f;
   ay = 0.0f;
   az = 0.0f;
+  float gx, gy, gz;
+  if (pm::quot_dx_ok(dx) && fabsf(x) >= 0x1p-60f && fabsf(y) >= 0x1p-60f &&
+      fabsf(z) >= 0x1p-60f) {
+    const float r1 = pm::quot_recip(dx);
+    gx = pm::quot(x, dx, r1);
+    gy = pm::quot(y, dx, r1);
+    gz = pm::quot(z, dx, r1);
+  } else {
+    gx = x / dx;
+    gy = y / dx;
+    gz = z / dx;
+  }
   int ix, iy, iz;
   float fx, fy, fz;
-  if (!pm::cic_axis(x, dx, n, ix, fx) || !pm::cic_axis(y, dx, n, iy, fy) ||
-      !pm::cic_axis(z, dx, n, iz, fz))
+  if (!pm::cic_axis_g(gx, n, ix, fx) || !pm::cic_axis_g(gy, n, iy, fy) ||
+      !pm::cic_axis_g(gz, n, iz, fz))
     return;
-  const int64_t n2 = static_cast<int64_t>(n) * n;
+  // base index ix >= -1 and ix <= n - 1 per axis: corner 0 lies in the box along it when
+  // ix >= 0, corner 1 when ix + 1 < n
+  const uint32_t un = static_cast<uint32_t>(n);
+  const uint32_t base =
+      (static_cast<uint32_t>(ix) * un + static_cast<uint32_t>(iy)) * un + static_cast<uint32_t>(iz);
+  const bool okx[2] = {ix >= 0, ix + 1 < n}, oky[2] = {iy >= 0, iy + 1 < n},
+             okz[2] = {iz >= 0, iz + 1 < n};
   bool ok[8];
   float4 g[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const int jx = ix + (c >> 2), jy = iy + ((c >> 1) & 1), jz = iz + (c & 1);
-    ok[c] = jx >= 0 && jx < n && jy >= 0 && jy < n && jz >= 0 && jz < n;
-    g[c] = g4[ok[c] ? jx * n2 + static_cast<int64_t>(jy) * n + jz : 0];
+    ok[c] = okx[c >> 2] && oky[(c >> 1) & 1] && okz[c & 1];
+    const uint32_t off = (c >> 2) * un * un + ((c >> 1) & 1) * un + (c & 1);
+    g[c] = g4[ok[c] ? base + off : 0u];
   }
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
