@@ -4,8 +4,8 @@
   (M/dx^3), deterministic in every mode: "sorted" (cell-key sort + segmented reduction in a
   defined float32 order, also yields the keys and particle order; the default here),
   "fixed" (64-bit fixed-point integer sums with device-scope atomics, order independent) or
-  "bricked" (the "fixed" result bit for bit, summed per 8^3-cell brick in LDS; the fused
-  step's default, which reuses one binning over several steps).
+  "bricked" (the "fixed" result bit for bit, summed per 8^3-cell brick in LDS; the mode the
+  benchmark workloads give ``PMChain``, which reuses one binning over several steps).
   Reference deposit.py:177-182.
 - ``potential_to_accel(phi, ax, ay, az, dx)``: g = -grad(phi), central inside, one-sided on
   the faces. Reference deposit.py:204-207.
