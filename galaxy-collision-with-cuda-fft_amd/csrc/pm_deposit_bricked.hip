@@ -573,7 +573,7 @@ __device__ __forceinline__ float ldbf(__amdgpu_buffer_rsrc_t r, uint32_t index) 
 // registers over a round of kPhases phases; they go to the LDS sums at the end of the round,
 // and a slot taken over by a third cell goes there at once, those that many lanes share
 // summed over the wave first (flush_sums).
-constexpr int kPhaseRun = 8;                   // consecutive positions per lane and phase
+constexpr int kPhaseRun = 8;                   // consecutive positions per lane and phase, at most
 constexpr int kPhasePos = 64 * kPhaseRun;      // positions per wave and phase
 constexpr int kPhases = 2;                     // phases per round
 constexpr int kWavePos = kPhases * kPhasePos;  // positions per wave and round
