@@ -72,12 +72,17 @@ __device__ __forceinline__ void block_sum(double (&v)[Q], double* s_red) {
 }
 
 // v[q] += p[b*sb + q*sq] for b = threadIdx.x, threadIdx.x + blockDim.x, ... < blocks, in that
-// order (the stage-2 loop of every fold), with the loads of kFoldUnroll iterations issued
-// before they are added.
-constexpr int kFoldUnroll = 4;
+// order (the stage-2 loop of every fold), with the loads of fold_unroll<Q>() iterations
+// issued before they are added: a fold is a chain of memory round trips, one per batch
+// (16 iterations hold 32*Q registers: 16 batches up to 3 quantities, fewer beyond).
+template <int Q>
+constexpr int fold_unroll() {
+  return Q <= 3 ? 16 : (Q <= 6 ? 8 : 4);
+}
 template <int Q>
 __device__ __forceinline__ void fold_loop(const double* __restrict__ p, int blocks, int64_t sb,
                                           int64_t sq, double (&v)[Q]) {
+  constexpr int kFoldUnroll = fold_unroll<Q>();
   for (int b0 = threadIdx.x; b0 < blocks; b0 += kFoldUnroll * blockDim.x) {
     double t[kFoldUnroll][Q];
 #pragma unroll

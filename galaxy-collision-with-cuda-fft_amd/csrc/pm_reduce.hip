@@ -133,11 +133,7 @@ __global__ void __launch_bounds__(256) fold_moments_kernel(const double* __restr
   double v[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) v[q] = 0.0;
-  // a plain loop: with 11 quantities the unrolled fold_loop holds too many registers
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = v[q] + partials[q * blocks + b];
-  }
+  pm::fold_loop<Q>(partials, blocks, 1, blocks, v);  // the plain loop's order
   pm::block_sum<Q>(v, s_red);
   if (threadIdx.x == 0) {
 #pragma unroll
