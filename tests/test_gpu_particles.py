@@ -100,6 +100,40 @@ def test_gather_and_fused_gather_kick_bitwise(device, npart, n, dx):
         assert_bitwise(v2[:, a], O.kick(v1[:, a], ref[a], h))
 
 
+@pytest.mark.parametrize("dx", [2.0 ** -34, 0.3125, 2.0 ** 31])
+def test_gather4_quotient_ranges(device, dx):
+    """The interleaved gather forms x/dx from a refined reciprocal where that equals the IEEE
+    quotient (dx in [2^-30, 2^30], every |coordinate| >= 2^-60) and divides otherwise: both
+    sides of each bound, zeros, subnormals and non-finite coordinates, bitwise vs the
+    oracle's float32 division."""
+    announce(f"gather4 quotient ranges dx={dx}")
+    from galaxy_collision_amd import _native as nat
+
+    n = 16
+    rng = np.random.default_rng(7)
+    npart = 4096
+    pos = _particles(rng, npart, n, F32(dx))
+    special = np.array([0.0, -0.0, 1e-40, -1e-40, 1e-20, 2.0 ** -60, 2.0 ** -61, -(2.0 ** -60),
+                        2.0 ** 60, -(2.0 ** 61), np.inf, -np.inf, np.nan, 3.0e38], dtype=F32)
+    k = len(special)
+    for axis in range(3):
+        rows = slice(axis * k, (axis + 1) * k)
+        pos[rows, axis] = special
+    # a coordinate near 2^-60 on one axis while the others lie in the box
+    pos[3 * k:3 * k + 64, 0] = (np.arange(64, dtype=F32) - 32) * F32(2.0 ** -66)
+    grids = [rng.normal(size=(n, n, n)).astype(F32) for _ in range(3)]
+    with np.errstate(over="ignore", invalid="ignore"):  # 3e38/dx and the non-finite rows
+        ref = O.gather(pos, *grids, F32(dx))
+    g4 = torch.from_numpy(np.stack([*grids, np.zeros_like(grids[0])], -1)).to(device).contiguous()
+    p = [_t(pos[:, a], device) for a in range(3)]
+    vel = [torch.zeros(npart, device=device) for _ in range(3)]
+    acc = [torch.empty(npart, device=device) for _ in range(3)]
+    nat.call("pm_gather_kick4", *(nat.ptr(t) for t in p), nat.ptr(g4), *(nat.ptr(a) for a in acc),
+             *(nat.ptr(v) for v in vel), npart, n, float(dx), 0.0, nat.stream())
+    for a in range(3):
+        assert_bitwise(_np(acc[a]), ref[a])
+
+
 @pytest.mark.parametrize("mode,how", [("fixed", "step"), ("sorted", "step"), ("sorted", "run"),
                                       ("fixed", "run")])
 def test_fused_chain_step_equals_unfused_sequence(device, mode, how):
