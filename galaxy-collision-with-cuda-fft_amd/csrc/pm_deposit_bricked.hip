@@ -717,84 +717,91 @@ __device__ __forceinline__ void unit_loop(
           load(un_next, first_pos(un_next.count, 0), ph);
           pref = true;
         }
-#pragma unroll 1
-        for (int g = 0; g < run; g += kGroup) {
-          float4 grp[kGroup];
-#pragma unroll
-          for (int i = 0; i < kGroup; ++i) {
-            const int t = stage_slot(run * lane + g + i);  // run * lane + g + i < kPhasePos
-            grp[i] = make_float4(sx[t], sy[t], sz[t], sm[t]);
+        // one binned position of this lane (every lane of the wave calls it together)
+        auto body = [&](const float4& pp, bool valid) {
+          const float x = pp.x, y = pp.y, z = pp.z;
+          float gx = pm::quot(x, dx, r1), gy = pm::quot(y, dx, r1), gz = pm::quot(z, dx, r1);
+          const bool slow = !fast || !(fabsf(x) >= 0x1p-60f) || !(fabsf(y) >= 0x1p-60f) ||
+                            !(fabsf(z) >= 0x1p-60f);
+          if (slow) {
+            gx = x / dx;
+            gy = y / dx;
+            gz = z / dx;
           }
+          const bool in = valid && gx >= -1.0f && gx < nf && gy >= -1.0f && gy < nf &&
+                          gz >= -1.0f && gz < nf;
+          const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+          const int ix = in ? static_cast<int>(flx) : 0, iy = in ? static_cast<int>(fly) : 0,
+                    iz = in ? static_cast<int>(flz) : 0;
+          const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
+          const float mp = pp.w * inv_vol;
+          // m' <= 0 or not finite: every term is dropped (t <= 0 or not finite)
+          const bool good = in && mp > 0.0f && mp < __builtin_huge_valf();
+          const int rx = ix - ox, ry = iy - oy, rz = iz - oz;
+          const bool reg = good && home && static_cast<unsigned>(rx) <= 9u &&
+                           static_cast<unsigned>(ry) <= 9u && static_cast<unsigned>(rz) <= 9u;
+          const float gx0 = 1.0f - fx, gy0 = 1.0f - fy, gz0 = 1.0f - fz;
+          if (good && !reg) {
+            // t_c = ((m'*wx)*wy)*wz, the products shared between corners
+            const float a0 = mp * gx0, a1 = mp * fx;
+            const float bxy[4] = {a0 * gy0, a0 * fy, a1 * gy0, a1 * fy};
+            unsigned long long q[8];
 #pragma unroll
-          for (int i = 0; i < kGroup; ++i) {
-            const bool valid = g + i < run && run * lane + g + i < m;
-            const float x = grp[i].x, y = grp[i].y, z = grp[i].z;
-            float gx = pm::quot(x, dx, r1), gy = pm::quot(y, dx, r1), gz = pm::quot(z, dx, r1);
-            const bool slow = !fast || !(fabsf(x) >= 0x1p-60f) || !(fabsf(y) >= 0x1p-60f) ||
-                              !(fabsf(z) >= 0x1p-60f);
-            if (slow) {
-              gx = x / dx;
-              gy = y / dx;
-              gz = z / dx;
+            for (int c = 0; c < 8; ++c) q[c] = pm::fixed_q(bxy[c >> 1] * ((c & 1) ? fz : gz0), s);
+            add_outside(q, ix, iy, iz, n, nbn, esc, eflag, esc_flag);
+          }
+          // slot of the particle's cell: slot 0 when it holds that cell (or is free), else
+          // slot 1, whose cell, when another, first goes to the LDS sums; the lanes doing so
+          // at one step mostly share that cell, so it is summed over the wave (every lane
+          // takes part in flush_sums)
+          const int key = (rx * kR + ry) * kR + rz;
+          const bool h0 = reg && (key == k0 || k0 < 0);
+          const bool h1 = reg && !h0;
+          const bool evict = h1 && k1 >= 0 && key != k1;
+          if (__builtin_amdgcn_ballot_w64(evict) != 0ull) {
+            flush_sums(acc, k1, s1, evict, lane);
+            if (evict) {
+#pragma unroll
+              for (int c = 0; c < 8; ++c) s1[c] = T(0);
             }
-            const bool in = valid && gx >= -1.0f && gx < nf && gy >= -1.0f && gy < nf &&
-                            gz >= -1.0f && gz < nf;
-            const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
-            const int ix = in ? static_cast<int>(flx) : 0, iy = in ? static_cast<int>(fly) : 0,
-                      iz = in ? static_cast<int>(flz) : 0;
-            const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
-            const float mp = grp[i].w * inv_vol;
-            // m' <= 0 or not finite: every term is dropped (t <= 0 or not finite)
-            const bool good = in && mp > 0.0f && mp < __builtin_huge_valf();
-            const int rx = ix - ox, ry = iy - oy, rz = iz - oz;
-            const bool reg = good && home && static_cast<unsigned>(rx) <= 9u &&
-                             static_cast<unsigned>(ry) <= 9u && static_cast<unsigned>(rz) <= 9u;
-            const float gx0 = 1.0f - fx, gy0 = 1.0f - fy, gz0 = 1.0f - fz;
-            if (good && !reg) {
-              // t_c = ((m'*wx)*wy)*wz, the products shared between corners
-              const float a0 = mp * gx0, a1 = mp * fx;
-              const float bxy[4] = {a0 * gy0, a0 * fy, a1 * gy0, a1 * fy};
-              unsigned long long q[8];
+          }
+          k0 = h0 ? key : k0;
+          k1 = h1 ? key : k1;
+          if (reg) {
+            const float m2 = FOLD ? mp * scale : mp;
+            const float a0 = m2 * gx0, a1 = m2 * fx;
+            const float bxy[4] = {a0 * gy0, a0 * fy, a1 * gy0, a1 * fy};
+            T v[8];
 #pragma unroll
-              for (int c = 0; c < 8; ++c) q[c] = pm::fixed_q(bxy[c >> 1] * ((c & 1) ? fz : gz0), s);
-              add_outside(q, ix, iy, iz, n, nbn, esc, eflag, esc_flag);
+            for (int c = 0; c < 8; ++c) {
+              const float t = bxy[c >> 1] * ((c & 1) ? fz : gz0);
+              v[c] = FOLD ? S::folded(t) : S::scaled(t, s);
             }
-            // slot of the particle's cell: slot 0 when it holds that cell (or is free), else
-            // slot 1, whose cell, when another, first goes to the LDS sums; the lanes doing so
-            // at one step mostly share that cell, so it is summed over the wave (every lane
-            // takes part in flush_sums)
-            const int key = (rx * kR + ry) * kR + rz;
-            const bool h0 = reg && (key == k0 || k0 < 0);
-            const bool h1 = reg && !h0;
-            const bool evict = h1 && k1 >= 0 && key != k1;
-            if (__builtin_amdgcn_ballot_w64(evict) != 0ull) {
-              flush_sums(acc, k1, s1, evict, lane);
-              if (evict) {
+            if (h0) {
 #pragma unroll
-                for (int c = 0; c < 8; ++c) s1[c] = T(0);
-              }
+              for (int c = 0; c < 8; ++c) s0[c] = s0[c] + v[c];
             }
-            k0 = h0 ? key : k0;
-            k1 = h1 ? key : k1;
-            if (reg) {
-              const float m2 = FOLD ? mp * scale : mp;
-              const float a0 = m2 * gx0, a1 = m2 * fx;
-              const float bxy[4] = {a0 * gy0, a0 * fy, a1 * gy0, a1 * fy};
-              T v[8];
+            if (h1) {
 #pragma unroll
-              for (int c = 0; c < 8; ++c) {
-                const float t = bxy[c >> 1] * ((c & 1) ? fz : gz0);
-                v[c] = FOLD ? S::folded(t) : S::scaled(t, s);
-              }
-              if (h0) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) s0[c] = s0[c] + v[c];
-              }
-              if (h1) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) s1[c] = s1[c] + v[c];
-              }
+              for (int c = 0; c < 8; ++c) s1[c] = s1[c] + v[c];
             }
+          }
+        };
+        if (run == 1) {
+          // at most 64 positions: one per lane (a group of kGroup would compute three unused)
+          const int t = stage_slot(lane);
+          body(make_float4(sx[t], sy[t], sz[t], sm[t]), lane < m);
+        } else {
+#pragma unroll 1
+          for (int g = 0; g < run; g += kGroup) {
+            float4 grp[kGroup];
+#pragma unroll
+            for (int i = 0; i < kGroup; ++i) {
+              const int t = stage_slot(run * lane + g + i);  // run * lane + g + i < kPhasePos
+              grp[i] = make_float4(sx[t], sy[t], sz[t], sm[t]);
+            }
+#pragma unroll
+            for (int i = 0; i < kGroup; ++i) body(grp[i], g + i < run && run * lane + g + i < m);
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
